@@ -1,0 +1,190 @@
+"""ctypes binding of libhydro_hip.so (include/hydro_capi.h).
+
+The library is built in-tree by hydrodynamicsgpu_amd/build.py.  Loading fails
+loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import config
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhydro_hip.so")
+_lib = None
+
+
+class HydroError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HydroError("libhydro_hip.so is not built: run hydrodynamicsgpu_amd/build.py "
+                             "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, ip, sz = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_size_t
+        L.hydro_last_error.restype = ctypes.c_char_p
+        L.hydro_version.restype = ctypes.c_char_p
+        L.hydro_sim_desc_default.argtypes = [vp]
+        L.hydro_ctx_create.argtypes = [ip, ctypes.POINTER(vp)]
+        L.hydro_ctx_destroy.argtypes = [vp]
+        L.hydro_sim_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+        L.hydro_sim_destroy.argtypes = [vp]
+        L.hydro_sim_num_states.argtypes = [vp]
+        L.hydro_sim_volume.argtypes = [vp]
+        L.hydro_sim_volume.restype = sz
+        for n in ("hydro_sim_set_state", "hydro_sim_get_state", "hydro_sim_get_primitives"):
+            getattr(L, n).argtypes = [vp, dp, sz]
+        L.hydro_sim_boundary.argtypes = [vp]
+        L.hydro_sim_calc_timestep.argtypes = [vp, dp]
+        L.hydro_sim_step.argtypes = [vp, ctypes.c_double]
+        L.hydro_sim_update.argtypes = [vp, dp]
+        L.hydro_sim_run.argtypes = [vp, ip]
+        L.hydro_sim_sync.argtypes = [vp]
+        L.hydro_sim_time.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.hydro_sim_totals.argtypes = [vp, dp]
+        L.hydro_sim_health.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        L.hydro_sim_set_profiling.argtypes = [vp, ip]
+        L.hydro_sim_stage_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.hydro_sim_stream.argtypes = [vp]
+        L.hydro_sim_stream.restype = vp
+        L.hydro_batch_create.argtypes = [vp, vp, ip, ctypes.POINTER(vp)]
+        L.hydro_batch_destroy.argtypes = [vp]
+        L.hydro_batch_set_state.argtypes = [vp, ip, dp, sz]
+        L.hydro_batch_get_state.argtypes = [vp, ip, dp, sz]
+        L.hydro_batch_run.argtypes = [vp, ip, ip]
+        L.hydro_batch_sync.argtypes = [vp]
+        L.hydro_batch_member_dt.argtypes = [vp, ip, dp]
+        L.hydro_batch_interior_cells.argtypes = [vp]
+        L.hydro_batch_interior_cells.restype = ctypes.c_longlong
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise HydroError("hydro error %d: %s" % (rc, lib().hydro_last_error().decode()))
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+_ctx = {}
+
+
+def context(device=0):
+    """Process-wide context per device."""
+    if device not in _ctx:
+        h = ctypes.c_void_p()
+        check(lib().hydro_ctx_create(device, ctypes.byref(h)))
+        _ctx[device] = h
+    return _ctx[device]
+
+
+class Sim:
+    """One grid on the GPU through the C ABI."""
+
+    def __init__(self, desc, device=0):
+        self.desc = desc
+        h = ctypes.c_void_p()
+        check(lib().hydro_sim_create(context(device), ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+        self.N = lib().hydro_sim_num_states(h)
+        self.volume = lib().hydro_sim_volume(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hydro_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, aos):
+        a = np.ascontiguousarray(aos, dtype=np.float64)
+        check(lib().hydro_sim_set_state(self._h, _dp(a), a.size))
+
+    def get_state(self):
+        out = np.empty(self.N * self.volume)
+        check(lib().hydro_sim_get_state(self._h, _dp(out), out.size))
+        return out
+
+    def get_primitives(self):
+        out = np.empty(self.N * self.volume)
+        check(lib().hydro_sim_get_primitives(self._h, _dp(out), out.size))
+        return out
+
+    def boundary(self):
+        check(lib().hydro_sim_boundary(self._h))
+
+    def calc_timestep(self):
+        dt = ctypes.c_double()
+        check(lib().hydro_sim_calc_timestep(self._h, ctypes.byref(dt)))
+        return dt.value
+
+    def step(self, dt):
+        check(lib().hydro_sim_step(self._h, float(dt)))
+
+    def update(self):
+        dt = ctypes.c_double()
+        check(lib().hydro_sim_update(self._h, ctypes.byref(dt)))
+        return dt.value
+
+    def run(self, nsteps):
+        check(lib().hydro_sim_run(self._h, int(nsteps)))
+
+    def sync(self):
+        check(lib().hydro_sim_sync(self._h))
+
+    def time(self):
+        t = ctypes.c_double()
+        n = ctypes.c_longlong()
+        dt = ctypes.c_double()
+        check(lib().hydro_sim_time(self._h, ctypes.byref(t), ctypes.byref(n), ctypes.byref(dt)))
+        return t.value, n.value, dt.value
+
+    def totals(self):
+        out = np.empty(self.N)
+        check(lib().hydro_sim_totals(self._h, _dp(out)))
+        return out
+
+    def health(self):
+        f = ctypes.c_uint()
+        check(lib().hydro_sim_health(self._h, ctypes.byref(f)))
+        return f.value
+
+    def set_profiling(self, enable):
+        check(lib().hydro_sim_set_profiling(self._h, 1 if enable else 0))
+
+    def stage_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        check(lib().hydro_sim_stage_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    @property
+    def stream(self):
+        return lib().hydro_sim_stream(self._h)
+
+
+EXPORTED_SYMBOLS = [
+    "hydro_sim_desc_default", "hydro_last_error", "hydro_version", "hydro_ctx_create",
+    "hydro_ctx_destroy", "hydro_sim_create", "hydro_sim_destroy", "hydro_sim_num_states",
+    "hydro_sim_volume", "hydro_sim_set_state", "hydro_sim_get_state", "hydro_sim_get_primitives",
+    "hydro_sim_boundary", "hydro_sim_calc_timestep", "hydro_sim_step", "hydro_sim_update",
+    "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
+    "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_batch_create", "hydro_batch_destroy", "hydro_batch_set_state",
+    "hydro_batch_get_state", "hydro_batch_run", "hydro_batch_sync", "hydro_batch_member_dt",
+    "hydro_batch_interior_cells",
+]
+
+__all__ = ["Sim", "HydroError", "lib", "check", "context", "config", "EXPORTED_SYMBOLS", "LIB_PATH"]

@@ -1,0 +1,428 @@
+// aux_kernels.hip -- ghost fill, CFL reduction, HLL wave speeds, totals.
+#include "device_common.hpp"
+#include "phys_euler.hpp"
+#include "launch.hpp"
+
+namespace hydro {
+
+// ---------------------------------------------------------------- boundary
+// res/Common.cl:49-194: per line along `axis`, per state, min side then max
+// side.  kind: 0 periodic, 1 mirror, 2 reflect (mirror with negation),
+// 3 freeflow (chained copy), -1 none.
+template <typename S>
+HYDRO_DEV void ghost_line(S* p, long long base, long long st, int n, int kind, int minmax, bool neg_ok) {
+  (void)neg_ok;
+#define G(i) p[base + st * (long long)(i)]
+  if (kind == 0) {
+    if (!minmax) { G(0) = G(n - 4); G(1) = G(n - 3); }
+    else { G(n - 2) = G(2); G(n - 1) = G(3); }
+  } else if (kind == 1) {
+    if (!minmax) { G(0) = G(3); G(1) = G(2); }
+    else { G(n - 1) = G(n - 4); G(n - 2) = G(n - 3); }
+  } else if (kind == 2) {
+    if (!minmax) { G(0) = -G(3); G(1) = -G(2); }
+    else { G(n - 1) = -G(n - 4); G(n - 2) = -G(n - 3); }
+  } else if (kind == 3) {
+    if (!minmax) { G(1) = G(2); G(0) = G(1); }
+    else { G(n - 2) = G(n - 3); G(n - 1) = G(n - 2); }
+  }
+#undef G
+}
+
+template <typename S>
+__global__ void boundary_kernel(BoundaryArgs b, S* planes, float* lo) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  long long base, st;
+  int n;
+  if (b.axis == 0) {
+    if (l >= b.sy * b.sz) return;
+    base = (long long)b.sx * l;  // (0, y, z)
+    st = 1;
+    n = b.sx;
+  } else if (b.axis == 1) {
+    if (l >= b.sx * b.sz) return;
+    const int x = l % b.sx, z = l / b.sx;
+    base = x + (long long)b.sx * b.sy * z;
+    st = b.sx;
+    n = b.sy;
+  } else {
+    if (l >= b.sx * b.sy) return;
+    base = l;
+    st = (long long)b.sx * b.sy;
+    n = b.sz;
+  }
+  for (int s = 0; s < b.nstates; ++s) {
+    for (int mm = 0; mm < 2; ++mm) {
+      const int kind = b.kind[s][mm];
+      if (kind < 0) continue;
+      ghost_line<S>(planes + s * b.plane, base, st, n, kind, mm, true);
+      if (lo) ghost_line<float>(lo + s * b.plane, base, st, n, kind, mm, true);
+    }
+  }
+}
+
+void launch_boundary_f64(const BoundaryArgs& b, double* planes, hipStream_t st) {
+  const int lines = b.axis == 0 ? b.sy * b.sz : (b.axis == 1 ? b.sx * b.sz : b.sx * b.sy);
+  hipLaunchKernelGGL((boundary_kernel<double>), dim3((lines + 255) / 256), dim3(256), 0, st, b, planes,
+                     (float*)nullptr);
+}
+void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, hipStream_t st) {
+  const int lines = b.axis == 0 ? b.sy * b.sz : (b.axis == 1 ? b.sx * b.sz : b.sx * b.sy);
+  hipLaunchKernelGGL((boundary_kernel<float>), dim3((lines + 255) / 256), dim3(256), 0, st, b, planes, lo);
+}
+
+// ---------------------------------------------------------------- ghosts
+// Enumerates exactly the ghost cells (some coordinate in {0,1,S-2,S-1}).
+HYDRO_DEV bool ghost_cell(int g, int dim, int sx, int sy, int sz, long long& cell) {
+  auto pick = [](int k, int n) { return k < 2 ? k : n - 4 + k; };  // k in 0..3
+  if (dim == 1) {
+    if (g >= 4) return false;
+    cell = pick(g, sx);
+    return true;
+  }
+  if (dim == 2) {
+    const int rows = 4 * sx;
+    if (g < rows) {
+      const int y = pick(g / sx, sy), x = g % sx;
+      cell = (long long)y * sx + x;
+      return true;
+    }
+    g -= rows;
+    if (g >= 4 * (sy - 4)) return false;
+    const int y = 2 + g / 4, x = pick(g % 4, sx);
+    cell = (long long)y * sx + x;
+    return true;
+  }
+  const long long planes = 4LL * sx * sy;
+  if (g < planes) {
+    const int z = pick((int)(g / ((long long)sx * sy)), sz);
+    const long long r = g % ((long long)sx * sy);
+    cell = (long long)z * sx * sy + r;
+    return true;
+  }
+  long long h = g - planes;
+  const long long rows = 4LL * sx * (sz - 4);
+  if (h < rows) {
+    const int z = 2 + (int)(h / (4LL * sx));
+    const long long r = h % (4LL * sx);
+    const int y = pick((int)(r / sx), sy), x = (int)(r % sx);
+    cell = ((long long)z * sy + y) * sx + x;
+    return true;
+  }
+  h -= rows;
+  if (h >= 4LL * (sy - 4) * (sz - 4)) return false;
+  const int z = 2 + (int)(h / (4LL * (sy - 4)));
+  const long long r = h % (4LL * (sy - 4));
+  const int y = 2 + (int)(r / 4), x = pick((int)(r % 4), sx);
+  cell = ((long long)z * sy + y) * sx + x;
+  return true;
+}
+
+HYDRO_DEV long long ghost_count(int dim, int sx, int sy, int sz) {
+  if (dim == 1) return 4;
+  if (dim == 2) return 4LL * sx + 4LL * (sy - 4);
+  return 4LL * sx * sy + 4LL * sx * (sz - 4) + 4LL * (sy - 4) * (sz - 4);
+}
+
+template <typename S>
+__global__ void copy_ghosts_kernel(int dim, int sx, int sy, int sz, long long plane, int n, const S* src,
+                                   S* dst) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long cell;
+  if (g >= ghost_count(dim, sx, sy, sz)) return;
+  if (!ghost_cell((int)g, dim, sx, sy, sz, cell)) return;
+  for (int s = 0; s < n; ++s) dst[s * plane + cell] = src[s * plane + cell];
+}
+
+static long long host_ghost_count(int dim, int sx, int sy, int sz) {
+  if (dim == 1) return 4;
+  if (dim == 2) return 4LL * sx + 4LL * (sy - 4);
+  return 4LL * sx * sy + 4LL * sx * (sz - 4) + 4LL * (sy - 4) * (sz - 4);
+}
+
+void launch_copy_ghosts_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* src,
+                            double* dst, hipStream_t st) {
+  const long long g = host_ghost_count(dim, sx, sy, sz);
+  hipLaunchKernelGGL((copy_ghosts_kernel<double>), dim3((unsigned)((g + 255) / 256)), dim3(256), 0, st, dim,
+                     sx, sy, sz, plane, n, src, dst);
+}
+void launch_copy_ghosts_f32(int dim, int sx, int sy, int sz, long long plane, int n, const float* src,
+                            float* dst, hipStream_t st) {
+  const long long g = host_ghost_count(dim, sx, sy, sz);
+  hipLaunchKernelGGL((copy_ghosts_kernel<float>), dim3((unsigned)((g + 255) / 256)), dim3(256), 0, st, dim,
+                     sx, sy, sz, plane, n, src, dst);
+}
+
+// ---------------------------------------------------------------- CFL
+HYDRO_DEV double block_min(double v, double* sh) {
+  // wave min via shuffles, then across the block's waves through LDS
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int i = 1; i < nw; ++i) v = fmin(v, sh[i]);
+  }
+  return v;
+}
+
+template <typename S, int DIM>
+HYDRO_DEV typename EulerRoe<DIM, S>::Cell load_cell(const S* u, long long plane, long long cell, S gamma) {
+  S q[DIM + 2];
+#pragma unroll
+  for (int s = 0; s < DIM + 2; ++s) q[s] = u[s * plane + cell];
+  return EulerRoe<DIM, S>::template cell<S>(q, gamma);
+}
+
+// Roe: res/Roe.cl:4-65 on the frozen basis (lambda_max of the left face,
+// lambda_min of the right face, |.| + 1e-9), interior cells only.
+// HLL/HLLC: res/EulerHLL.cl:142-175 from the stored speeds, cells 2..S-2.
+template <typename S, int DIM>
+__global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0, const S* speeds) {
+  __shared__ double sh[4];
+  const int sx = a.sx, sy = a.sy, sz = a.sz;
+  const int hi = (a.solver == 0) ? 3 : 2;  // cells 2 .. S-hi
+  const int nx = sx - 2 - hi + 1;
+  const int ny = DIM > 1 ? sy - 2 - hi + 1 : 1;
+  const int nz = DIM > 2 ? sz - 2 - hi + 1 : 1;
+  const long long total = (long long)nx * ny * nz;
+  const S gamma = S(a.gamma);
+  const long long step[3] = {1, (long long)sx, (long long)sx * sy};
+  double m = INFINITY;
+  bool bad = false;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = 2 + (int)(i % nx);
+    const int y = DIM > 1 ? 2 + (int)((i / nx) % ny) : 0;
+    const int z = DIM > 2 ? 2 + (int)(i / ((long long)nx * ny)) : 0;
+    const long long cell = x + (long long)sx * (y + (long long)sy * z);
+    double r = INFINITY;
+    if (a.solver == 0) {
+      const auto cc = load_cell<S, DIM>(u0, a.plane, cell, gamma);
+#pragma unroll
+      for (int side = 0; side < DIM; ++side) {
+        const auto cm = load_cell<S, DIM>(u0, a.plane, cell - step[side], gamma);
+        const auto cp = load_cell<S, DIM>(u0, a.plane, cell + step[side], gamma);
+        const auto fL = EulerRoe<DIM, S>::face(cm, cc, side, gamma);
+        const auto fR = EulerRoe<DIM, S>::face(cc, cp, side, gamma);
+        const double lmax = fmax(0.0, double(fL.v[0] + fL.c));
+        const double lmin = fmin(0.0, double(fR.v[0] - fR.c));
+        const double d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
+        bad |= !(d == d);
+        r = fmin(r, d);
+      }
+    } else {
+      const int ci[3] = {x, y, z};
+      const int cs[3] = {sx, sy, sz};
+#pragma unroll
+      for (int side = 0; side < DIM; ++side) {
+        // speeds of the face at this cell (left) and at the next cell (right);
+        // faces outside 2..S-2 were never written and read as 0
+        const double lmaxL = double(speeds[(2 * side + 1) * a.plane + cell]);
+        const long long nxt = cell + step[side];
+        const double lminR = (ci[side] + 1 <= cs[side] - 2) ? double(speeds[(2 * side) * a.plane + nxt]) : 0.0;
+        const double minL = fmin(0.0, lminR);
+        const double maxL = fmax(0.0, lmaxL);
+        const double d = a.kdx[side] / (maxL - minL);
+        bad |= !(d == d);
+        r = fmin(r, d);
+      }
+    }
+    m = fmin(m, r);
+  }
+  m = block_min(m, sh);
+  // any NaN in the block is reported as NaN for the finaliser's health check
+  __shared__ int anybad;
+  if (threadIdx.x == 0) anybad = 0;
+  __syncthreads();
+  if (bad) atomicOr(&anybad, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) a.blockmin[blockIdx.x] = anybad ? NAN : m;
+}
+
+void launch_dt_euler_f64(const DtArgs& a, const double* u0, const double* speeds, hipStream_t st) {
+  dim3 g(a.nblocks), b(256);
+  if (a.dim == 1) hipLaunchKernelGGL((dt_kernel<double, 1>), g, b, 0, st, a, u0, speeds);
+  else if (a.dim == 2) hipLaunchKernelGGL((dt_kernel<double, 2>), g, b, 0, st, a, u0, speeds);
+  else hipLaunchKernelGGL((dt_kernel<double, 3>), g, b, 0, st, a, u0, speeds);
+}
+void launch_dt_euler_f32(const DtArgs& a, const float* u0, const float* speeds, hipStream_t st) {
+  dim3 g(a.nblocks), b(256);
+  if (a.dim == 1) hipLaunchKernelGGL((dt_kernel<float, 1>), g, b, 0, st, a, u0, speeds);
+  else if (a.dim == 2) hipLaunchKernelGGL((dt_kernel<float, 2>), g, b, 0, st, a, u0, speeds);
+  else hipLaunchKernelGGL((dt_kernel<float, 3>), g, b, 0, st, a, u0, speeds);
+}
+
+// Solver::findMinTimestep tail (src/Solver/Solver.cpp:510-515): dt = min * cfl;
+// plus fixed dt, clipping to t_end, time/step bookkeeping and health flags.
+__global__ void __launch_bounds__(256) finalize_dt_kernel(FinalizeArgs f) {
+  __shared__ double sh[4];
+  double m = INFINITY;
+  bool nan = false;
+  for (int i = threadIdx.x; i < f.nblocks; i += blockDim.x) {
+    const double v = f.blockmin[i];
+    nan |= !(v == v);
+    m = fmin(m, v);
+  }
+  __shared__ int anynan;
+  if (threadIdx.x == 0) anynan = 0;
+  __syncthreads();
+  if (nan) atomicOr(&anynan, 1);
+  m = block_min(m, sh);
+  if (threadIdx.x == 0) {
+    double dt = f.use_fixed ? f.fixed_dt : m * f.cfl;
+    const double t = *f.t;
+    if (f.clip && t + dt > f.t_end) dt = f.t_end - t;
+    unsigned h = *f.health;
+    if (anynan || !(dt > 0.0) || dt == INFINITY) h |= 1u;
+    *f.health = h;
+    *f.dt = dt;
+    *f.t = t + dt;
+    *f.steps = *f.steps + 1;
+  }
+}
+
+void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st) {
+  hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(256), 0, st, f);
+}
+
+// ---------------------------------------------------------------- HLL speeds
+template <typename S, int DIM>
+HYDRO_DEV void hll_face_speeds(const S* u0, long long plane, long long cl, long long cr, int side, S gamma,
+                               int solver, S& sl, S& sr) {
+  constexpr int N = DIM + 2, E = DIM + 1;
+  S qL[N], qR[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) {
+    qL[s] = u0[s * plane + cl];
+    qR[s] = u0[s * plane + cr];
+  }
+  // rotate into x (res/EulerHLL.cl:34-49)
+  if (side != 0) {
+    S t = qL[1]; qL[1] = qL[1 + side]; qL[1 + side] = t;
+    t = qR[1]; qR[1] = qR[1 + side]; qR[1 + side] = t;
+  }
+  S vL[DIM], vR[DIM], vsqL = S(0), vsqR = S(0);
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    vL[a] = tdiv(qL[1 + a], qL[0]);
+    vR[a] = tdiv(qR[1 + a], qR[0]);
+    vsqL += vL[a] * vL[a];
+    vsqR += vR[a] * vR[a];
+  }
+  const S g1 = gamma - S(1);
+  const S eTL = qL[E] * trcp(qL[0]), eTR = qR[E] * trcp(qR[0]);
+  const S pL = g1 * qL[0] * (eTL - S(.5) * vsqL);
+  const S pR = g1 * qR[0] * (eTR - S(.5) * vsqR);
+  const S hL = eTL + pL * trcp(qL[0]);
+  const S hR = eTR + pR * trcp(qR[0]);
+  const S csL = tsqrt(g1 * (hL - S(.5) * vsqL));
+  const S csR = tsqrt(g1 * (hR - S(.5) * vsqR));
+  const S wL = tsqrt(qL[0]), wR = tsqrt(qR[0]);
+  const S norm = trcp(wL + wR);
+  if (solver == 1) {
+    S v[DIM], vsq = S(0);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      v[a] = (wL * vL[a] + wR * vR[a]) * norm;
+      vsq += v[a] * v[a];
+    }
+    const S H = (wL * hL + wR * hR) * norm;
+    const S c = tsqrt(g1 * (H - S(.5) * vsq));
+    sl = tmin(vL[0] - csL, v[0] - c);
+    sr = tmax(vR[0] + csR, v[0] + c);
+  } else {
+    // res/EulerHLLC.cl:85-97 (PVRS pressure, Toro's shock factors)
+    const S ps = tmax(S(0), S(.5) * (pL + pR) - S(.5) * (vR[0] - vL[0]) * S(.5) * (qL[0] + qR[0]) * S(.5) * (csL + csR));
+    const S k = (gamma + S(1)) / (S(2) * gamma);
+    const S fqL = ps <= pL ? S(1) : tsqrt(S(1) + k * (tdiv(ps, pL) - S(1)));
+    const S fqR = ps <= pR ? S(1) : tsqrt(S(1) + k * (tdiv(ps, pR) - S(1)));
+    sl = vL[0] - csL * fqL;
+    sr = vR[0] + csR * fqR;
+  }
+}
+
+template <typename S, int DIM>
+__global__ void __launch_bounds__(256) hll_speeds_kernel(int solver, int sx, int sy, int sz, long long plane,
+                                                         double gamma, const S* u0, S* speeds) {
+  // faces with every coordinate in 2..S-2 (res/EulerHLL.cl:122-130)
+  const int nx = sx - 3, ny = DIM > 1 ? sy - 3 : 1, nz = DIM > 2 ? sz - 3 : 1;
+  const long long total = (long long)nx * ny * nz;
+  const long long step[3] = {1, (long long)sx, (long long)sx * sy};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = 2 + (int)(i % nx);
+    const int y = DIM > 1 ? 2 + (int)((i / nx) % ny) : 0;
+    const int z = DIM > 2 ? 2 + (int)(i / ((long long)nx * ny)) : 0;
+    const long long cell = x + (long long)sx * (y + (long long)sy * z);
+#pragma unroll
+    for (int side = 0; side < DIM; ++side) {
+      S sl, sr;
+      hll_face_speeds<S, DIM>(u0, plane, cell - step[side], cell, side, S(gamma), solver, sl, sr);
+      speeds[(2 * side) * plane + cell] = sl;
+      speeds[(2 * side + 1) * plane + cell] = sr;
+    }
+  }
+}
+
+template <typename S>
+static void hll_speeds(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma, const S* u0,
+                       S* speeds, hipStream_t st) {
+  dim3 g(2048), b(256);
+  if (dim == 1) hipLaunchKernelGGL((hll_speeds_kernel<S, 1>), g, b, 0, st, solver, sx, sy, sz, plane, gamma, u0, speeds);
+  else if (dim == 2) hipLaunchKernelGGL((hll_speeds_kernel<S, 2>), g, b, 0, st, solver, sx, sy, sz, plane, gamma, u0, speeds);
+  else hipLaunchKernelGGL((hll_speeds_kernel<S, 3>), g, b, 0, st, solver, sx, sy, sz, plane, gamma, u0, speeds);
+}
+void launch_hll_speeds_f64(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
+                           const double* u0, double* speeds, hipStream_t st) {
+  hll_speeds<double>(dim, solver, sx, sy, sz, plane, gamma, u0, speeds, st);
+}
+void launch_hll_speeds_f32(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
+                           const float* u0, float* speeds, hipStream_t st) {
+  hll_speeds<float>(dim, solver, sx, sy, sz, plane, gamma, u0, speeds, st);
+}
+
+// ---------------------------------------------------------------- totals
+template <typename S>
+__global__ void __launch_bounds__(256) totals_kernel(int dim, int sx, int sy, int sz, long long plane, int n,
+                                                     const S* u, const float* lo, double* partial) {
+  __shared__ double sh[256];
+  const int nx = sx - 4, ny = dim > 1 ? sy - 4 : 1, nz = dim > 2 ? sz - 4 : 1;
+  const long long total = (long long)nx * ny * nz;
+  for (int s = 0; s < n; ++s) {
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+      const int x = 2 + (int)(i % nx);
+      const int y = dim > 1 ? 2 + (int)((i / nx) % ny) : 0;
+      const int z = dim > 2 ? 2 + (int)(i / ((long long)nx * ny)) : 0;
+      const long long cell = x + (long long)sx * (y + (long long)sy * z);
+      double v = double(u[s * plane + cell]);
+      if (lo) v += double(lo[s * plane + cell]);
+      acc += v;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long long)blockIdx.x * n + s] = sh[0];
+    __syncthreads();
+  }
+}
+
+void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u, double* partial,
+                       int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((totals_kernel<double>), dim3(nblocks), dim3(256), 0, st, dim, sx, sy, sz, plane, n, u,
+                     (const float*)nullptr, partial);
+}
+void launch_totals_f32(int dim, int sx, int sy, int sz, long long plane, int n, const float* u, const float* lo,
+                       double* partial, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((totals_kernel<float>), dim3(nblocks), dim3(256), 0, st, dim, sx, sy, sz, plane, n, u, lo,
+                     partial);
+}
+
+}  // namespace hydro

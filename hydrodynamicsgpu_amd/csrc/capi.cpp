@@ -1,0 +1,749 @@
+// capi.cpp -- C ABI of the MI355X update path (include/hydro_capi.h).
+//
+// Owns the device buffers of one grid and sequences the kernels of a step:
+//   boundary (per axis) -> [HLL: frozen wave speeds] -> CFL per-block minima ->
+//   finalise dt on the device -> S fused RK stages -> ghost copy.
+// dt never leaves the device on the run() path; the only host sync is the
+// one a caller asks for (update() returns dt, get_state, totals).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hydro_capi.h"
+#include "launch.hpp"
+#include "stage_args.hpp"
+
+using namespace hydro;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) return fail(HYDRO_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// include/HydroGPU/Integrator/RungeKutta.h:129-294 (alphas(i,j), betas(i,j))
+struct Tableau {
+  int order = 0;
+  double a[4][4] = {};
+  double b[4][4] = {};
+};
+
+bool get_tableau(int id, Tableau& t) {
+  t = Tableau();
+  auto A = [&](int i, int j, double v) { t.a[i][j] = v; };
+  auto B = [&](int i, int j, double v) { t.b[i][j] = v; };
+  switch (id) {
+  case HYDRO_INTEGRATOR_FORWARDEULER: t.order = 1; A(0, 0, 1); B(0, 0, 1); return true;
+  case HYDRO_INTEGRATOR_RK2: t.order = 2; A(0, 0, 1); A(1, 0, 1); B(0, 0, .5); B(1, 1, 1); return true;
+  case HYDRO_INTEGRATOR_RK2HEUN:
+    t.order = 2; A(0, 0, 1); A(1, 0, 1); B(0, 0, 1); B(1, 0, .5); B(1, 1, .5); return true;
+  case HYDRO_INTEGRATOR_RK2RALSTON:
+    t.order = 2; A(0, 0, 1); A(1, 0, 1); B(0, 0, 2. / 3.); B(1, 0, 1. / 4.); B(1, 1, 3. / 4.); return true;
+  case HYDRO_INTEGRATOR_RK3:
+    t.order = 3; A(0, 0, 1); A(1, 0, 1); A(2, 0, 1);
+    B(0, 0, .5); B(1, 0, -1); B(1, 1, 2); B(2, 0, 1. / 6.); B(2, 1, 2. / 6.); B(2, 2, 1. / 6.); return true;
+  case HYDRO_INTEGRATOR_RK4:
+    t.order = 4; A(0, 0, 1); A(1, 0, 1); A(2, 0, 1); A(3, 0, 1);
+    B(0, 0, .5); B(1, 1, .5); B(2, 2, 1); B(3, 0, 1. / 6.); B(3, 1, 2. / 6.); B(3, 2, 2. / 6.); B(3, 3, 1. / 6.);
+    return true;
+  case HYDRO_INTEGRATOR_RK4_3_8THSRULE:
+    t.order = 4; A(0, 0, 1); A(1, 0, 1); A(2, 0, 1); A(3, 0, 1);
+    B(0, 0, 1. / 3.); B(1, 0, -1. / 3.); B(2, 0, 1); B(2, 1, -1); B(2, 2, 1);
+    B(3, 0, 1. / 8.); B(3, 1, 3. / 8.); B(3, 2, 3. / 8.); B(3, 3, 1. / 8.); return true;
+  case HYDRO_INTEGRATOR_RK2TVD:
+    t.order = 2; A(0, 0, 1); A(1, 0, .5); A(1, 1, .5); B(0, 0, 1); B(1, 1, .5); return true;
+  case HYDRO_INTEGRATOR_RK2NONTVD:
+    t.order = 2; A(0, 0, 1); A(1, 0, 1); B(0, 0, -20); B(1, 0, 41. / 40.); B(1, 1, -1. / 40.); return true;
+  case HYDRO_INTEGRATOR_RK3TVD:
+    t.order = 3; A(0, 0, 1); A(1, 0, 3. / 4.); A(1, 1, 1. / 4.); A(2, 0, 1. / 3.); A(2, 1, 2. / 3.);
+    B(0, 0, 1); B(1, 1, 1. / 4.); B(2, 2, 2. / 3.); return true;
+  case HYDRO_INTEGRATOR_RK4TVD:
+    t.order = 4; A(0, 0, 1); A(1, 0, 649. / 1600.); A(1, 1, 951. / 1600.);
+    A(2, 0, 53989. / 2500000.); A(2, 1, 4806213. / 20000000.); A(2, 2, 23619. / 32000.);
+    A(3, 0, 1. / 5.); A(3, 1, 6127. / 30000.); A(3, 2, 7873. / 30000.); A(3, 3, 1. / 3.);
+    B(0, 0, .5); B(1, 0, -10890423. / 25193600.); B(1, 1, 5000. / 7873);
+    B(2, 0, -102261. / 5000000.); B(2, 1, -5121. / 20000.); B(2, 2, 7873. / 10000.);
+    B(3, 0, 1. / 10.); B(3, 1, 1. / 6.); B(3, 3, 1. / 6.); return true;
+  case HYDRO_INTEGRATOR_RK4NONTVD:
+    t.order = 4; A(0, 0, 1); A(1, 0, 1); A(2, 0, 1);
+    A(3, 0, -1. / 3.); A(3, 1, 1. / 3.); A(3, 2, 2. / 3.); A(3, 3, 1. / 3.);
+    B(0, 0, .5); B(1, 1, .5); B(2, 2, 1); B(3, 3, 1. / 6.); return true;
+  }
+  return false;
+}
+
+// Per-stage plan of partial Shu-Osher sums (increment form d_k = u^(k) - u0).
+struct StagePlan {
+  int pk = -1;
+  int nf = 0;
+  int fslot[kMaxPartials] = {-1, -1, -1};
+  int fload[kMaxPartials] = {0, 0, 0};
+  double am[kMaxPartials] = {0, 0, 0};
+  double bm[kMaxPartials] = {0, 0, 0};
+  double a_self = 0, b_self = 0;
+  bool final_stage = false;
+};
+
+bool plan_stages(const Tableau& t, std::vector<StagePlan>& out, int& nslots) {
+  const int S = t.order;
+  out.assign(S, StagePlan());
+  bool live[4] = {false, false, false, false};
+  int slot_of[4] = {-1, -1, -1, -1};
+  bool used[kMaxPartials] = {false, false, false};
+  nslots = 0;
+  for (int k = 0; k < S; ++k) {
+    StagePlan& p = out[k];
+    p.pk = live[k] ? slot_of[k] : -1;
+    p.a_self = t.a[k][k];
+    p.b_self = t.b[k][k];
+    p.final_stage = (k == S - 1);
+    // row k's slot is consumed in this stage; the same thread reads it before
+    // any write below, so the slot may be reused at once
+    if (live[k]) used[slot_of[k]] = false;
+    live[k] = false;
+    for (int m = k + 1; m < S; ++m) {
+      const bool add = (k > 0 && t.a[m][k] != 0.0) || t.b[m][k] != 0.0;
+      if (!add) continue;
+      if (p.nf >= kMaxPartials) return false;
+      int j = p.nf++;
+      if (!live[m]) {
+        int s = -1;
+        for (int q = 0; q < kMaxPartials; ++q)
+          if (!used[q]) { s = q; break; }
+        if (s < 0) return false;
+        used[s] = true;
+        slot_of[m] = s;
+        p.fload[j] = 0;
+        live[m] = true;
+        nslots = std::max(nslots, s + 1);
+      } else {
+        p.fload[j] = 1;
+      }
+      p.fslot[j] = slot_of[m];
+      p.am[j] = (k > 0) ? t.a[m][k] : 0.0;
+      p.bm[j] = t.b[m][k];
+    }
+  }
+  return true;
+}
+
+// include/HydroGPU/toNumericString.h:12-22 (6 significant digits)
+double kernel_dx(double dx, bool six) {
+  if (!six) return dx;
+  char buf[64];
+  snprintf(buf, sizeof buf, "%.6g", dx);
+  return strtod(buf, nullptr);
+}
+
+struct DevScalars {
+  double dt;
+  double t;
+  long long steps;
+  unsigned health;
+  unsigned pad;
+};
+
+}  // namespace
+
+struct hydro_ctx {
+  int device;
+};
+
+struct hydro_sim {
+  hydro_ctx* ctx = nullptr;
+  hydro_sim_desc d{};
+  int dim = 1, N = 3;
+  int sz[3] = {1, 1, 1};
+  long long vol = 0, plane = 0;
+  bool mixed = false, kahan = false;
+  double kdx[3] = {1, 1, 1};
+  double cfl = 0.5;
+  hipStream_t stream = nullptr;
+  Tableau tab;
+  std::vector<StagePlan> plan;
+  int nslots = 0;
+  // device buffers (element type double for FP64, float for FP32-mixed)
+  void* U[2] = {nullptr, nullptr};
+  float* Ulo[2] = {nullptr, nullptr};
+  void* D[2] = {nullptr, nullptr};
+  void* P[kMaxPartials] = {nullptr, nullptr, nullptr};
+  void* speeds = nullptr;
+  double* blockmin = nullptr;
+  double* partial = nullptr;
+  DevScalars* sc = nullptr;
+  DevScalars* host_sc = nullptr;  // pinned
+  int cur = 0;
+  int dt_blocks = 1024;
+  int tot_blocks = 256;
+  // stage decomposition
+  int rows = 32, wpb = 4;
+  // measurement: event pairs around fused stage launches
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+namespace {
+
+size_t esize(const hydro_sim* s) { return s->mixed ? sizeof(float) : sizeof(double); }
+
+int validate(const hydro_sim_desc* d) {
+  if (!d) return fail(HYDRO_E_INVALID, "null descriptor");
+  if (d->dim < 1 || d->dim > 3) return fail(HYDRO_E_INVALID, "dim must be 1..3");
+  for (int a = 0; a < d->dim; ++a)
+    if (d->size[a] < 8) return fail(HYDRO_E_INVALID, "each used axis needs size >= 8 (2+2 ghosts + 4)");
+  for (int a = d->dim; a < 3; ++a)
+    if (d->size[a] != 1) return fail(HYDRO_E_INVALID, "unused axes must have size 1");
+  if (d->equation != HYDRO_EQN_EULER)
+    return fail(HYDRO_E_UNSUPPORTED, "this build of the HIP path implements the Euler equations");
+  if (d->solver < HYDRO_SOLVER_ROE || d->solver > HYDRO_SOLVER_HLLC) return fail(HYDRO_E_INVALID, "bad solver");
+  if (d->solver == HYDRO_SOLVER_ROE && d->dim == 3)
+    return fail(HYDRO_E_UNSUPPORTED, "3D Roe is not built in this version");
+  if (d->solver != HYDRO_SOLVER_ROE)
+    return fail(HYDRO_E_UNSUPPORTED, "HLL/HLLC are not built in this version");
+  if (d->limiter < 0 || d->limiter >= HYDRO_NUM_LIMITERS) return fail(HYDRO_E_INVALID, "bad limiter");
+  if (d->integrator < 0 || d->integrator >= HYDRO_NUM_INTEGRATORS) return fail(HYDRO_E_INVALID, "bad integrator");
+  if (d->precision != HYDRO_PRECISION_FP64 && d->precision != HYDRO_PRECISION_FP32_MIXED)
+    return fail(HYDRO_E_INVALID, "bad precision");
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < 2; ++m)
+      if (d->boundary[a][m] < HYDRO_BOUNDARY_NONE || d->boundary[a][m] > HYDRO_BOUNDARY_FREEFLOW)
+        return fail(HYDRO_E_INVALID, "bad boundary method");
+  if (d->use_fixed_dt && !(d->fixed_dt > 0)) return fail(HYDRO_E_INVALID, "fixed_dt must be > 0");
+  return HYDRO_OK;
+}
+
+// Euler::stateGetBoundaryKernelForBoundaryMethod (src/Equation/Euler.cpp:63-75)
+BoundaryArgs boundary_args(const hydro_sim* s, int axis) {
+  BoundaryArgs b{};
+  b.sx = s->sz[0];
+  b.sy = s->sz[1];
+  b.sz = s->sz[2];
+  b.plane = s->plane;
+  b.nstates = s->N;
+  b.axis = axis;
+  for (int st = 0; st < 8; ++st)
+    for (int mm = 0; mm < 2; ++mm) {
+      signed char k = -1;
+      if (st < s->N) {
+        switch (s->d.boundary[axis][mm]) {
+        case HYDRO_BOUNDARY_PERIODIC: k = 0; break;
+        case HYDRO_BOUNDARY_MIRROR: k = (axis + 1 == st) ? 2 : 1; break;
+        case HYDRO_BOUNDARY_FREEFLOW: k = 3; break;
+        default: k = -1;
+        }
+      }
+      b.kind[st][mm] = k;
+    }
+  return b;
+}
+
+int enqueue_boundary(hydro_sim* s) {
+  for (int axis = 0; axis < s->dim; ++axis) {
+    BoundaryArgs b = boundary_args(s, axis);
+    if (s->mixed) launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->stream);
+    else launch_boundary_f64(b, (double*)s->U[s->cur], s->stream);
+  }
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
+int enqueue_dt(hydro_sim* s, bool advance_time) {
+  if (!s->d.use_fixed_dt) {
+    DtArgs a{};
+    a.dim = s->dim;
+    a.sx = s->sz[0];
+    a.sy = s->sz[1];
+    a.sz = s->sz[2];
+    a.plane = s->plane;
+    for (int i = 0; i < 3; ++i) a.kdx[i] = s->kdx[i];
+    a.gamma = s->d.gamma;
+    a.solver = s->d.solver;
+    a.nblocks = s->dt_blocks;
+    a.blockmin = s->blockmin;
+    if (s->mixed) launch_dt_euler_f32(a, (const float*)s->U[s->cur], (const float*)s->speeds, s->stream);
+    else launch_dt_euler_f64(a, (const double*)s->U[s->cur], (const double*)s->speeds, s->stream);
+  }
+  FinalizeArgs f{};
+  f.blockmin = s->blockmin;
+  f.nblocks = s->d.use_fixed_dt ? 0 : s->dt_blocks;
+  f.cfl = s->cfl;
+  f.use_fixed = s->d.use_fixed_dt;
+  f.fixed_dt = s->d.fixed_dt;
+  f.clip = (s->d.flags & HYDRO_FLAG_CLIP_TO_TEND) ? 1 : 0;
+  f.t_end = s->d.t_end;
+  f.dt = &s->sc->dt;
+  // without advance the time/steps land in a scratch copy
+  f.t = advance_time ? &s->sc->t : &s->sc[1].t;
+  f.steps = advance_time ? &s->sc->steps : &s->sc[1].steps;
+  f.health = &s->sc->health;
+  launch_finalize_dt(f, s->stream);
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
+template <typename T>
+void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
+  const StagePlan& p = s->plan[k];
+  a.u0 = (const T*)s->U[s->cur];
+  a.u0lo = s->kahan ? s->Ulo[s->cur] : nullptr;
+  a.dk = (k == 0) ? nullptr : (const T*)s->D[(k - 1) & 1];
+  a.dnext = (T*)s->D[k & 1];
+  a.unew = (T*)s->U[s->cur ^ 1];
+  a.unewlo = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
+  for (int j = 0; j < kMaxPartials; ++j) {
+    a.part[j] = (T*)s->P[j];
+    a.fslot[j] = p.fslot[j] < 0 ? 0 : p.fslot[j];
+    a.fload[j] = p.fload[j];
+    a.am[j] = p.am[j];
+    a.bm[j] = p.bm[j];
+  }
+  a.pk = p.pk;
+  a.nf = p.nf;
+  a.a_self = p.a_self;
+  a.b_self = p.b_self;
+  a.final_stage = p.final_stage ? 1 : 0;
+  a.dt = &s->sc->dt;
+  a.sx = s->sz[0];
+  a.sy = s->sz[1];
+  a.sz = s->sz[2];
+  a.plane = s->plane;
+  for (int i = 0; i < 3; ++i) a.inv_dx[i] = 1.0 / s->kdx[i];
+  a.gamma = s->d.gamma;
+  a.lim = s->d.limiter;
+  const int interior_x = s->sz[0] - 4;
+  a.nstrips = (interior_x + 59) / 60;
+  if (s->dim == 1) {
+    a.rows = 1;
+    a.nchunks = 1;
+  } else {
+    a.rows = s->rows;
+    a.nchunks = (s->sz[1] - 4 + a.rows - 1) / a.rows;
+  }
+  a.nwaves = a.nstrips * a.nchunks;
+}
+
+int enqueue_stages(hydro_sim* s) {
+  const int S = s->tab.order;
+  for (int k = 0; k < S; ++k) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s->profiling) {
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      HIP_TRY(hipEventRecord(e0, s->stream));
+    }
+    if (s->mixed) {
+      StageArgs<float, float> a{};
+      fill_stage_args<float>(s, k, a);
+      launch_roe_stage_mixed(s->dim, s->d.limiter, s->kahan, a, s->wpb, s->stream);
+    } else {
+      StageArgs<double, double> a{};
+      fill_stage_args<double>(s, k, a);
+      launch_roe_stage_f64(s->dim, s->d.limiter, a, s->wpb, s->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    if (s->profiling) {
+      HIP_TRY(hipEventRecord(e1, s->stream));
+      s->ev.emplace_back(e0, e1);
+    }
+  }
+  // ghost cells of the new state are those of u0 (the RK combination leaves
+  // them unchanged: their derivative is zero)
+  const int nxt = s->cur ^ 1;
+  if (s->mixed) {
+    launch_copy_ghosts_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const float*)s->U[s->cur],
+                           (float*)s->U[nxt], s->stream);
+    if (s->kahan)
+      launch_copy_ghosts_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, s->Ulo[s->cur], s->Ulo[nxt],
+                             s->stream);
+  } else {
+    launch_copy_ghosts_f64(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const double*)s->U[s->cur],
+                           (double*)s->U[nxt], s->stream);
+  }
+  HIP_TRY(hipGetLastError());
+  s->cur = nxt;
+  // ForwardEuler's zero-gravity pass refreshes the ghosts after the update
+  // (src/Solver/SelfGravitation.cpp:122 + src/Integrator/ForwardEuler.cpp:29-31)
+  if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER) {
+    int rc = enqueue_boundary(s);
+    if (rc) return rc;
+  }
+  return HYDRO_OK;
+}
+
+int enqueue_update(hydro_sim* s) {
+  int rc = enqueue_boundary(s);
+  if (rc) return rc;
+  rc = enqueue_dt(s, true);
+  if (rc) return rc;
+  return enqueue_stages(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+void hydro_sim_desc_default(hydro_sim_desc* d) {
+  std::memset(d, 0, sizeof *d);
+  d->dim = 1;
+  d->size[0] = 256;
+  d->size[1] = d->size[2] = 1;
+  for (int a = 0; a < 3; ++a) {
+    d->xmin[a] = -.5;
+    d->xmax[a] = .5;
+    d->boundary[a][0] = d->boundary[a][1] = HYDRO_BOUNDARY_FREEFLOW;
+  }
+  d->equation = HYDRO_EQN_EULER;
+  d->solver = HYDRO_SOLVER_ROE;
+  d->limiter = HYDRO_LIMITER_SUPERBEE;
+  d->integrator = HYDRO_INTEGRATOR_FORWARDEULER;
+  d->precision = HYDRO_PRECISION_FP64;
+  d->cfl = 0.5;
+  d->fixed_dt = .001;
+  d->gamma = 1.4;
+  d->srhd_solve_prim_max_iter = 1000;
+  d->srhd_solve_prim_stop_epsilon = 1e-7;
+  d->srhd_solve_prim_vel_epsilon = 1e-15;
+  d->srhd_solve_prim_pmin_epsilon = 1e-16;
+  d->srhd_rho_min = 1e-15;
+  d->srhd_rho_max = 1e20;
+  d->srhd_eint_max = 1e20;
+  d->srhd_D_min = 1e-15;
+  d->srhd_D_max = 1e20;
+  d->srhd_tau_min = 1e-15;
+  d->srhd_tau_max = 1e20;
+  d->maxwell_permittivity = d->maxwell_permeability = d->maxwell_conductivity = 1.0;
+  d->flags = HYDRO_FLAG_REFERENCE;
+}
+
+const char* hydro_last_error(void) { return g_err.c_str(); }
+const char* hydro_version(void) { return "hydro-mi355x 0.1 (gfx950)"; }
+
+int hydro_ctx_create(int device, hydro_ctx** out) {
+  if (!out) return fail(HYDRO_E_INVALID, "null out");
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(HYDRO_E_INVALID, "no such device");
+  HIP_TRY(hipSetDevice(device));
+  hydro_ctx* c = new hydro_ctx;
+  c->device = device;
+  *out = c;
+  return HYDRO_OK;
+}
+
+void hydro_ctx_destroy(hydro_ctx* ctx) { delete ctx; }
+
+void hydro_sim_destroy(hydro_sim* s) {
+  if (!s) return;
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (auto& p : s->ev) {
+    hipEventDestroy(p.first);
+    hipEventDestroy(p.second);
+  }
+  for (int i = 0; i < 2; ++i) {
+    if (s->U[i]) hipFree(s->U[i]);
+    if (s->Ulo[i]) hipFree(s->Ulo[i]);
+    if (s->D[i]) hipFree(s->D[i]);
+  }
+  for (int j = 0; j < kMaxPartials; ++j)
+    if (s->P[j]) hipFree(s->P[j]);
+  if (s->speeds) hipFree(s->speeds);
+  if (s->blockmin) hipFree(s->blockmin);
+  if (s->partial) hipFree(s->partial);
+  if (s->sc) hipFree(s->sc);
+  if (s->host_sc) hipHostFree(s->host_sc);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
+  if (!ctx || !out) return fail(HYDRO_E_INVALID, "null argument");
+  int rc = validate(d);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hydro_sim* s = new hydro_sim;
+  s->ctx = ctx;
+  s->d = *d;
+  s->dim = d->dim;
+  s->N = d->dim + 2;
+  for (int a = 0; a < 3; ++a) s->sz[a] = a < d->dim ? d->size[a] : 1;
+  s->vol = (long long)s->sz[0] * s->sz[1] * s->sz[2];
+  s->plane = (s->vol + 63) & ~63LL;
+  s->mixed = d->precision == HYDRO_PRECISION_FP32_MIXED;
+  s->kahan = s->mixed && (d->flags & HYDRO_FLAG_KAHAN);
+  for (int a = 0; a < 3; ++a) {
+    // src/HydroGPUApp.cpp:224-226: (xmax - xmin) / (float)size
+    const double szf = a < d->dim ? (double)(float)d->size[a] : 1.0;
+    const double dx = (d->xmax[a] - d->xmin[a]) / szf;
+    s->kdx[a] = kernel_dx(dx, (d->flags & HYDRO_FLAG_REF_DX_6DIGITS) != 0);
+  }
+  s->cfl = (double)(float)d->cfl;  // HydroGPUApp.h:84 stores cfl as float
+  if (!get_tableau(d->integrator, s->tab) || !plan_stages(s->tab, s->plan, s->nslots)) {
+    delete s;
+    return fail(HYDRO_E_INVALID, "integrator not supported");
+  }
+  // rows per wave: enough waves to fill 256 CUs several times over
+  if (s->dim == 2) {
+    const long long strips = (s->sz[0] - 4 + 59) / 60;
+    int rows = 64;
+    while (rows > 8 && strips * ((s->sz[1] - 4 + rows - 1) / rows) < 4096) rows >>= 1;
+    s->rows = rows;
+  }
+  if (const char* e = getenv("HYDRO_ROWS_PER_WAVE")) s->rows = std::max(1, atoi(e));
+  if (const char* e = getenv("HYDRO_WAVES_PER_BLOCK")) s->wpb = std::min(4, std::max(1, atoi(e)));
+
+  auto cleanup = [&](int code, const std::string& m) {
+    hydro_sim_destroy(s);
+    return fail(code, m);
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(HYDRO_E_HIP, "stream create failed");
+  const size_t es = esize(s);
+  const size_t bytes = es * s->plane * s->N;
+  for (int i = 0; i < 2; ++i) {
+    if (hipMalloc(&s->U[i], bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (state)");
+    if (hipMemset(s->U[i], 0, bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "memset");
+    if (s->kahan) {
+      if (hipMalloc((void**)&s->Ulo[i], sizeof(float) * s->plane * s->N) != hipSuccess)
+        return cleanup(HYDRO_E_HIP, "out of device memory (state low words)");
+      hipMemset(s->Ulo[i], 0, sizeof(float) * s->plane * s->N);
+    }
+    if (s->tab.order > 1) {
+      if (hipMalloc(&s->D[i], bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (stages)");
+      if (hipMemset(s->D[i], 0, bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "memset");
+    }
+  }
+  for (int j = 0; j < s->nslots; ++j) {
+    if (hipMalloc(&s->P[j], bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (partials)");
+    hipMemset(s->P[j], 0, bytes);
+  }
+  if (hipMalloc((void**)&s->blockmin, sizeof(double) * s->dt_blocks) != hipSuccess)
+    return cleanup(HYDRO_E_HIP, "alloc");
+  if (hipMalloc((void**)&s->partial, sizeof(double) * s->tot_blocks * 8) != hipSuccess)
+    return cleanup(HYDRO_E_HIP, "alloc");
+  if (hipMalloc((void**)&s->sc, sizeof(DevScalars) * 2) != hipSuccess) return cleanup(HYDRO_E_HIP, "alloc");
+  hipMemset(s->sc, 0, sizeof(DevScalars) * 2);
+  if (hipHostMalloc((void**)&s->host_sc, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess)
+    return cleanup(HYDRO_E_HIP, "pinned alloc");
+  if (hipDeviceSynchronize() != hipSuccess) return cleanup(HYDRO_E_HIP, "sync");
+  *out = s;
+  return HYDRO_OK;
+}
+
+int hydro_sim_num_states(const hydro_sim* s) { return s ? s->N : 0; }
+size_t hydro_sim_volume(const hydro_sim* s) { return s ? (size_t)s->vol : 0; }
+void* hydro_sim_stream(hydro_sim* s) { return s ? (void*)s->stream : nullptr; }
+
+int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
+  if (!s || !aos) return fail(HYDRO_E_INVALID, "null argument");
+  if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "state size must be N * volume");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int N = s->N;
+  const long long V = s->vol;
+  if (s->mixed) {
+    std::vector<float> hi((size_t)s->plane * N, 0.f), lo((size_t)s->plane * N, 0.f);
+    for (long long c = 0; c < V; ++c)
+      for (int k = 0; k < N; ++k) {
+        const double v = aos[k + (size_t)N * c];
+        const float h = (float)v;
+        hi[(size_t)k * s->plane + c] = h;
+        lo[(size_t)k * s->plane + c] = (float)(v - (double)h);
+      }
+    HIP_TRY(hipMemcpy(s->U[s->cur], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (s->kahan) HIP_TRY(hipMemcpy(s->Ulo[s->cur], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> buf((size_t)s->plane * N, 0.0);
+    for (long long c = 0; c < V; ++c)
+      for (int k = 0; k < N; ++k) buf[(size_t)k * s->plane + c] = aos[k + (size_t)N * c];
+    HIP_TRY(hipMemcpy(s->U[s->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // a new initial state restarts the clock
+  DevScalars z{};
+  HIP_TRY(hipMemcpy(s->sc, &z, sizeof z, hipMemcpyHostToDevice));
+  return HYDRO_OK;
+}
+
+int hydro_sim_get_state(hydro_sim* s, double* aos, size_t n) {
+  if (!s || !aos) return fail(HYDRO_E_INVALID, "null argument");
+  if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "state size must be N * volume");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int N = s->N;
+  const long long V = s->vol;
+  if (s->mixed) {
+    std::vector<float> hi((size_t)s->plane * N), lo;
+    HIP_TRY(hipMemcpy(hi.data(), s->U[s->cur], hi.size() * sizeof(float), hipMemcpyDeviceToHost));
+    if (s->kahan) {
+      lo.resize(hi.size());
+      HIP_TRY(hipMemcpy(lo.data(), s->Ulo[s->cur], lo.size() * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    for (long long c = 0; c < V; ++c)
+      for (int k = 0; k < N; ++k) {
+        double v = hi[(size_t)k * s->plane + c];
+        if (s->kahan) v += (double)lo[(size_t)k * s->plane + c];
+        aos[k + (size_t)N * c] = v;
+      }
+  } else {
+    std::vector<double> buf((size_t)s->plane * N);
+    HIP_TRY(hipMemcpy(buf.data(), s->U[s->cur], buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long long c = 0; c < V; ++c)
+      for (int k = 0; k < N; ++k) aos[k + (size_t)N * c] = buf[(size_t)k * s->plane + c];
+  }
+  return HYDRO_OK;
+}
+
+int hydro_sim_get_primitives(hydro_sim* s, double* aos, size_t n) {
+  (void)aos;
+  (void)n;
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  return fail(HYDRO_E_UNSUPPORTED, "primitives exist for SRHD only");
+}
+
+int hydro_sim_boundary(hydro_sim* s) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  return enqueue_boundary(s);
+}
+
+int hydro_sim_calc_timestep(hydro_sim* s, double* dt_out) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  int rc = enqueue_dt(s, false);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(s->host_sc, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (dt_out) *dt_out = s->host_sc->dt;
+  return HYDRO_OK;
+}
+
+int hydro_sim_step(hydro_sim* s, double dt) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  s->host_sc->dt = dt;
+  HIP_TRY(hipMemcpyAsync(&s->sc->dt, &s->host_sc->dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  return enqueue_stages(s);
+}
+
+int hydro_sim_update(hydro_sim* s, double* dt_out) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  int rc = enqueue_update(s);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(s->host_sc, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (dt_out) *dt_out = s->host_sc->dt;
+  return HYDRO_OK;
+}
+
+int hydro_sim_run(hydro_sim* s, int nsteps) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = enqueue_update(s);
+    if (rc) return rc;
+  }
+  return HYDRO_OK;
+}
+
+int hydro_sim_sync(hydro_sim* s) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return HYDRO_OK;
+}
+
+int hydro_sim_time(hydro_sim* s, double* t, long long* steps, double* last_dt) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  HIP_TRY(hipMemcpyAsync(s->host_sc, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (t) *t = s->host_sc->t;
+  if (steps) *steps = s->host_sc->steps;
+  if (last_dt) *last_dt = s->host_sc->dt;
+  return HYDRO_OK;
+}
+
+int hydro_sim_health(hydro_sim* s, unsigned* flags_out) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  HIP_TRY(hipMemcpyAsync(s->host_sc, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (flags_out) *flags_out = s->host_sc->health;
+  return HYDRO_OK;
+}
+
+int hydro_sim_set_profiling(hydro_sim* s, int enable) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (auto& p : s->ev) {
+    hipEventDestroy(p.first);
+    hipEventDestroy(p.second);
+  }
+  s->ev.clear();
+  s->profiling = enable != 0;
+  return HYDRO_OK;
+}
+
+int hydro_sim_stage_timing(hydro_sim* s, double* avg_ms, long long* launches) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  double tot = 0.0;
+  for (auto& p : s->ev) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p.first, p.second));
+    tot += ms;
+  }
+  if (launches) *launches = (long long)s->ev.size();
+  if (avg_ms) *avg_ms = s->ev.empty() ? 0.0 : tot / (double)s->ev.size();
+  return HYDRO_OK;
+}
+
+int hydro_sim_totals(hydro_sim* s, double* totals_out) {
+  if (!s || !totals_out) return fail(HYDRO_E_INVALID, "null argument");
+  if (s->mixed)
+    launch_totals_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const float*)s->U[s->cur],
+                      s->kahan ? s->Ulo[s->cur] : nullptr, s->partial, s->tot_blocks, s->stream);
+  else
+    launch_totals_f64(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const double*)s->U[s->cur],
+                      s->partial, s->tot_blocks, s->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> part((size_t)s->tot_blocks * s->N);
+  HIP_TRY(hipMemcpyAsync(part.data(), s->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  double cellvol = 1.0;
+  for (int a = 0; a < s->dim; ++a)
+    cellvol *= (s->d.xmax[a] - s->d.xmin[a]) / (double)(float)s->d.size[a];
+  for (int k = 0; k < s->N; ++k) {
+    double acc = 0.0;
+    for (int b = 0; b < s->tot_blocks; ++b) acc += part[(size_t)b * s->N + k];
+    totals_out[k] = acc * cellvol;
+  }
+  return HYDRO_OK;
+}
+
+// batch API: provided by batch.cpp when built; stubs keep the ABI complete
+int hydro_batch_create(hydro_ctx*, const hydro_sim_desc*, int, hydro_batch**) __attribute__((weak));
+int hydro_batch_create(hydro_ctx*, const hydro_sim_desc*, int, hydro_batch**) {
+  return fail(HYDRO_E_UNSUPPORTED, "batched launch not built");
+}
+void hydro_batch_destroy(hydro_batch*) __attribute__((weak));
+void hydro_batch_destroy(hydro_batch*) {}
+int hydro_batch_set_state(hydro_batch*, int, const double*, size_t) __attribute__((weak));
+int hydro_batch_set_state(hydro_batch*, int, const double*, size_t) { return fail(HYDRO_E_UNSUPPORTED, "batched launch not built"); }
+int hydro_batch_get_state(hydro_batch*, int, double*, size_t) __attribute__((weak));
+int hydro_batch_get_state(hydro_batch*, int, double*, size_t) { return fail(HYDRO_E_UNSUPPORTED, "batched launch not built"); }
+int hydro_batch_run(hydro_batch*, int, int) __attribute__((weak));
+int hydro_batch_run(hydro_batch*, int, int) { return fail(HYDRO_E_UNSUPPORTED, "batched launch not built"); }
+int hydro_batch_sync(hydro_batch*) __attribute__((weak));
+int hydro_batch_sync(hydro_batch*) { return fail(HYDRO_E_UNSUPPORTED, "batched launch not built"); }
+int hydro_batch_member_dt(hydro_batch*, int, double*) __attribute__((weak));
+int hydro_batch_member_dt(hydro_batch*, int, double*) { return fail(HYDRO_E_UNSUPPORTED, "batched launch not built"); }
+long long hydro_batch_interior_cells(const hydro_batch*) __attribute__((weak));
+long long hydro_batch_interior_cells(const hydro_batch*) { return 0; }
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// launch.hpp -- host-side launchers of the device kernels (one translation
+// unit per kernel family, called by the C ABI in capi.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "stage_args.hpp"
+
+namespace hydro {
+
+// fused Roe stage, 1D/2D
+void launch_roe_stage_f64(int dim, int lim, const StageArgs<double, double>& a, int wpb, hipStream_t st);
+void launch_roe_stage_mixed(int dim, int lim, bool kahan, const StageArgs<float, float>& a, int wpb,
+                            hipStream_t st);
+
+// fused HLL/HLLC stage (first order: no limiter), 1D/2D/3D
+void launch_hll_stage_f64(int dim, int solver, const StageArgs<double, double>& a, const double* speeds,
+                          int wpb, hipStream_t st);
+void launch_hll_stage_mixed(int dim, int solver, bool kahan, const StageArgs<float, float>& a,
+                            const float* speeds, int wpb, hipStream_t st);
+
+// ghost fill along one axis (res/Common.cl:49-194), SoA planes.
+// kinds[s][minmax]: 0 periodic, 1 mirror, 2 reflect, 3 freeflow, -1 none
+struct BoundaryArgs {
+  int sx, sy, sz;
+  long long plane;
+  int nstates;
+  int axis;
+  signed char kind[8][2];
+};
+void launch_boundary_f64(const BoundaryArgs& b, double* planes, hipStream_t st);
+void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, hipStream_t st);
+
+// copy ghost cells of every plane from src to dst (the cells the stage
+// kernels do not write)
+void launch_copy_ghosts_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* src,
+                            double* dst, hipStream_t st);
+void launch_copy_ghosts_f32(int dim, int sx, int sy, int sz, long long plane, int n, const float* src,
+                            float* dst, hipStream_t st);
+
+// CFL: per-block minima of dt_cell, then a one-block finalise
+struct DtArgs {
+  int dim, sx, sy, sz;
+  long long plane;
+  double kdx[3];
+  double gamma;
+  int solver;            // 0 Roe, 1 HLL, 2 HLLC
+  int nblocks;
+  double* blockmin;      // [nblocks]
+};
+void launch_dt_euler_f64(const DtArgs& a, const double* u0, const double* speeds, hipStream_t st);
+void launch_dt_euler_f32(const DtArgs& a, const float* u0, const float* speeds, hipStream_t st);
+
+struct FinalizeArgs {
+  const double* blockmin;
+  int nblocks;
+  double cfl;
+  int use_fixed;
+  double fixed_dt;
+  int clip;
+  double t_end;
+  double* dt;       // out: this step's dt
+  double* t;        // in/out: simulated time
+  long long* steps; // in/out
+  unsigned* health; // in/out
+};
+void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st);
+
+// HLL/HLLC frozen wave speeds per face (res/EulerHLL.cl:17-138,
+// res/EulerHLLC.cl:15-131): 2 values (sl, sr) per face, SoA [side][2][cell]
+void launch_hll_speeds_f64(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
+                           const double* u0, double* speeds, hipStream_t st);
+void launch_hll_speeds_f32(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
+                           const float* u0, float* speeds, hipStream_t st);
+
+// FP64 interior sums, per block: partial[b * n + s]
+void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u,
+                       double* partial, int nblocks, hipStream_t st);
+void launch_totals_f32(int dim, int sx, int sy, int sz, long long plane, int n, const float* u,
+                       const float* lo, double* partial, int nblocks, hipStream_t st);
+
+}  // namespace hydro

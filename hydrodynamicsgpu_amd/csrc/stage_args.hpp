@@ -1,0 +1,42 @@
+// stage_args.hpp -- kernel argument blocks shared by the host driver and the
+// device code.
+#pragma once
+#include <stdint.h>
+
+namespace hydro {
+
+// Maximum number of Shu-Osher rows whose partial sums can be in flight.
+constexpr int kMaxPartials = 3;
+
+// One fused RK stage: L(u^(k)) with the frozen start-of-step data, then the
+// Shu-Osher combination.  States are carried as increments over the
+// start-of-step state u0:  u^(k) = u0 + d_k,  d_0 = 0, and row m of the
+// tableau becomes  d_{m+1} = sum_k (alpha[m][k] d_k + dt beta[m][k] L_k).
+template <typename T, typename S>
+struct StageArgs {
+  const S* u0;          // start-of-step state planes (FP64, or FP32 high words)
+  const float* u0lo;    // FP32 low words (compensated storage), or null
+  const T* dk;          // d_k planes, null for k == 0
+  T* dnext;             // d_{k+1} planes (non-final stage)
+  S* unew;              // final stage: u0 + d_S
+  float* unewlo;        // final stage low words (compensated), or null
+  T* part[kMaxPartials];     // partial-sum buffers (slots)
+  int pk;                    // slot holding row k's partial sum, -1 none
+  int nf;                    // future rows carried
+  int fslot[kMaxPartials];   // slot per future row
+  int fload[kMaxPartials];   // 1 if that slot already holds a partial sum
+  double am[kMaxPartials];   // alpha[m][k] per future row
+  double bm[kMaxPartials];   // beta[m][k] per future row
+  double a_self, b_self;     // alpha[k][k], beta[k][k]
+  int final_stage;
+  const double* dt;          // device-resident step size
+  int sx, sy, sz;
+  long long plane;           // elements between state planes
+  double inv_dx[3];          // 1 / kernel dx per axis
+  double gamma;
+  int lim;                   // runtime limiter id (generic instantiation)
+  int nstrips, rows, nchunks;  // wave -> (strip, chunk) decomposition
+  int nwaves;
+};
+
+}  // namespace hydro

@@ -1,0 +1,173 @@
+"""Initial conditions in the reference's conventions.
+
+Cell centres follow Solver::resetState (src/Solver/Solver.cpp:357-366):
+    pos = (xmax - xmin) * (i + .5) / size + xmin   over ALL cells, ghosts included.
+States follow buildStateEuler (res/util.lua:36-80): rho, rho*v, E with
+    E = rho * (|v|^2/2 + eInt),  eInt = p / ((gamma - 1) * rho)
+and Euler::readStateCell's channel map (src/Equation/Euler.cpp:78-89).
+Random perturbations use a counter hash (SplitMix64 of seed ^ cell index) so
+that every language reproduces the same field; the reference's crand() is
+Lua's math.random and is not reproducible across interpreters.
+"""
+import numpy as np
+
+from .config import num_states
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x):
+    """SplitMix64 finaliser of uint64 array `x` (Steele, Lea, Flood 2014)."""
+    with np.errstate(over="ignore"):
+        z = (np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)) & _M64
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed, cells, stream=0):
+    """U in [0,1) per cell: top 53 bits of splitmix64(seed ^ cell ^ stream<<56)."""
+    key = np.asarray(cells, dtype=np.uint64) ^ np.uint64(seed) ^ (np.uint64(stream) << np.uint64(56))
+    return (splitmix64(key) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def cell_centers(desc):
+    """Per-axis 1D arrays of cell centres (ghosts included)."""
+    out = []
+    for a in range(3):
+        n = desc.size[a]
+        i = np.arange(n, dtype=np.float64)
+        out.append((desc.xmax[a] - desc.xmin[a]) * (i + 0.5) / float(n) + desc.xmin[a])
+    return out
+
+
+def grid(desc):
+    """x, y, z arrays shaped (SZ, SY, SX) and flat cell indices."""
+    cx, cy, cz = cell_centers(desc)
+    z, y, x = np.meshgrid(cz, cy, cx, indexing="ij")
+    cells = np.arange(x.size, dtype=np.uint64).reshape(x.shape)
+    return x, y, z, cells
+
+
+def build_state_euler(desc, rho, vx, vy, vz, p, gamma=None):
+    """AoS state[s + N*cell] from primitive fields (arrays shaped (SZ,SY,SX))."""
+    g = desc.gamma if gamma is None else gamma
+    dim = desc.dim
+    vy = vy if dim > 1 else np.zeros_like(rho)
+    vz = vz if dim > 2 else np.zeros_like(rho)
+    eint = p / ((g - 1.0) * rho)
+    ekin = 0.5 * (vx * vx + vy * vy + vz * vz)
+    E = rho * (ekin + eint)
+    comps = [rho, rho * vx]
+    if dim > 1:
+        comps.append(rho * vy)
+    if dim > 2:
+        comps.append(rho * vz)
+    comps.append(E)
+    N = num_states(desc)
+    assert len(comps) == N
+    return np.stack([c.reshape(-1) for c in comps], axis=1).reshape(-1).copy()
+
+
+def sod(desc, gamma=None):
+    """res/initConds.lua:62-77: rho,p = 1,1 where x<=0 (and y<=0, z<=0), else .125,.1."""
+    x, y, z, _ = grid(desc)
+    inside = x <= 0
+    if desc.dim > 1:
+        inside &= y <= 0
+    if desc.dim > 2:
+        inside &= z <= 0
+    rho = np.where(inside, 1.0, 0.125)
+    p = np.where(inside, 1.0, 0.1)
+    zero = np.zeros_like(rho)
+    return build_state_euler(desc, rho, zero, zero, zero, p, gamma)
+
+
+def shear_kh(desc, seed=1234):
+    """2D Kelvin-Helmholtz shear layer: rho=2, vx=+.5 for |y|<.25 else rho=1,
+    vx=-.5; p=2.5; vy = .01 sin(4 pi x) + 1e-3 (2U-1)."""
+    x, y, z, cells = grid(desc)
+    inside = np.abs(y) < 0.25
+    rho = np.where(inside, 2.0, 1.0)
+    vx = np.where(inside, 0.5, -0.5)
+    u = uniform01(seed, cells)
+    vy = 0.01 * np.sin(4.0 * np.pi * x) + 1e-3 * (2.0 * u - 1.0)
+    p = np.full_like(rho, 2.5)
+    return build_state_euler(desc, rho, vx, vy, np.zeros_like(rho), p)
+
+
+def blast(desc, seed=4321):
+    """3D blast: rho = 1 + 1e-4 (2U-1), u=0, p=10 inside r<.2 else .1."""
+    x, y, z, cells = grid(desc)
+    r = np.sqrt(x * x + y * y + z * z)
+    u = uniform01(seed, cells)
+    rho = 1.0 + 1e-4 * (2.0 * u - 1.0)
+    p = np.where(r < 0.2, 10.0, 0.1)
+    zero = np.zeros_like(rho)
+    return build_state_euler(desc, rho, zero, zero, zero, p)
+
+
+def srhd_quadrants(desc, seed=777):
+    """Four-quadrant relativistic Riemann problem split at the domain centre.
+    Each quadrant draws rho in [.1,1], p in [.01,1] (log-uniform) and a
+    velocity of magnitude up to .995 (Lorentz factor up to ~10) from the
+    counter hash.  Returns the Newtonian (rho, rho v, E) the reference's
+    initVariables converts (res/SRHDCommon.cl:24-92)."""
+    x, y, z, _ = grid(desc)
+    cxm = 0.5 * (desc.xmin[0] + desc.xmax[0])
+    cym = 0.5 * (desc.xmin[1] + desc.xmax[1])
+    q = (x > cxm).astype(np.int64) + 2 * (y > cym).astype(np.int64)
+    qs = np.arange(4, dtype=np.uint64)
+    u0 = uniform01(seed, qs, 1)
+    u1 = uniform01(seed, qs, 2)
+    u2 = uniform01(seed, qs, 3)
+    u3 = uniform01(seed, qs, 4)
+    rho_q = 10.0 ** (-1.0 + u0)           # [0.1, 1]
+    p_q = 10.0 ** (-2.0 + 2.0 * u1)        # [0.01, 1]
+    speed = 0.995 * np.sqrt(u2)            # |v| <= .995
+    ang = 2.0 * np.pi * u3
+    vx_q = speed * np.cos(ang)
+    vy_q = speed * np.sin(ang)
+    rho = rho_q[q]
+    p = p_q[q]
+    vx = vx_q[q]
+    vy = vy_q[q]
+    return build_state_euler(desc, rho, vx, vy, np.zeros_like(rho), p)
+
+
+def riemann(desc, left, right):
+    """Two-state Riemann problem split at the x-centre; left/right are
+    (rho, u, p) tuples (u along x)."""
+    x, y, z, _ = grid(desc)
+    xc = 0.5 * (desc.xmin[0] + desc.xmax[0])
+    inside = x <= xc
+    rho = np.where(inside, left[0], right[0])
+    vx = np.where(inside, left[1], right[1])
+    p = np.where(inside, left[2], right[2])
+    zero = np.zeros_like(rho)
+    return build_state_euler(desc, rho, vx, zero, zero, p)
+
+
+def maxwell_1(desc):
+    """res/initConds.lua:779-795 (Maxwell-1): E=(0,0,eps), B=(0,0,+-1)."""
+    x, y, z, _ = grid(desc)
+    inside = x <= 0
+    if desc.dim > 1:
+        inside &= y <= 0
+    if desc.dim > 2:
+        inside &= z <= 0
+    eps = desc.maxwell_permittivity
+    zero = np.zeros(x.shape)
+    comps = [zero, zero, np.full(x.shape, eps), zero, zero, np.where(inside, 1.0, -1.0)]
+    return np.stack([c.reshape(-1) for c in comps], axis=1).reshape(-1).copy()
+
+
+def primitives_euler(desc, aos):
+    """(rho, v[dim], p) from an AoS Euler state."""
+    N = num_states(desc)
+    q = np.asarray(aos).reshape(-1, N)
+    rho = q[:, 0]
+    v = q[:, 1:1 + desc.dim] / rho[:, None]
+    E = q[:, desc.dim + 1]
+    p = (desc.gamma - 1.0) * (E - 0.5 * rho * (v * v).sum(axis=1))
+    return rho, v, p

@@ -1,0 +1,266 @@
+/*
+ * hydro_capi.h -- C ABI of the MI355X finite-volume update path.
+ *
+ * This is the drop-in boundary for the update path of KennethYeah/HydrodynamicsGPU:
+ * the reference's C++ classes (Solver, Roe, HLL, EulerRoe, EulerHLL, EulerHLLC,
+ * SRHDRoe, MaxwellRoe, Integrator/ForwardEuler/RungeKutta<...>) call these entry
+ * points instead of binding OpenCL kernels by index.  Only plain C types cross
+ * the boundary: ints, doubles, pointers and sizes.
+ *
+ * Conventions shared by every entry point
+ *   - Status: 0 = ok, negative = error (HYDRO_E_*).  hydro_last_error() returns
+ *     a thread-local message for the last failing call on this thread.
+ *   - Host arrays are borrowed for the duration of the call only.
+ *   - State arrays at the boundary use the reference layout: array-of-structs
+ *     state[s + N*cell], cell = x + SX*(y + SY*z), ghost cells included
+ *     (res/include/HydroGPU/Shared/Common.h:59-60).  On the device the library
+ *     stores structure-of-arrays planes; the conversion happens in the copy.
+ *   - One HIP stream per hydro_sim.  A handle is not re-entrant; distinct
+ *     handles may be driven from different threads.
+ */
+#ifndef HYDRO_CAPI_H
+#define HYDRO_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums --------------------------------------------------------------- */
+
+/* status codes */
+#define HYDRO_OK 0
+#define HYDRO_E_INVALID (-1)   /* bad argument / unsupported combination */
+#define HYDRO_E_HIP (-2)       /* HIP runtime error */
+#define HYDRO_E_NUMERIC (-3)   /* non-finite dt, negative density/pressure, ... */
+#define HYDRO_E_UNSUPPORTED (-4)
+
+/* equations (src/Equation/{Euler,SRHD,Maxwell}.cpp) */
+enum { HYDRO_EQN_EULER = 0, HYDRO_EQN_SRHD = 1, HYDRO_EQN_MAXWELL = 2 };
+
+/* flux solvers (src/Solver/{Roe,HLL}.cpp, res/EulerHLLC.cl) */
+enum { HYDRO_SOLVER_ROE = 0, HYDRO_SOLVER_HLL = 1, HYDRO_SOLVER_HLLC = 2 };
+
+/* slope limiters, in the order of res/SlopeLimiter.cl:5-68 */
+enum {
+  HYDRO_LIMITER_DONORCELL = 0,
+  HYDRO_LIMITER_LAXWENDROFF,
+  HYDRO_LIMITER_BEAMWARMING,
+  HYDRO_LIMITER_FROMM,
+  HYDRO_LIMITER_CHARM,
+  HYDRO_LIMITER_HCUS,
+  HYDRO_LIMITER_HQUICK,
+  HYDRO_LIMITER_KOREN,
+  HYDRO_LIMITER_MINMOD,
+  HYDRO_LIMITER_OSHKER,
+  HYDRO_LIMITER_OSPRE,
+  HYDRO_LIMITER_SMART,
+  HYDRO_LIMITER_SWEBY,
+  HYDRO_LIMITER_UMIST,
+  HYDRO_LIMITER_VANALBADA1,
+  HYDRO_LIMITER_VANALBADA2,
+  HYDRO_LIMITER_VANLEER,
+  HYDRO_LIMITER_MONOTIZEDCENTRAL,
+  HYDRO_LIMITER_SUPERBEE,
+  HYDRO_LIMITER_BARTHJESPERSEN,
+  HYDRO_NUM_LIMITERS
+};
+
+/* integrators: the registry of src/Solver/Solver.cpp:178-204 minus the
+ * unfinished BackwardEulerConjugateGradient (src/Integrator/BackwardEulerConjugateGradient.cpp:80) */
+enum {
+  HYDRO_INTEGRATOR_FORWARDEULER = 0,
+  HYDRO_INTEGRATOR_RK2,
+  HYDRO_INTEGRATOR_RK2HEUN,
+  HYDRO_INTEGRATOR_RK2RALSTON,
+  HYDRO_INTEGRATOR_RK3,
+  HYDRO_INTEGRATOR_RK4,
+  HYDRO_INTEGRATOR_RK4_3_8THSRULE,
+  HYDRO_INTEGRATOR_RK2TVD,
+  HYDRO_INTEGRATOR_RK2NONTVD,
+  HYDRO_INTEGRATOR_RK3TVD,
+  HYDRO_INTEGRATOR_RK4TVD,
+  HYDRO_INTEGRATOR_RK4NONTVD,
+  HYDRO_NUM_INTEGRATORS
+};
+
+/* boundary methods (include/HydroGPU/Boundary/Boundary.h:8-14) */
+enum {
+  HYDRO_BOUNDARY_NONE = -1,
+  HYDRO_BOUNDARY_PERIODIC = 0,
+  HYDRO_BOUNDARY_MIRROR = 1,
+  HYDRO_BOUNDARY_FREEFLOW = 2
+};
+
+/* arithmetic / storage precision of the update path */
+enum {
+  HYDRO_PRECISION_FP64 = 0,       /* FP64 state, FP64 arithmetic */
+  HYDRO_PRECISION_FP32_MIXED = 1  /* FP32 state + flux/eigen arithmetic; FP64 dt
+                                     reduction, totals and SRHD con2prim */
+};
+
+/* behaviour flags */
+#define HYDRO_FLAG_REF_DX_6DIGITS 0x1u  /* kernel dx = 6-significant-digit rendering
+                                           (include/HydroGPU/toNumericString.h:12-22,
+                                           src/Solver/Solver.cpp:272-280) */
+#define HYDRO_FLAG_GRAVITY_IDENTITY_PASS 0x2u /* replay the zero-potential self-gravity
+                                           RK pass of EulerRoe/EulerHLL and the zero
+                                           source pass of MaxwellRoe
+                                           (src/Solver/EulerRoe.cpp:31-34,
+                                           src/Solver/MaxwellRoe.cpp:38-46) */
+#define HYDRO_FLAG_KAHAN 0x4u           /* FP32_MIXED: compensated (Kahan) state update */
+#define HYDRO_FLAG_CLIP_TO_TEND 0x8u    /* clip the last step so t reaches t_end exactly */
+#define HYDRO_FLAG_REFERENCE 0x3u       /* REF_DX_6DIGITS | GRAVITY_IDENTITY_PASS */
+
+/* ---- simulation descriptor ---------------------------------------------- */
+
+/* One grid.  The fields restate the reference's Lua config keys
+ * (res/config.lua, src/HydroGPUApp.cpp:164-230) and its kernel `defs`. */
+typedef struct hydro_sim_desc {
+  int dim;               /* 1..3 ("last non-1 size", src/HydroGPUApp.cpp:213-222) */
+  int size[3];           /* cells per axis INCLUDING 2+2 ghosts; unused axes = 1 */
+  double xmin[3];        /* domain, ghosts inside (src/HydroGPUApp.cpp:224-226) */
+  double xmax[3];
+  int equation;          /* HYDRO_EQN_* */
+  int solver;            /* HYDRO_SOLVER_* */
+  int limiter;           /* HYDRO_LIMITER_*  (Roe only; HLL/HLLC have none) */
+  int integrator;        /* HYDRO_INTEGRATOR_* */
+  int precision;         /* HYDRO_PRECISION_* */
+  int boundary[3][2];    /* [axis][min,max] HYDRO_BOUNDARY_* */
+  double cfl;            /* stored as float by the reference (include/HydroGPU/HydroGPUApp.h:84) */
+  int use_fixed_dt;      /* useFixedDT */
+  double fixed_dt;       /* fixedDT */
+  double t_end;          /* used with HYDRO_FLAG_CLIP_TO_TEND */
+  /* defs (res/config.lua:70-93) */
+  double gamma;          /* idealGas_heatCapacityRatio */
+  int srhd_solve_prim_max_iter;
+  double srhd_solve_prim_stop_epsilon;
+  double srhd_solve_prim_vel_epsilon;
+  double srhd_solve_prim_pmin_epsilon;
+  double srhd_rho_min, srhd_rho_max, srhd_eint_max;
+  double srhd_D_min, srhd_D_max, srhd_tau_min, srhd_tau_max;
+  double maxwell_permittivity, maxwell_permeability, maxwell_conductivity;
+  unsigned flags;        /* HYDRO_FLAG_* */
+} hydro_sim_desc;
+
+typedef struct hydro_ctx hydro_ctx;
+typedef struct hydro_sim hydro_sim;
+typedef struct hydro_batch hydro_batch;
+
+/* Fill *d with the reference defaults (res/config.lua: Superbee, ForwardEuler,
+ * FREEFLOW everywhere, cfl 0.5, gamma 1.4, srhd_* defs). */
+void hydro_sim_desc_default(hydro_sim_desc* d);
+
+/* Last error message of this thread ("" if none). */
+const char* hydro_last_error(void);
+
+/* Library version string. */
+const char* hydro_version(void);
+
+/* ---- context ------------------------------------------------------------- */
+
+/* Replaces CLCommon's device/queue pick (src/HydroGPUApp.cpp:250-279). */
+int hydro_ctx_create(int device, hydro_ctx** out);
+void hydro_ctx_destroy(hydro_ctx* ctx);
+
+/* ---- one simulation ------------------------------------------------------ */
+
+/* Replaces Solver::init (src/Solver/Solver.cpp:48-205): validates the
+ * descriptor, allocates every device buffer once. */
+int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out);
+void hydro_sim_destroy(hydro_sim* sim);
+
+/* Number of states N per cell, and number of cells (ghosts included). */
+int hydro_sim_num_states(const hydro_sim* sim);
+size_t hydro_sim_volume(const hydro_sim* sim);
+
+/* Replaces Solver::Converter::toGPU / Solver::resetState's upload
+ * (src/Solver/Solver.cpp:333-337, 350-392).  `aos` holds n = N*volume doubles
+ * in the reference layout.  For SRHD the values are the Newtonian
+ * (rho, rho v, E) the reference's initState writes, and the call then runs
+ * initVariables (src/Solver/SRHDRoe.cpp:40-45). */
+int hydro_sim_set_state(hydro_sim* sim, const double* aos, size_t n);
+
+/* Replaces Solver::Converter::fromGPU (src/Solver/Solver.cpp:339-343). */
+int hydro_sim_get_state(hydro_sim* sim, double* aos, size_t n);
+
+/* SRHD primitives (rho, v, eInt) in the reference layout (src/Solver/SRHDRoe.cpp:9-13). */
+int hydro_sim_get_primitives(hydro_sim* sim, double* aos, size_t n);
+
+/* Replaces Solver::boundary (src/Solver/Solver.cpp:446-464; SRHD adds the
+ * primitive buffer, src/Solver/SRHDRoe.cpp:79-96). */
+int hydro_sim_boundary(hydro_sim* sim);
+
+/* Replaces Roe::calcTimestep / HLL::initStep + HLL::calcTimestep
+ * (src/Solver/Roe.cpp:88-92, src/Solver/HLL.cpp:225-232): freezes the
+ * start-of-step eigen data and returns cfl * min cell dt (FP64). */
+int hydro_sim_calc_timestep(hydro_sim* sim, double* dt_out);
+
+/* Replaces Solver::step(dt) for the solver (src/Solver/Roe.cpp:94-99,
+ * src/Solver/HLL.cpp:234-241) including the integrator stages
+ * (include/HydroGPU/Integrator/RungeKutta.h:59-125,
+ * src/Integrator/ForwardEuler.cpp:8-32) and the per-solver tail
+ * (gravity/source pass, SRHD constrainState + updatePrimitives).
+ * Must follow hydro_sim_calc_timestep (the frozen eigen data). */
+int hydro_sim_step(hydro_sim* sim, double dt);
+
+/* Replaces Solver::update (src/Solver/Solver.cpp:521-545):
+ * boundary -> initStep -> calcTimestep (or fixedDT) -> step.  Writes the dt
+ * used to *dt_out (may be NULL).  Synchronises the stream. */
+int hydro_sim_update(hydro_sim* sim, double* dt_out);
+
+/* Runs `nsteps` updates back to back without any host synchronisation: dt
+ * stays on the device (no per-step readback).  Asynchronous on the sim's
+ * stream; use hydro_sim_sync / hydro_sim_time to wait. */
+int hydro_sim_run(hydro_sim* sim, int nsteps);
+
+/* Waits for all work on the sim's stream. */
+int hydro_sim_sync(hydro_sim* sim);
+
+/* Simulated time, number of completed steps and the last dt (synchronises). */
+int hydro_sim_time(hydro_sim* sim, double* t, long long* steps, double* last_dt);
+
+/* FP64 domain-integrated conserved totals over the interior cells
+ * (sum of state * cell volume); totals_out has N entries. */
+int hydro_sim_totals(hydro_sim* sim, double* totals_out);
+
+/* Non-zero per-step health flags accumulated since creation:
+ * bit0 non-finite dt, bit1 non-positive density, bit2 con2prim not converged. */
+int hydro_sim_health(hydro_sim* sim, unsigned* flags_out);
+
+/* Device HIP stream of the sim (hipStream_t as void*), for callers that want
+ * to order their own work against it. */
+void* hydro_sim_stream(hydro_sim* sim);
+
+/* Measurement hook (no counterpart in the reference, whose event profiling
+ * is commented out: src/Solver/Solver.cpp:537-544).  While enabled, every
+ * fused RK-stage launch is bracketed by HIP events on the sim's stream. */
+int hydro_sim_set_profiling(hydro_sim* sim, int enable);
+/* Average duration in ms of the fused stage launches timed since profiling
+ * was enabled, and their count.  Synchronises the stream. */
+int hydro_sim_stage_timing(hydro_sim* sim, double* avg_ms, long long* launches);
+
+/* ---- many uneven simulations in one launch ------------------------------ */
+
+/* A batch of independent Euler-Roe grids (1D or 2D, any mix of sizes) that
+ * advance together: one persistent launch per RK stage covers every member,
+ * dt is reduced per member.  All members share equation, solver, limiter,
+ * integrator and precision; sizes, domains and boundaries may differ. */
+int hydro_batch_create(hydro_ctx* ctx, const hydro_sim_desc* descs, int count, hydro_batch** out);
+void hydro_batch_destroy(hydro_batch* b);
+int hydro_batch_set_state(hydro_batch* b, int member, const double* aos, size_t n);
+int hydro_batch_get_state(hydro_batch* b, int member, double* aos, size_t n);
+/* nsteps updates of every member; use_graph != 0 captures one step in a
+ * hipGraph and replays it. */
+int hydro_batch_run(hydro_batch* b, int nsteps, int use_graph);
+int hydro_batch_sync(hydro_batch* b);
+int hydro_batch_member_dt(hydro_batch* b, int member, double* dt_out);
+/* total interior cells over all members */
+long long hydro_batch_interior_cells(const hydro_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDRO_CAPI_H */

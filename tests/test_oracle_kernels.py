@@ -1,0 +1,172 @@
+"""Pins of the oracle's kernel-level functions against mathematics.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign, a transposed matrix, a swapped index) fails it:
+  * slope limiters: textbook closed forms (Sweby 1984 diagrams) at known points;
+  * Euler Roe basis: L R = I, the Roe property A(qR-qL) = F(qR)-F(qL), and
+    eigenvalues u-c, u, u+c of the Roe-averaged state;
+  * HLL/HLLC: consistency F(q,q) = F(q) and supersonic upwinding;
+  * SRHD: prim->cons->prim round trip through the Newton recovery, L R = I.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hydrodynamicsgpu_amd import config
+
+LIM = {n: i for i, n in enumerate(config.LIMITERS)}
+
+
+def euler_flux(q, dim, side, gamma):
+    rho = q[0]
+    v = q[1:1 + dim] / rho
+    E = q[dim + 1]
+    p = (gamma - 1) * (E - 0.5 * rho * np.dot(v, v))
+    un = v[side]
+    f = np.empty(dim + 2)
+    f[0] = rho * un
+    for a in range(dim):
+        f[1 + a] = rho * v[a] * un + (p if a == side else 0.0)
+    f[dim + 1] = (E + p) * un
+    return f
+
+
+def cons(rho, v, p, gamma):
+    v = np.asarray(v, dtype=float)
+    E = p / (gamma - 1) + 0.5 * rho * np.dot(v, v)
+    return np.concatenate([[rho], rho * v, [E]])
+
+
+@pytest.mark.parametrize("r,expect", [(0.0, 0.0), (0.5, 1.0), (1.0, 1.0), (1.5, 1.5), (3.0, 2.0), (-1.0, 0.0)])
+def test_superbee_known_values(r, expect):
+    assert oracle.slope_limiter(LIM["Superbee"], r) == expect
+
+
+@pytest.mark.parametrize("r", [-2.0, -0.3, 0.0, 0.25, 0.7, 1.0, 1.8, 4.0])
+def test_limiter_closed_forms(r):
+    assert oracle.slope_limiter(LIM["MinMod"], r) == max(0.0, min(r, 1.0))
+    assert oracle.slope_limiter(LIM["MonotizedCentral"], r) == max(0.0, min(2.0, 0.5 * (1 + r), 2 * r))
+    assert oracle.slope_limiter(LIM["DonorCell"], r) == 0.0
+    assert oracle.slope_limiter(LIM["LaxWendroff"], r) == 1.0
+    assert oracle.slope_limiter(LIM["BeamWarming"], r) == r
+    if r > 0:
+        # van Leer in its symmetric form (r + |r|)/(1 + |r|)
+        assert oracle.slope_limiter(LIM["VanLeer"], r) == pytest.approx(2 * r / (1 + r), rel=1e-15)
+
+
+def test_limiters_symmetric_property():
+    # second-order TVD limiters satisfy phi(r)/r = phi(1/r) (Sweby)
+    for name in ("Superbee", "MinMod", "MonotizedCentral", "VanLeer", "VanAlbada1"):
+        for r in (0.3, 0.8, 2.5, 7.0):
+            a = oracle.slope_limiter(LIM[name], r) / r
+            b = oracle.slope_limiter(LIM[name], 1.0 / r)
+            assert a == pytest.approx(b, rel=1e-13), name
+
+
+def test_limiter_nan_input_is_finite_for_tvd_limiters():
+    # r = 0/0 happens in every uniform region; fmin/fmax keep phi finite
+    for name in ("Superbee", "MinMod", "MonotizedCentral", "Koren", "Sweby", "UMIST", "Smart"):
+        assert np.isfinite(oracle.slope_limiter(LIM[name], float("nan"))), name
+
+
+def test_kernel_dx_six_digits():
+    assert oracle.kernel_dx(2.0 / 1024) == 0.00195312
+    assert oracle.kernel_dx(1.0 / 2048) == 0.000488281
+    assert oracle.kernel_dx(2.0 / 1024, six_digits=False) == 2.0 / 1024
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_euler_roe_basis(dim):
+    rng = np.random.default_rng(dim)
+    gamma = 1.4
+    for side in range(dim):
+        for _ in range(5):
+            qL = cons(rng.uniform(0.2, 3), rng.uniform(-1, 1, dim), rng.uniform(0.1, 5), gamma)
+            qR = cons(rng.uniform(0.2, 3), rng.uniform(-1, 1, dim), rng.uniform(0.1, 5), gamma)
+            lam, L, R = oracle.euler_eigenbasis(dim, side, gamma, qL, qR)
+            N = dim + 2
+            np.testing.assert_allclose(L @ R, np.eye(N), atol=1e-12)
+            A = R @ np.diag(lam) @ L
+            dF = euler_flux(qR, dim, side, gamma) - euler_flux(qL, dim, side, gamma)
+            np.testing.assert_allclose(A @ (qR - qL), dF, rtol=1e-11, atol=1e-11)
+            # eigenvalues of the Roe average: u_n - c, u_n (dim times), u_n + c
+            wL, wR = np.sqrt(qL[0]), np.sqrt(qR[0])
+            vL, vR = qL[1:1 + dim] / qL[0], qR[1:1 + dim] / qR[0]
+            pL = (gamma - 1) * (qL[-1] - 0.5 * qL[0] * vL @ vL)
+            pR = (gamma - 1) * (qR[-1] - 0.5 * qR[0] * vR @ vR)
+            HL, HR = (qL[-1] + pL) / qL[0], (qR[-1] + pR) / qR[0]
+            v = (wL * vL + wR * vR) / (wL + wR)
+            H = (wL * HL + wR * HR) / (wL + wR)
+            c = np.sqrt((gamma - 1) * (H - 0.5 * v @ v))
+            expect = [v[side] - c] + [v[side]] * dim + [v[side] + c]
+            np.testing.assert_allclose(lam, expect, rtol=1e-13, atol=1e-13)
+
+
+def test_euler_roe_basis_consistency():
+    # qL == qR: the Roe matrix is the flux Jacobian; A q = F(q) (degree-1 homogeneity)
+    gamma = 1.4
+    q = cons(1.3, [0.4, -0.2], 2.0, gamma)
+    for side in (0, 1):
+        lam, L, R = oracle.euler_eigenbasis(2, side, gamma, q, q)
+        A = R @ np.diag(lam) @ L
+        np.testing.assert_allclose(A @ q, euler_flux(q, 2, side, gamma), rtol=1e-12)
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+def test_hll_consistency_and_upwinding(solver):
+    gamma = 1.4
+    for dim in (1, 2, 3):
+        q = cons(1.1, np.linspace(0.1, 0.3, dim), 1.7, gamma)
+        for side in range(dim):
+            ev = oracle.hll_speeds(solver, dim, side, gamma, q, q)
+            f = oracle.hll_flux(solver, dim, side, gamma, q, q, ev)
+            np.testing.assert_allclose(f, euler_flux(q, dim, side, gamma), rtol=1e-13)
+        # supersonic to the right: both speeds > 0 -> left flux
+        qL = cons(1.0, [3.0] + [0.1] * (dim - 1), 1.0, gamma)
+        qR = cons(0.5, [2.8] + [0.0] * (dim - 1), 0.8, gamma)
+        ev = oracle.hll_speeds(solver, dim, 0, gamma, qL, qR)
+        assert ev[0] > 0
+        f = oracle.hll_flux(solver, dim, 0, gamma, qL, qR, ev)
+        np.testing.assert_allclose(f, euler_flux(qL, dim, 0, gamma), rtol=1e-13)
+
+
+def test_hll_middle_state_matches_toro():
+    # HLL flux between the speeds: Toro eq. 10.21
+    gamma = 1.4
+    qL = cons(1.0, [0.0], 1.0, gamma)
+    qR = cons(0.125, [0.0], 0.1, gamma)
+    ev = oracle.hll_speeds(1, 1, 0, gamma, qL, qR)
+    sl, sr = ev[0], ev[-1]
+    fL, fR = euler_flux(qL, 1, 0, gamma), euler_flux(qR, 1, 0, gamma)
+    expect = (sr * fL - sl * fR + sl * sr * (qR - qL)) / (sr - sl)
+    np.testing.assert_allclose(oracle.hll_flux(1, 1, 0, gamma, qL, qR, ev), expect, rtol=1e-14)
+
+
+def test_srhd_round_trip_and_basis():
+    d = config.make_desc([16, 16], solverName="SRHDRoe", defs={"idealGas_heatCapacityRatio": 5.0 / 3.0})
+    gamma = d.gamma
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        rho = rng.uniform(0.1, 1.0)
+        p = rng.uniform(0.01, 1.0)
+        speed = rng.uniform(0, 0.99)
+        ang = rng.uniform(0, 2 * np.pi)
+        v = np.array([speed * np.cos(ang), speed * np.sin(ang)])
+        newton = cons(rho, v, p, gamma)
+        c, prim = oracle.srhd_prim2cons(2, gamma, newton)
+        # closed forms: D = rho W, S = rho h W^2 v
+        W = 1 / np.sqrt(1 - v @ v)
+        eint = p / ((gamma - 1) * rho)
+        h = 1 + eint + p / rho
+        np.testing.assert_allclose(c[0], rho * W, rtol=1e-13)
+        np.testing.assert_allclose(c[1:3], rho * h * W * W * v, rtol=1e-12)
+        guess = prim * (1 + 0.05)
+        it, rec = oracle.srhd_con2prim(d, c, guess)
+        assert it > 0
+        np.testing.assert_allclose(rec, prim, rtol=2e-6, atol=1e-9)
+    primL = np.array([1.0, 0.3, -0.2, 1.5])
+    primR = np.array([0.5, 0.1, 0.4, 0.7])
+    for side in (0, 1):
+        lam, L, R = oracle.srhd_eigenbasis(2, side, gamma, primL, primR)
+        np.testing.assert_allclose(L @ R, np.eye(4), atol=1e-11)
+        assert lam[0] < lam[1] < lam[3]
