@@ -22,7 +22,7 @@ COMMON = [
     "-I", CSRC, "-I", INCLUDE, "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["roe_stage.hip", "aux_kernels.hip", "capi.cpp"]
+SOURCES = ["stage2d.hip", "stage3d.hip", "aux_kernels.hip", "capi.cpp"]
 
 
 def _hipcc():

@@ -1,6 +1,7 @@
 // aux_kernels.hip -- ghost fill, CFL reduction, HLL wave speeds, totals.
 #include "device_common.hpp"
 #include "phys_euler.hpp"
+#include "policies.hpp"
 #include "launch.hpp"
 
 namespace hydro {
@@ -169,18 +170,18 @@ HYDRO_DEV double block_min(double v, double* sh) {
 }
 
 template <typename S, int DIM>
-HYDRO_DEV typename EulerRoe<DIM, S>::Cell load_cell(const S* u, long long plane, long long cell, S gamma) {
-  S q[DIM + 2];
+HYDRO_DEV void load_q(const S* u, long long plane, long long cell, S q[DIM + 2]) {
 #pragma unroll
   for (int s = 0; s < DIM + 2; ++s) q[s] = u[s * plane + cell];
-  return EulerRoe<DIM, S>::template cell<S>(q, gamma);
 }
 
 // Roe: res/Roe.cl:4-65 on the frozen basis (lambda_max of the left face,
 // lambda_min of the right face, |.| + 1e-9), interior cells only.
-// HLL/HLLC: res/EulerHLL.cl:142-175 from the stored speeds, cells 2..S-2.
+// HLL/HLLC: res/EulerHLL.cl:142-175 on the frozen wave speeds of
+// res/EulerHLL.cl:17-115 / res/EulerHLLC.cl:15-108, cells 2..S-2; the right
+// face of cell S-2 is never written by the reference and reads as 0.
 template <typename S, int DIM>
-__global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0, const S* speeds) {
+__global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
   __shared__ double sh[4];
   const int sx = a.sx, sy = a.sy, sz = a.sz;
   const int hi = (a.solver == 0) ? 3 : 2;  // cells 2 .. S-hi
@@ -190,6 +191,7 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0, const S*
   const long long total = (long long)nx * ny * nz;
   const S gamma = S(a.gamma);
   const long long step[3] = {1, (long long)sx, (long long)sx * sy};
+  const int cs[3] = {sx, sy, sz};
   double m = INFINITY;
   bool bad = false;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -197,38 +199,44 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0, const S*
     const int x = 2 + (int)(i % nx);
     const int y = DIM > 1 ? 2 + (int)((i / nx) % ny) : 0;
     const int z = DIM > 2 ? 2 + (int)(i / ((long long)nx * ny)) : 0;
+    const int ci[3] = {x, y, z};
     const long long cell = x + (long long)sx * (y + (long long)sy * z);
+    S q[DIM + 2], qm[DIM + 2], qp[DIM + 2];
+    load_q<S, DIM>(u0, a.plane, cell, q);
     double r = INFINITY;
-    if (a.solver == 0) {
-      const auto cc = load_cell<S, DIM>(u0, a.plane, cell, gamma);
 #pragma unroll
-      for (int side = 0; side < DIM; ++side) {
-        const auto cm = load_cell<S, DIM>(u0, a.plane, cell - step[side], gamma);
-        const auto cp = load_cell<S, DIM>(u0, a.plane, cell + step[side], gamma);
-        const auto fL = EulerRoe<DIM, S>::face(cm, cc, side, gamma);
-        const auto fR = EulerRoe<DIM, S>::face(cc, cp, side, gamma);
+    for (int side = 0; side < DIM; ++side) {
+      const bool right_ok = ci[side] + 1 <= cs[side] - 2;
+      load_q<S, DIM>(u0, a.plane, cell - step[side], qm);
+      load_q<S, DIM>(u0, a.plane, right_ok ? cell + step[side] : cell, qp);
+      double d;
+      if (a.solver == 0) {
+        using P = EulerRoe<DIM, S>;
+        const auto cc = P::template cell<S>(q, gamma);
+        const auto fL = P::face(P::template cell<S>(qm, gamma), cc, side, gamma);
+        const auto fR = P::face(cc, P::template cell<S>(qp, gamma), side, gamma);
         const double lmax = fmax(0.0, double(fL.v[0] + fL.c));
         const double lmin = fmin(0.0, double(fR.v[0] - fR.c));
-        const double d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
-        bad |= !(d == d);
-        r = fmin(r, d);
+        d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
+      } else {
+        S slL, srL, slR, srR;
+        if (a.solver == 1) {
+          using P = HLLPolicy<DIM, S, 1>;
+          const auto cc = P::template frozen<S>(q, gamma);
+          P::speeds(P::template frozen<S>(qm, gamma), cc, side, gamma, slL, srL);
+          P::speeds(cc, P::template frozen<S>(qp, gamma), side, gamma, slR, srR);
+        } else {
+          using P = HLLPolicy<DIM, S, 2>;
+          const auto cc = P::template frozen<S>(q, gamma);
+          P::speeds(P::template frozen<S>(qm, gamma), cc, side, gamma, slL, srL);
+          P::speeds(cc, P::template frozen<S>(qp, gamma), side, gamma, slR, srR);
+        }
+        const double minL = fmin(0.0, right_ok ? double(slR) : 0.0);
+        const double maxL = fmax(0.0, double(srL));
+        d = a.kdx[side] / (maxL - minL);
       }
-    } else {
-      const int ci[3] = {x, y, z};
-      const int cs[3] = {sx, sy, sz};
-#pragma unroll
-      for (int side = 0; side < DIM; ++side) {
-        // speeds of the face at this cell (left) and at the next cell (right);
-        // faces outside 2..S-2 were never written and read as 0
-        const double lmaxL = double(speeds[(2 * side + 1) * a.plane + cell]);
-        const long long nxt = cell + step[side];
-        const double lminR = (ci[side] + 1 <= cs[side] - 2) ? double(speeds[(2 * side) * a.plane + nxt]) : 0.0;
-        const double minL = fmin(0.0, lminR);
-        const double maxL = fmax(0.0, lmaxL);
-        const double d = a.kdx[side] / (maxL - minL);
-        bad |= !(d == d);
-        r = fmin(r, d);
-      }
+      bad |= !(d == d);
+      r = fmin(r, d);
     }
     m = fmin(m, r);
   }
@@ -242,17 +250,17 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0, const S*
   if (threadIdx.x == 0) a.blockmin[blockIdx.x] = anybad ? NAN : m;
 }
 
-void launch_dt_euler_f64(const DtArgs& a, const double* u0, const double* speeds, hipStream_t st) {
+void launch_dt_euler_f64(const DtArgs& a, const double* u0, hipStream_t st) {
   dim3 g(a.nblocks), b(256);
-  if (a.dim == 1) hipLaunchKernelGGL((dt_kernel<double, 1>), g, b, 0, st, a, u0, speeds);
-  else if (a.dim == 2) hipLaunchKernelGGL((dt_kernel<double, 2>), g, b, 0, st, a, u0, speeds);
-  else hipLaunchKernelGGL((dt_kernel<double, 3>), g, b, 0, st, a, u0, speeds);
+  if (a.dim == 1) hipLaunchKernelGGL((dt_kernel<double, 1>), g, b, 0, st, a, u0);
+  else if (a.dim == 2) hipLaunchKernelGGL((dt_kernel<double, 2>), g, b, 0, st, a, u0);
+  else hipLaunchKernelGGL((dt_kernel<double, 3>), g, b, 0, st, a, u0);
 }
-void launch_dt_euler_f32(const DtArgs& a, const float* u0, const float* speeds, hipStream_t st) {
+void launch_dt_euler_f32(const DtArgs& a, const float* u0, hipStream_t st) {
   dim3 g(a.nblocks), b(256);
-  if (a.dim == 1) hipLaunchKernelGGL((dt_kernel<float, 1>), g, b, 0, st, a, u0, speeds);
-  else if (a.dim == 2) hipLaunchKernelGGL((dt_kernel<float, 2>), g, b, 0, st, a, u0, speeds);
-  else hipLaunchKernelGGL((dt_kernel<float, 3>), g, b, 0, st, a, u0, speeds);
+  if (a.dim == 1) hipLaunchKernelGGL((dt_kernel<float, 1>), g, b, 0, st, a, u0);
+  else if (a.dim == 2) hipLaunchKernelGGL((dt_kernel<float, 2>), g, b, 0, st, a, u0);
+  else hipLaunchKernelGGL((dt_kernel<float, 3>), g, b, 0, st, a, u0);
 }
 
 // Solver::findMinTimestep tail (src/Solver/Solver.cpp:510-515): dt = min * cfl;
@@ -286,102 +294,6 @@ __global__ void __launch_bounds__(256) finalize_dt_kernel(FinalizeArgs f) {
 
 void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st) {
   hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(256), 0, st, f);
-}
-
-// ---------------------------------------------------------------- HLL speeds
-template <typename S, int DIM>
-HYDRO_DEV void hll_face_speeds(const S* u0, long long plane, long long cl, long long cr, int side, S gamma,
-                               int solver, S& sl, S& sr) {
-  constexpr int N = DIM + 2, E = DIM + 1;
-  S qL[N], qR[N];
-#pragma unroll
-  for (int s = 0; s < N; ++s) {
-    qL[s] = u0[s * plane + cl];
-    qR[s] = u0[s * plane + cr];
-  }
-  // rotate into x (res/EulerHLL.cl:34-49)
-  if (side != 0) {
-    S t = qL[1]; qL[1] = qL[1 + side]; qL[1 + side] = t;
-    t = qR[1]; qR[1] = qR[1 + side]; qR[1 + side] = t;
-  }
-  S vL[DIM], vR[DIM], vsqL = S(0), vsqR = S(0);
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    vL[a] = tdiv(qL[1 + a], qL[0]);
-    vR[a] = tdiv(qR[1 + a], qR[0]);
-    vsqL += vL[a] * vL[a];
-    vsqR += vR[a] * vR[a];
-  }
-  const S g1 = gamma - S(1);
-  const S eTL = qL[E] * trcp(qL[0]), eTR = qR[E] * trcp(qR[0]);
-  const S pL = g1 * qL[0] * (eTL - S(.5) * vsqL);
-  const S pR = g1 * qR[0] * (eTR - S(.5) * vsqR);
-  const S hL = eTL + pL * trcp(qL[0]);
-  const S hR = eTR + pR * trcp(qR[0]);
-  const S csL = tsqrt(g1 * (hL - S(.5) * vsqL));
-  const S csR = tsqrt(g1 * (hR - S(.5) * vsqR));
-  const S wL = tsqrt(qL[0]), wR = tsqrt(qR[0]);
-  const S norm = trcp(wL + wR);
-  if (solver == 1) {
-    S v[DIM], vsq = S(0);
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      v[a] = (wL * vL[a] + wR * vR[a]) * norm;
-      vsq += v[a] * v[a];
-    }
-    const S H = (wL * hL + wR * hR) * norm;
-    const S c = tsqrt(g1 * (H - S(.5) * vsq));
-    sl = tmin(vL[0] - csL, v[0] - c);
-    sr = tmax(vR[0] + csR, v[0] + c);
-  } else {
-    // res/EulerHLLC.cl:85-97 (PVRS pressure, Toro's shock factors)
-    const S ps = tmax(S(0), S(.5) * (pL + pR) - S(.5) * (vR[0] - vL[0]) * S(.5) * (qL[0] + qR[0]) * S(.5) * (csL + csR));
-    const S k = (gamma + S(1)) / (S(2) * gamma);
-    const S fqL = ps <= pL ? S(1) : tsqrt(S(1) + k * (tdiv(ps, pL) - S(1)));
-    const S fqR = ps <= pR ? S(1) : tsqrt(S(1) + k * (tdiv(ps, pR) - S(1)));
-    sl = vL[0] - csL * fqL;
-    sr = vR[0] + csR * fqR;
-  }
-}
-
-template <typename S, int DIM>
-__global__ void __launch_bounds__(256) hll_speeds_kernel(int solver, int sx, int sy, int sz, long long plane,
-                                                         double gamma, const S* u0, S* speeds) {
-  // faces with every coordinate in 2..S-2 (res/EulerHLL.cl:122-130)
-  const int nx = sx - 3, ny = DIM > 1 ? sy - 3 : 1, nz = DIM > 2 ? sz - 3 : 1;
-  const long long total = (long long)nx * ny * nz;
-  const long long step[3] = {1, (long long)sx, (long long)sx * sy};
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int x = 2 + (int)(i % nx);
-    const int y = DIM > 1 ? 2 + (int)((i / nx) % ny) : 0;
-    const int z = DIM > 2 ? 2 + (int)(i / ((long long)nx * ny)) : 0;
-    const long long cell = x + (long long)sx * (y + (long long)sy * z);
-#pragma unroll
-    for (int side = 0; side < DIM; ++side) {
-      S sl, sr;
-      hll_face_speeds<S, DIM>(u0, plane, cell - step[side], cell, side, S(gamma), solver, sl, sr);
-      speeds[(2 * side) * plane + cell] = sl;
-      speeds[(2 * side + 1) * plane + cell] = sr;
-    }
-  }
-}
-
-template <typename S>
-static void hll_speeds(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma, const S* u0,
-                       S* speeds, hipStream_t st) {
-  dim3 g(2048), b(256);
-  if (dim == 1) hipLaunchKernelGGL((hll_speeds_kernel<S, 1>), g, b, 0, st, solver, sx, sy, sz, plane, gamma, u0, speeds);
-  else if (dim == 2) hipLaunchKernelGGL((hll_speeds_kernel<S, 2>), g, b, 0, st, solver, sx, sy, sz, plane, gamma, u0, speeds);
-  else hipLaunchKernelGGL((hll_speeds_kernel<S, 3>), g, b, 0, st, solver, sx, sy, sz, plane, gamma, u0, speeds);
-}
-void launch_hll_speeds_f64(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
-                           const double* u0, double* speeds, hipStream_t st) {
-  hll_speeds<double>(dim, solver, sx, sy, sz, plane, gamma, u0, speeds, st);
-}
-void launch_hll_speeds_f32(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
-                           const float* u0, float* speeds, hipStream_t st) {
-  hll_speeds<float>(dim, solver, sx, sy, sz, plane, gamma, u0, speeds, st);
 }
 
 // ---------------------------------------------------------------- totals

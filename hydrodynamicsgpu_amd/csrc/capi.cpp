@@ -211,10 +211,6 @@ int validate(const hydro_sim_desc* d) {
   if (d->equation != HYDRO_EQN_EULER)
     return fail(HYDRO_E_UNSUPPORTED, "this build of the HIP path implements the Euler equations");
   if (d->solver < HYDRO_SOLVER_ROE || d->solver > HYDRO_SOLVER_HLLC) return fail(HYDRO_E_INVALID, "bad solver");
-  if (d->solver == HYDRO_SOLVER_ROE && d->dim == 3)
-    return fail(HYDRO_E_UNSUPPORTED, "3D Roe is not built in this version");
-  if (d->solver != HYDRO_SOLVER_ROE)
-    return fail(HYDRO_E_UNSUPPORTED, "HLL/HLLC are not built in this version");
   if (d->limiter < 0 || d->limiter >= HYDRO_NUM_LIMITERS) return fail(HYDRO_E_INVALID, "bad limiter");
   if (d->integrator < 0 || d->integrator >= HYDRO_NUM_INTEGRATORS) return fail(HYDRO_E_INVALID, "bad integrator");
   if (d->precision != HYDRO_PRECISION_FP64 && d->precision != HYDRO_PRECISION_FP32_MIXED)
@@ -275,8 +271,8 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
     a.solver = s->d.solver;
     a.nblocks = s->dt_blocks;
     a.blockmin = s->blockmin;
-    if (s->mixed) launch_dt_euler_f32(a, (const float*)s->U[s->cur], (const float*)s->speeds, s->stream);
-    else launch_dt_euler_f64(a, (const double*)s->U[s->cur], (const double*)s->speeds, s->stream);
+    if (s->mixed) launch_dt_euler_f32(a, (const float*)s->U[s->cur], s->stream);
+    else launch_dt_euler_f64(a, (const double*)s->U[s->cur], s->stream);
   }
   FinalizeArgs f{};
   f.blockmin = s->blockmin;
@@ -330,11 +326,19 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   if (s->dim == 1) {
     a.rows = 1;
     a.nchunks = 1;
-  } else {
+    a.nwaves = a.nstrips;
+  } else if (s->dim == 2) {
     a.rows = s->rows;
     a.nchunks = (s->sz[1] - 4 + a.rows - 1) / a.rows;
+    a.nwaves = a.nstrips * a.nchunks;
+  } else {
+    // 3D: rows = z-planes per workgroup, nchunks = y-chunks, nwaves = workgroups
+    const int ry = stage3d_rows_per_block(s->d.solver);
+    a.rows = s->rows;
+    a.nchunks = (s->sz[1] - 4 + ry - 1) / ry;
+    const int nz = (s->sz[2] - 4 + a.rows - 1) / a.rows;
+    a.nwaves = a.nstrips * a.nchunks * nz;
   }
-  a.nwaves = a.nstrips * a.nchunks;
 }
 
 int enqueue_stages(hydro_sim* s) {
@@ -349,11 +353,13 @@ int enqueue_stages(hydro_sim* s) {
     if (s->mixed) {
       StageArgs<float, float> a{};
       fill_stage_args<float>(s, k, a);
-      launch_roe_stage_mixed(s->dim, s->d.limiter, s->kahan, a, s->wpb, s->stream);
+      if (s->dim == 3) launch_stage3d_mixed(s->d.solver, s->d.limiter, s->kahan, a, s->stream);
+      else launch_stage2d_mixed(s->dim, s->d.solver, s->d.limiter, s->kahan, a, s->wpb, s->stream);
     } else {
       StageArgs<double, double> a{};
       fill_stage_args<double>(s, k, a);
-      launch_roe_stage_f64(s->dim, s->d.limiter, a, s->wpb, s->stream);
+      if (s->dim == 3) launch_stage3d_f64(s->d.solver, s->d.limiter, a, s->stream);
+      else launch_stage2d_f64(s->dim, s->d.solver, s->d.limiter, a, s->wpb, s->stream);
     }
     HIP_TRY(hipGetLastError());
     if (s->profiling) {
@@ -496,11 +502,18 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     delete s;
     return fail(HYDRO_E_INVALID, "integrator not supported");
   }
-  // rows per wave: enough waves to fill 256 CUs several times over
+  // rows (2D) or z-planes (3D) per wave: enough waves to fill 256 CUs
+  // several times over
   if (s->dim == 2) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
     int rows = 64;
     while (rows > 8 && strips * ((s->sz[1] - 4 + rows - 1) / rows) < 4096) rows >>= 1;
+    s->rows = rows;
+  } else if (s->dim == 3) {
+    const long long strips = (s->sz[0] - 4 + 59) / 60;
+    const long long ych = (s->sz[1] - 4 + stage3d_rows_per_block(d->solver) - 1) / stage3d_rows_per_block(d->solver);
+    int rows = 64;
+    while (rows > 8 && strips * ych * ((s->sz[2] - 4 + rows - 1) / rows) < 1024) rows >>= 1;
     s->rows = rows;
   }
   if (const char* e = getenv("HYDRO_ROWS_PER_WAVE")) s->rows = std::max(1, atoi(e));

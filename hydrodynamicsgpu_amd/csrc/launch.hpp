@@ -6,16 +6,17 @@
 
 namespace hydro {
 
-// fused Roe stage, 1D/2D
-void launch_roe_stage_f64(int dim, int lim, const StageArgs<double, double>& a, int wpb, hipStream_t st);
-void launch_roe_stage_mixed(int dim, int lim, bool kahan, const StageArgs<float, float>& a, int wpb,
-                            hipStream_t st);
+// fused RK stage, 1D/2D (stage2d.hip).  solver: 0 Roe, 1 HLL, 2 HLLC
+void launch_stage2d_f64(int dim, int solver, int lim, const StageArgs<double, double>& a, int wpb,
+                        hipStream_t st);
+void launch_stage2d_mixed(int dim, int solver, int lim, bool kahan, const StageArgs<float, float>& a, int wpb,
+                          hipStream_t st);
 
-// fused HLL/HLLC stage (first order: no limiter), 1D/2D/3D
-void launch_hll_stage_f64(int dim, int solver, const StageArgs<double, double>& a, const double* speeds,
-                          int wpb, hipStream_t st);
-void launch_hll_stage_mixed(int dim, int solver, bool kahan, const StageArgs<float, float>& a,
-                            const float* speeds, int wpb, hipStream_t st);
+// fused RK stage, 3D (stage3d.hip): workgroups of kStage3dWaves rows
+constexpr int kStage3dWaves = 8;
+int stage3d_rows_per_block(int solver);
+void launch_stage3d_f64(int solver, int lim, const StageArgs<double, double>& a, hipStream_t st);
+void launch_stage3d_mixed(int solver, int lim, bool kahan, const StageArgs<float, float>& a, hipStream_t st);
 
 // ghost fill along one axis (res/Common.cl:49-194), SoA planes.
 // kinds[s][minmax]: 0 periodic, 1 mirror, 2 reflect, 3 freeflow, -1 none
@@ -46,8 +47,8 @@ struct DtArgs {
   int nblocks;
   double* blockmin;      // [nblocks]
 };
-void launch_dt_euler_f64(const DtArgs& a, const double* u0, const double* speeds, hipStream_t st);
-void launch_dt_euler_f32(const DtArgs& a, const float* u0, const float* speeds, hipStream_t st);
+void launch_dt_euler_f64(const DtArgs& a, const double* u0, hipStream_t st);
+void launch_dt_euler_f32(const DtArgs& a, const float* u0, hipStream_t st);
 
 struct FinalizeArgs {
   const double* blockmin;
@@ -63,13 +64,6 @@ struct FinalizeArgs {
   unsigned* health; // in/out
 };
 void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st);
-
-// HLL/HLLC frozen wave speeds per face (res/EulerHLL.cl:17-138,
-// res/EulerHLLC.cl:15-131): 2 values (sl, sr) per face, SoA [side][2][cell]
-void launch_hll_speeds_f64(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
-                           const double* u0, double* speeds, hipStream_t st);
-void launch_hll_speeds_f32(int dim, int solver, int sx, int sy, int sz, long long plane, double gamma,
-                           const float* u0, float* speeds, hipStream_t st);
 
 // FP64 interior sums, per block: partial[b * n + s]
 void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u,

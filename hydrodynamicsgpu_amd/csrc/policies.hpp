@@ -1,0 +1,262 @@
+// policies.hpp -- face solvers plugged into the fused stage kernels.
+//
+// A policy turns (frozen start-of-step data, stage state) of the two cells of
+// an interface into the interface flux in grid order.  It runs in two passes
+// so the Roe limiter can see its neighbouring interfaces:
+//   face_a: per-interface work that needs only the interface's two cells
+//           (Roe: frozen basis, deltaQTilde = L (qR - qL), L qavg);
+//   face_b: the flux, given the neighbours' deltaQTilde (Roe) or nothing
+//           (HLL/HLLC, first order: res/EulerHLL.cl:317-335 keeps the
+//           limiter disabled).
+#pragma once
+#include "device_common.hpp"
+#include "phys_euler.hpp"
+
+namespace hydro {
+
+// ------------------------------------------------------------------ Roe
+template <int DIM, typename T, int LIM>
+struct RoePolicy {
+  using P = EulerRoe<DIM, T>;
+  static constexpr int N = P::N;
+  static constexpr bool kNeighbors = true;
+  using Frozen = typename P::Cell;
+  static constexpr int NF = DIM + 2;  // w, v[DIM], H
+
+  struct FaceA {
+    typename P::Face f;
+    T dq[N];  // deltaQTilde of this interface (0 where the reference never writes it)
+    T lq[N];  // L * qavg
+  };
+
+  template <typename S>
+  static HYDRO_DEV Frozen frozen(const S q0[N], T gamma) {
+    return P::template cell<S>(q0, gamma);
+  }
+  static HYDRO_DEV void pack(const Frozen& c, T o[NF]) {
+    o[0] = c.w;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) o[1 + d] = c.v[d];
+    o[DIM + 1] = c.H;
+  }
+  static HYDRO_DEV Frozen unpack(const T o[NF]) {
+    Frozen c;
+    c.w = o[0];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) c.v[d] = o[1 + d];
+    c.H = o[DIM + 1];
+    return c;
+  }
+
+  // res/Roe.cl:77-150 (deltaQTilde) and the L*qavg half of res/Roe.cl:264-270
+  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const T uL[N], const T uR[N], int side,
+                                bool valid, T gamma) {
+    FaceA a;
+    a.f = P::face(cL, cR, side, gamma);
+    T dlt[N], avg[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      dlt[s] = uR[s] - uL[s];
+      avg[s] = T(.5) * (uR[s] + uL[s]);
+    }
+    P::rotate(dlt, side);
+    P::rotate(avg, side);
+    P::left(a.f, dlt, a.dq, gamma);
+    P::left(a.f, avg, a.lq, gamma);
+    if (!valid) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) a.dq[s] = T(0);
+    }
+    return a;
+  }
+
+  // res/Roe.cl:272-298: limited characteristic flux, then R back to grid order
+  static HYDRO_DEV void face_b(const FaceA& a, const T dqlo[N], const T dqhi[N], T dt_dx, int side, int lim,
+                               T F[N]) {
+    T lam[N], ft[N];
+    P::eig(a.f, lam);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const T l = lam[k];
+      const bool up = l >= T(0);
+      const T num = up ? dqlo[k] : dqhi[k];
+      const T theta = up ? T(1) : T(-1);
+      const T r = tdiv(num, a.dq[k]);
+      const T phi = limiter<LIM, T>(lim, r);
+      const T eps = l * dt_dx;
+      ft[k] = l * a.lq[k] - T(.5) * (l * a.dq[k]) * (theta + phi * (eps - theta));
+    }
+    P::right(a.f, ft, F);
+    P::rotate(F, side);
+  }
+};
+
+// ------------------------------------------------------------------ HLL / HLLC
+template <int DIM, typename T, int SOLVER>  // SOLVER: 1 HLL, 2 HLLC
+struct HLLPolicy {
+  static constexpr int N = DIM + 2;
+  static constexpr int E = DIM + 1;
+  static constexpr bool kNeighbors = false;
+  struct Frozen {
+    T rho, p, cs, w, H;
+    T v[DIM];
+  };
+  static constexpr int NF = DIM + 5;
+  struct FaceA {
+    T F[N];
+    T dq[N];  // unused (no limiter); kept so the kernels stay generic
+  };
+
+  template <typename S>
+  static HYDRO_DEV Frozen frozen(const S q0[N], T gamma) {
+    Frozen c;
+    c.rho = T(q0[0]);
+    const T inv = trcp(c.rho);
+    T vsq = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      c.v[d] = tdiv(T(q0[1 + d]), c.rho);
+      vsq += c.v[d] * c.v[d];
+    }
+    const T eT = T(q0[E]) * inv;
+    c.p = (gamma - T(1)) * c.rho * (eT - T(.5) * vsq);
+    c.H = eT + c.p * inv;
+    c.cs = tsqrt((gamma - T(1)) * (c.H - T(.5) * vsq));
+    c.w = tsqrt(c.rho);
+    return c;
+  }
+  static HYDRO_DEV void pack(const Frozen& c, T o[NF]) {
+    o[0] = c.rho; o[1] = c.p; o[2] = c.cs; o[3] = c.w; o[4] = c.H;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) o[5 + d] = c.v[d];
+  }
+  static HYDRO_DEV Frozen unpack(const T o[NF]) {
+    Frozen c;
+    c.rho = o[0]; c.p = o[1]; c.cs = o[2]; c.w = o[3]; c.H = o[4];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) c.v[d] = o[5 + d];
+    return c;
+  }
+
+  // frozen wave speeds of one interface (res/EulerHLL.cl:56-103,
+  // res/EulerHLLC.cl:54-97), normal = `side`
+  static HYDRO_DEV void speeds(const Frozen& L, const Frozen& R, int side, T gamma, T& sl, T& sr) {
+    const T vLn = L.v[side], vRn = R.v[side];
+    if constexpr (SOLVER == 1) {
+      const T norm = trcp(L.w + R.w);
+      T vsq = T(0), vn = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const T v = (L.w * L.v[d] + R.w * R.v[d]) * norm;
+        vsq += v * v;
+        if (d == side) vn = v;
+      }
+      const T H = (L.w * L.H + R.w * R.H) * norm;
+      const T c = tsqrt((gamma - T(1)) * (H - T(.5) * vsq));
+      sl = tmin(vLn - L.cs, vn - c);
+      sr = tmax(vRn + R.cs, vn + c);
+    } else {
+      const T ps = tmax(T(0), T(.5) * (L.p + R.p) - T(.5) * (vRn - vLn) * T(.5) * (L.rho + R.rho) * T(.5) * (L.cs + R.cs));
+      const T k = (gamma + T(1)) / (T(2) * gamma);
+      const T qL = ps <= L.p ? T(1) : tsqrt(T(1) + k * (tdiv(ps, L.p) - T(1)));
+      const T qR = ps <= R.p ? T(1) : tsqrt(T(1) + k * (tdiv(ps, R.p) - T(1)));
+      sl = vLn - L.cs * qL;
+      sr = vRn + R.cs * qR;
+    }
+  }
+
+  // res/EulerHLL.cl:185-343, res/EulerHLLC.cl:178-419 (HLLC_METHOD 2)
+  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const T uL_[N], const T uR_[N], int side,
+                                bool valid, T gamma) {
+    (void)valid;
+    FaceA a;
+    T sl, sr;
+    speeds(cL, cR, side, gamma, sl, sr);
+    T qL[N], qR[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      qL[s] = uL_[s];
+      qR[s] = uR_[s];
+      a.dq[s] = T(0);
+    }
+    EulerRoe<DIM, T>::rotate(qL, side);
+    EulerRoe<DIM, T>::rotate(qR, side);
+    const T g1 = gamma - T(1);
+    const T dL = qL[0], dR = qR[0];
+    const T iL = trcp(dL), iR = trcp(dR);
+    T vL[DIM], vR[DIM], vsqL = T(0), vsqR = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      vL[d] = tdiv(qL[1 + d], dL);
+      vR[d] = tdiv(qR[1 + d], dR);
+      vsqL += vL[d] * vL[d];
+      vsqR += vR[d] * vR[d];
+    }
+    const T eTL = qL[E] * iL, eTR = qR[E] * iR;
+    const T pL = g1 * dL * (eTL - T(.5) * vsqL);
+    const T pR = g1 * dR * (eTR - T(.5) * vsqR);
+    const T hL = eTL + pL * iL, hR = eTR + pR * iR;
+    T fL[N], fR[N];
+    fL[0] = dL * vL[0];
+    fR[0] = dR * vR[0];
+    fL[1] = dL * vL[0] * vL[0] + pL;
+    fR[1] = dR * vR[0] * vR[0] + pR;
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) {
+      fL[1 + d] = dL * vL[d] * vL[0];
+      fR[1 + d] = dR * vR[d] * vR[0];
+    }
+    fL[E] = dL * hL * vL[0];
+    fR[E] = dR * hR * vR[0];
+    T F[N];
+    if constexpr (SOLVER == 1) {
+      if (T(0) <= sl) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) F[s] = fL[s];
+      } else if (sr <= T(0)) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) F[s] = fR[s];
+      } else {
+        const T inv = trcp(sr - sl);
+#pragma unroll
+        for (int s = 0; s < N; ++s) F[s] = (sr * fL[s] - sl * fR[s] + sl * sr * (qR[s] - qL[s])) * inv;
+      }
+    } else {
+      const T sStar = tdiv(dR * vR[0] * (sr - vR[0]) - dL * vL[0] * (sl - vL[0]) + pL - pR,
+                           dR * (sr - vR[0]) - dL * (sl - vL[0]));
+      if (T(0) <= sl) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) F[s] = fL[s];
+      } else if (sr <= T(0)) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) F[s] = fR[s];
+      } else {
+        const T pLR = T(.5) * (pL + pR + dL * (sl - vL[0]) * (sStar - vL[0]) + dR * (sr - vR[0]) * (sStar - vR[0]));
+        const bool left = sStar >= T(0);
+        const T sK = left ? sl : sr;
+        const T inv = trcp(sK - sStar);
+#pragma unroll
+        for (int s = 0; s < N; ++s) {
+          const T q = left ? qL[s] : qR[s];
+          const T f = left ? fL[s] : fR[s];
+          F[s] = sStar * (sK * q - f);
+        }
+        F[1] += sK * pLR;
+        F[E] += sK * pLR * sStar;
+#pragma unroll
+        for (int s = 0; s < N; ++s) F[s] *= inv;
+      }
+    }
+    EulerRoe<DIM, T>::rotate(F, side);
+#pragma unroll
+    for (int s = 0; s < N; ++s) a.F[s] = F[s];
+    return a;
+  }
+
+  static HYDRO_DEV void face_b(const FaceA& a, const T*, const T*, T, int, int, T F[N]) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) F[s] = a.F[s];
+  }
+};
+
+}  // namespace hydro

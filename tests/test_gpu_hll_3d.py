@@ -1,0 +1,127 @@
+"""GPU parity of the HLL/HLLC path (1D/2D/3D) and of 3D Roe against the
+oracle, through the C ABI.
+
+Tolerance: FP64 max |gpu - oracle| <= 1e-10 * max |oracle| after several
+steps (same reasoning as the 2D Roe tests: closed-form algebra and
+reciprocal multiplies differ from the reference's expression order at the
+1e-15 level); FP32-mixed within 2e-4 on smooth flow.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hydrodynamicsgpu_amd import config, ics
+
+pytestmark = pytest.mark.gpu
+
+
+def _mk(desc):
+    from hydrodynamicsgpu_amd import capi
+
+    return capi.Sim(desc)
+
+
+def _relerr(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def _rand_state(desc, seed=0):
+    rng = np.random.default_rng(seed)
+    x, y, z, _ = ics.grid(desc)
+    shp = x.shape
+    rho = rng.uniform(0.5, 2.0, shp)
+    p = rng.uniform(0.5, 2.0, shp)
+    v = [rng.uniform(-0.5, 0.5, shp) for _ in range(3)]
+    return ics.build_state_euler(desc, rho, v[0], v[1], v[2], p)
+
+
+HLL_CASES = [
+    ("EulerHLL", [96], "RungeKutta2TVD", "FREEFLOW"),
+    ("EulerHLLC", [96], "ForwardEuler", "MIRROR"),
+    ("EulerHLL", [70, 40], "RungeKutta4", "PERIODIC"),
+    ("EulerHLLC", [66, 50], "RungeKutta3TVD", "MIRROR"),
+    ("EulerHLLC", [40, 24, 20], "RungeKutta3TVD", "MIRROR"),
+    ("EulerHLL", [70, 20, 18], "RungeKutta2", "FREEFLOW"),
+    ("EulerHLLC", [36, 28, 30], "RungeKutta4", "PERIODIC"),
+]
+
+
+@pytest.mark.parametrize("solver,size,integ,bm", HLL_CASES)
+def test_hll_fp64_parity(gpu_lib, solver, size, integ, bm):
+    d = config.make_desc(size, solverName=solver, integratorName=integ,
+                         boundaryMethods=[{"min": bm, "max": bm}] * 3, cfl=0.4)
+    q = ics.sod(d) if len(size) == 1 else _rand_state(d, 7)
+    g = _mk(d)
+    g.set_state(q)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    for _ in range(6):
+        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+    assert _relerr(g.get_state(), o.get_state()) < 1e-10
+
+
+ROE3D_CASES = [
+    ([40, 24, 20], "RungeKutta2", "Superbee", "FREEFLOW"),
+    ([70, 22, 26], "RungeKutta4", "MonotizedCentral", "PERIODIC"),
+    ([30, 30, 30], "RungeKutta3TVD", "MinMod", "MIRROR"),
+    ([24, 20, 36], "ForwardEuler", "Koren", "FREEFLOW"),
+]
+
+
+@pytest.mark.parametrize("size,integ,lim,bm", ROE3D_CASES)
+def test_roe3d_fp64_parity(gpu_lib, size, integ, lim, bm):
+    d = config.make_desc(size, integratorName=integ, slopeLimiter=lim,
+                         boundaryMethods=[{"min": bm, "max": bm}] * 3, cfl=0.4)
+    q = _rand_state(d, 11)
+    g = _mk(d)
+    g.set_state(q)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    for _ in range(5):
+        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+    assert _relerr(g.get_state(), o.get_state()) < 1e-10
+
+
+@pytest.mark.parametrize("solver", ["EulerHLLC", "EulerRoe"])
+def test_3d_mixed_parity(gpu_lib, solver):
+    kw = dict(solverName=solver, integratorName="RungeKutta3TVD", slopeLimiter="MinMod",
+              boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4)
+    d64 = config.make_desc([36, 28, 30], **kw)
+    d32 = config.make_desc([36, 28, 30], precision="fp32_mixed",
+                           flags=config.FLAG_REFERENCE | config.FLAG_KAHAN, **kw)
+    q = ics.blast(d64)
+    o = oracle.OracleSim(d64)
+    o.set_state(q)
+    g = _mk(d32)
+    g.set_state(q)
+    for _ in range(4):
+        o.update()
+        g.update()
+    assert _relerr(g.get_state(), o.get_state()) < 5e-4
+
+
+def test_blast_3d_full_size_properties(gpu_lib):
+    # the 3D HLLC workload (256^3, mirror, TVD-RK3, CFL 0.4): FP32-mixed and
+    # FP64 stay finite and positive and agree to FP32 accuracy after 3 steps
+    kw = dict(xmin=[-1] * 3, xmax=[1] * 3, solverName="EulerHLLC", integratorName="RungeKutta3TVD",
+              slopeLimiter="MinMod", boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4)
+    d64 = config.make_desc([256] * 3, **kw)
+    d32 = config.make_desc([256] * 3, precision="fp32_mixed", flags=config.FLAG_REFERENCE | config.FLAG_KAHAN,
+                           **kw)
+    q = ics.blast(d64)
+    a = _mk(d64)
+    a.set_state(q)
+    b = _mk(d32)
+    b.set_state(q)
+    for _ in range(3):
+        a.update()
+        b.update()
+    qa = a.get_state().reshape(-1, 5)
+    qb = b.get_state().reshape(-1, 5)
+    assert np.all(np.isfinite(qa)) and np.all(qa[:, 0] > 0)
+    assert a.health() == 0 and b.health() == 0
+    assert _relerr(qb, qa) < 1e-4
+    # mirror symmetry of the centred blast: x -> -x maps the density onto itself
+    rho = qa[:, 0].reshape(256, 256, 256)[2:-2, 2:-2, 2:-2]
+    noise_free = np.abs(rho - rho[:, :, ::-1]).max()
+    assert noise_free < 1e-3  # only the 1e-4 seeded density noise breaks the symmetry
