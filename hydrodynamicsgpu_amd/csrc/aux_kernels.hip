@@ -1,6 +1,7 @@
 // aux_kernels.hip -- ghost fill, CFL reduction, HLL wave speeds, totals.
 #include "device_common.hpp"
 #include "phys_euler.hpp"
+#include "phys_srhd.hpp"
 #include "policies.hpp"
 #include "launch.hpp"
 
@@ -210,7 +211,26 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
       load_q<S, DIM>(u0, a.plane, cell - step[side], qm);
       load_q<S, DIM>(u0, a.plane, right_ok ? cell + step[side] : cell, qp);
       double d;
-      if (a.solver == 0) {
+      if (a.eqn == 1) {
+        // SRHD Roe: Marti-Mueller speeds of the mean primitives (res/SRHDRoe.cl:76-118)
+        using P = SRHDRoePolicy<DIM, double, 0>;
+        double pc[DIM + 2], pm[DIM + 2], pp[DIM + 2];
+#pragma unroll
+        for (int s = 0; s < DIM + 2; ++s) {
+          pc[s] = a.prim[s * a.plane + cell];
+          pm[s] = a.prim[s * a.plane + cell - step[side]];
+          pp[s] = a.prim[s * a.plane + cell + step[side]];
+        }
+        const auto fc = P::template frozen<double>(pc, a.gamma);
+        const auto mL = P::mean(P::template frozen<double>(pm, a.gamma), fc, side, a.gamma);
+        const auto mR = P::mean(fc, P::template frozen<double>(pp, a.gamma), side, a.gamma);
+        const double lmax = fmax(0.0, mL.lmax);
+        const double lmin = fmin(0.0, mR.lmin);
+        d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
+      } else if (a.eqn == 2) {
+        // Maxwell: constant speeds +-1/sqrt(eps mu) (res/MaxwellRoe.cl:139-146)
+        d = a.kdx[side] / (fabs(fmax(0.0, a.maxwell_lambda) - fmin(0.0, -a.maxwell_lambda)) + 1e-9);
+      } else if (a.solver == 0) {
         using P = EulerRoe<DIM, S>;
         const auto cc = P::template cell<S>(q, gamma);
         const auto fL = P::face(P::template cell<S>(qm, gamma), cc, side, gamma);
@@ -335,6 +355,75 @@ void launch_totals_f32(int dim, int sx, int sy, int sz, long long plane, int n, 
                        double* partial, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL((totals_kernel<float>), dim3(nblocks), dim3(256), 0, st, dim, sx, sy, sz, plane, n, u, lo,
                      partial);
+}
+
+}  // namespace hydro
+
+namespace hydro {
+
+// ---------------------------------------------------------------- SRHD
+// SRHDRoe::step tail (src/Solver/SRHDRoe.cpp:62-73): constrainState on every
+// cell (res/SRHDCommon.cl:94-109), then updatePrimitives on cells 2..S-2 of
+// every axis (res/SRHDCommon.cl:113-196), in FP64.
+template <typename S, int DIM>
+__global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, float* lo, double* prim) {
+  constexpr int N = DIM + 2, E = DIM + 1;
+  const long long vol = (long long)a.sx * a.sy * a.sz;
+  const SRHDDefs defs{a.gamma, a.max_iter, a.stop_eps, a.vel_eps, a.pmin_eps, a.rho_min, a.rho_max,
+                      a.eint_max, a.D_min, a.D_max, a.tau_min, a.tau_max};
+  bool fail = false;
+  for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < vol;
+       cell += (long long)gridDim.x * blockDim.x) {
+    double q[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      q[s] = double(u[s * a.plane + cell]);
+      if (lo) q[s] += double(lo[s * a.plane + cell]);
+    }
+    double D = fmin(fmax(q[0], a.D_min), a.D_max);
+    double tau = fmin(fmax(q[E], a.tau_min), a.tau_max);
+    if (D != q[0]) {
+      q[0] = D;
+      u[cell] = S(D);
+      if (lo) lo[cell] = float(D - double(S(D)));
+    }
+    if (tau != q[E]) {
+      q[E] = tau;
+      u[E * a.plane + cell] = S(tau);
+      if (lo) lo[E * a.plane + cell] = float(tau - double(S(tau)));
+    }
+    const int x = (int)(cell % a.sx);
+    const int y = (int)((cell / a.sx) % a.sy);
+    const int z = (int)(cell / ((long long)a.sx * a.sy));
+    bool act = x >= 2 && x <= a.sx - 2;
+    if (DIM > 1) act = act && y >= 2 && y <= a.sy - 2;
+    if (DIM > 2) act = act && z >= 2 && z <= a.sz - 2;
+    if (!act) continue;
+    double p[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) p[s] = prim[s * a.plane + cell];
+    if (srhd_con2prim<DIM>(defs, q, p)) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) prim[s * a.plane + cell] = p[s];
+    } else {
+      fail = true;
+    }
+  }
+  if (fail) atomicOr(a.health, 4u);
+}
+
+template <typename S>
+static void con2prim(const Con2PrimArgs& a, S* u, float* lo, double* prim, hipStream_t st) {
+  dim3 g(2048), b(256);
+  if (a.dim == 1) hipLaunchKernelGGL((con2prim_kernel<S, 1>), g, b, 0, st, a, u, lo, prim);
+  else if (a.dim == 2) hipLaunchKernelGGL((con2prim_kernel<S, 2>), g, b, 0, st, a, u, lo, prim);
+  else hipLaunchKernelGGL((con2prim_kernel<S, 3>), g, b, 0, st, a, u, lo, prim);
+}
+void launch_con2prim_f64(const Con2PrimArgs& a, double* u, double* prim, hipStream_t st) {
+  con2prim<double>(a, u, nullptr, prim, st);
+}
+void launch_con2prim_f32(const Con2PrimArgs& a, float* u, float* lo, double* prim, hipStream_t st) {
+  con2prim<float>(a, u, lo, prim, st);
 }
 
 }  // namespace hydro

@@ -182,7 +182,7 @@ struct hydro_sim {
   float* Ulo[2] = {nullptr, nullptr};
   void* D[2] = {nullptr, nullptr};
   void* P[kMaxPartials] = {nullptr, nullptr, nullptr};
-  void* speeds = nullptr;
+  double* prim = nullptr;  // SRHD primitive planes (FP64 in both precisions)
   double* blockmin = nullptr;
   double* partial = nullptr;
   DevScalars* sc = nullptr;
@@ -208,8 +208,9 @@ int validate(const hydro_sim_desc* d) {
     if (d->size[a] < 8) return fail(HYDRO_E_INVALID, "each used axis needs size >= 8 (2+2 ghosts + 4)");
   for (int a = d->dim; a < 3; ++a)
     if (d->size[a] != 1) return fail(HYDRO_E_INVALID, "unused axes must have size 1");
-  if (d->equation != HYDRO_EQN_EULER)
-    return fail(HYDRO_E_UNSUPPORTED, "this build of the HIP path implements the Euler equations");
+  if (d->equation < HYDRO_EQN_EULER || d->equation > HYDRO_EQN_MAXWELL) return fail(HYDRO_E_INVALID, "bad equation");
+  if (d->equation != HYDRO_EQN_EULER && d->solver != HYDRO_SOLVER_ROE)
+    return fail(HYDRO_E_UNSUPPORTED, "SRHD and Maxwell use the Roe solver (SRHDRoe, MaxwellRoe)");
   if (d->solver < HYDRO_SOLVER_ROE || d->solver > HYDRO_SOLVER_HLLC) return fail(HYDRO_E_INVALID, "bad solver");
   if (d->limiter < 0 || d->limiter >= HYDRO_NUM_LIMITERS) return fail(HYDRO_E_INVALID, "bad limiter");
   if (d->integrator < 0 || d->integrator >= HYDRO_NUM_INTEGRATORS) return fail(HYDRO_E_INVALID, "bad integrator");
@@ -238,7 +239,12 @@ BoundaryArgs boundary_args(const hydro_sim* s, int axis) {
       if (st < s->N) {
         switch (s->d.boundary[axis][mm]) {
         case HYDRO_BOUNDARY_PERIODIC: k = 0; break;
-        case HYDRO_BOUNDARY_MIRROR: k = (axis + 1 == st) ? 2 : 1; break;
+        case HYDRO_BOUNDARY_MIRROR:
+          if (s->d.equation == HYDRO_EQN_MAXWELL)  // src/Equation/Maxwell.cpp:80
+            k = (st == axis || st == 3 + axis) ? 2 : 1;
+          else  // src/Equation/Euler.cpp:70, src/Equation/SRHD.cpp:70
+            k = (axis + 1 == st) ? 2 : 1;
+          break;
         case HYDRO_BOUNDARY_FREEFLOW: k = 3; break;
         default: k = -1;
         }
@@ -254,8 +260,35 @@ int enqueue_boundary(hydro_sim* s) {
     if (s->mixed) launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->stream);
     else launch_boundary_f64(b, (double*)s->U[s->cur], s->stream);
   }
+  // SRHDRoe::boundary fills the primitives with the same kernels afterwards
+  // (src/Solver/SRHDRoe.cpp:79-96)
+  if (s->prim)
+    for (int axis = 0; axis < s->dim; ++axis) launch_boundary_f64(boundary_args(s, axis), s->prim, s->stream);
   HIP_TRY(hipGetLastError());
   return HYDRO_OK;
+}
+
+// kernel family: 0 Euler Roe, 1 Euler HLL, 2 Euler HLLC, 3 SRHD Roe, 4 Maxwell Roe
+int kernel_kind(const hydro_sim_desc& d) {
+  if (d.equation == HYDRO_EQN_SRHD) return 3;
+  if (d.equation == HYDRO_EQN_MAXWELL) return 4;
+  return d.solver;
+}
+
+// res/MaxwellRoe.cl:11, 30-35, 139: float-typed se/su and inverses, and the
+// eigenvalue from the 6-digit renderings of sqrt(eps), sqrt(mu)
+// (src/Equation/Maxwell.cpp:58-63)
+void maxwell_constants(const hydro_sim_desc& d, double mx[5]) {
+  const float sqrt_1_2 = 0.7071067811865475727373109293694142252206802368164f;
+  const double sp = kernel_dx(std::sqrt(d.maxwell_permittivity), true);
+  const double su = kernel_dx(std::sqrt(d.maxwell_permeability), true);
+  const float fse = (float)(sp * (double)sqrt_1_2);
+  const float fsu = (float)(su * (double)sqrt_1_2);
+  mx[0] = fse;
+  mx[1] = fsu;
+  mx[2] = 1.f / fse;
+  mx[3] = 1.f / fsu;
+  mx[4] = 1.0 / (sp * su);
 }
 
 int enqueue_dt(hydro_sim* s, bool advance_time) {
@@ -269,6 +302,11 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
     for (int i = 0; i < 3; ++i) a.kdx[i] = s->kdx[i];
     a.gamma = s->d.gamma;
     a.solver = s->d.solver;
+    a.eqn = s->d.equation;
+    a.prim = s->prim;
+    double mx[5];
+    maxwell_constants(s->d, mx);
+    a.maxwell_lambda = mx[4];
     a.nblocks = s->dt_blocks;
     a.blockmin = s->blockmin;
     if (s->mixed) launch_dt_euler_f32(a, (const float*)s->U[s->cur], s->stream);
@@ -320,6 +358,8 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   a.plane = s->plane;
   for (int i = 0; i < 3; ++i) a.inv_dx[i] = 1.0 / s->kdx[i];
   a.gamma = s->d.gamma;
+  a.prim = s->prim;
+  maxwell_constants(s->d, a.mx);
   a.lim = s->d.limiter;
   const int interior_x = s->sz[0] - 4;
   a.nstrips = (interior_x + 59) / 60;
@@ -333,7 +373,7 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
     a.nwaves = a.nstrips * a.nchunks;
   } else {
     // 3D: rows = z-planes per workgroup, nchunks = y-chunks, nwaves = workgroups
-    const int ry = stage3d_rows_per_block(s->d.solver);
+    const int ry = stage3d_rows_per_block(kernel_kind(s->d));
     a.rows = s->rows;
     a.nchunks = (s->sz[1] - 4 + ry - 1) / ry;
     const int nz = (s->sz[2] - 4 + a.rows - 1) / a.rows;
@@ -353,13 +393,13 @@ int enqueue_stages(hydro_sim* s) {
     if (s->mixed) {
       StageArgs<float, float> a{};
       fill_stage_args<float>(s, k, a);
-      if (s->dim == 3) launch_stage3d_mixed(s->d.solver, s->d.limiter, s->kahan, a, s->stream);
-      else launch_stage2d_mixed(s->dim, s->d.solver, s->d.limiter, s->kahan, a, s->wpb, s->stream);
+      if (s->dim == 3) launch_stage3d_mixed(kernel_kind(s->d), s->d.limiter, s->kahan, a, s->stream);
+      else launch_stage2d_mixed(s->dim, kernel_kind(s->d), s->d.limiter, s->kahan, a, s->wpb, s->stream);
     } else {
       StageArgs<double, double> a{};
       fill_stage_args<double>(s, k, a);
-      if (s->dim == 3) launch_stage3d_f64(s->d.solver, s->d.limiter, a, s->stream);
-      else launch_stage2d_f64(s->dim, s->d.solver, s->d.limiter, a, s->wpb, s->stream);
+      if (s->dim == 3) launch_stage3d_f64(kernel_kind(s->d), s->d.limiter, a, s->stream);
+      else launch_stage2d_f64(s->dim, kernel_kind(s->d), s->d.limiter, a, s->wpb, s->stream);
     }
     HIP_TRY(hipGetLastError());
     if (s->profiling) {
@@ -384,9 +424,35 @@ int enqueue_stages(hydro_sim* s) {
   s->cur = nxt;
   // ForwardEuler's zero-gravity pass refreshes the ghosts after the update
   // (src/Solver/SelfGravitation.cpp:122 + src/Integrator/ForwardEuler.cpp:29-31)
-  if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER) {
+  if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER &&
+      s->d.equation == HYDRO_EQN_EULER) {
     int rc = enqueue_boundary(s);
     if (rc) return rc;
+  }
+  // SRHDRoe::step tail: constrainState + updatePrimitives (src/Solver/SRHDRoe.cpp:62-73)
+  if (s->d.equation == HYDRO_EQN_SRHD) {
+    Con2PrimArgs c{};
+    c.dim = s->dim;
+    c.sx = s->sz[0];
+    c.sy = s->sz[1];
+    c.sz = s->sz[2];
+    c.plane = s->plane;
+    c.gamma = s->d.gamma;
+    c.max_iter = s->d.srhd_solve_prim_max_iter;
+    c.stop_eps = s->d.srhd_solve_prim_stop_epsilon;
+    c.vel_eps = s->d.srhd_solve_prim_vel_epsilon;
+    c.pmin_eps = s->d.srhd_solve_prim_pmin_epsilon;
+    c.rho_min = s->d.srhd_rho_min;
+    c.rho_max = s->d.srhd_rho_max;
+    c.eint_max = s->d.srhd_eint_max;
+    c.D_min = s->d.srhd_D_min;
+    c.D_max = s->d.srhd_D_max;
+    c.tau_min = s->d.srhd_tau_min;
+    c.tau_max = s->d.srhd_tau_max;
+    c.health = &s->sc->health;
+    if (s->mixed) launch_con2prim_f32(c, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->stream);
+    else launch_con2prim_f64(c, (double*)s->U[s->cur], s->prim, s->stream);
+    HIP_TRY(hipGetLastError());
   }
   return HYDRO_OK;
 }
@@ -467,7 +533,7 @@ void hydro_sim_destroy(hydro_sim* s) {
   }
   for (int j = 0; j < kMaxPartials; ++j)
     if (s->P[j]) hipFree(s->P[j]);
-  if (s->speeds) hipFree(s->speeds);
+  if (s->prim) hipFree(s->prim);
   if (s->blockmin) hipFree(s->blockmin);
   if (s->partial) hipFree(s->partial);
   if (s->sc) hipFree(s->sc);
@@ -485,7 +551,7 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   s->ctx = ctx;
   s->d = *d;
   s->dim = d->dim;
-  s->N = d->dim + 2;
+  s->N = d->equation == HYDRO_EQN_MAXWELL ? 6 : d->dim + 2;
   for (int a = 0; a < 3; ++a) s->sz[a] = a < d->dim ? d->size[a] : 1;
   s->vol = (long long)s->sz[0] * s->sz[1] * s->sz[2];
   s->plane = (s->vol + 63) & ~63LL;
@@ -511,7 +577,8 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     s->rows = rows;
   } else if (s->dim == 3) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
-    const long long ych = (s->sz[1] - 4 + stage3d_rows_per_block(d->solver) - 1) / stage3d_rows_per_block(d->solver);
+    const int ry = stage3d_rows_per_block(kernel_kind(*d));
+    const long long ych = (s->sz[1] - 4 + ry - 1) / ry;
     int rows = 64;
     while (rows > 8 && strips * ych * ((s->sz[2] - 4 + rows - 1) / rows) < 1024) rows >>= 1;
     s->rows = rows;
@@ -544,6 +611,11 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     if (hipMalloc(&s->P[j], bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (partials)");
     hipMemset(s->P[j], 0, bytes);
   }
+  if (d->equation == HYDRO_EQN_SRHD) {
+    if (hipMalloc((void**)&s->prim, sizeof(double) * s->plane * s->N) != hipSuccess)
+      return cleanup(HYDRO_E_HIP, "out of device memory (primitives)");
+    hipMemset(s->prim, 0, sizeof(double) * s->plane * s->N);
+  }
   if (hipMalloc((void**)&s->blockmin, sizeof(double) * s->dt_blocks) != hipSuccess)
     return cleanup(HYDRO_E_HIP, "alloc");
   if (hipMalloc((void**)&s->partial, sizeof(double) * s->tot_blocks * 8) != hipSuccess)
@@ -567,6 +639,36 @@ int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   const int N = s->N;
   const long long V = s->vol;
+  std::vector<double> conv;
+  if (s->d.equation == HYDRO_EQN_SRHD) {
+    // initVariables (res/SRHDCommon.cl:24-92): Newtonian (rho, rho v, E) ->
+    // conserved (D, S, tau) and primitives (rho, v, eInt), every cell, FP64
+    conv.resize((size_t)N * V);
+    std::vector<double> pr((size_t)s->plane * N, 0.0);
+    const double gamma = s->d.gamma;
+    const int D = s->dim, E = D + 1;
+    for (long long c = 0; c < V; ++c) {
+      const double* st = aos + (size_t)N * c;
+      double* out = conv.data() + (size_t)N * c;
+      const double rho = st[0];
+      double v[3] = {0, 0, 0};
+      for (int a = 0; a < D; ++a) v[a] = st[1 + a] / st[0];
+      const double vSq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      const double eInt = st[E] / rho - .5 * vSq;
+      const double P = (gamma - 1.) * rho * eInt;
+      const double h = 1. + eInt + P / rho;
+      const double WSq = 1. / (1. - vSq);
+      const double W = std::sqrt(WSq);
+      out[0] = rho * W;
+      for (int a = 0; a < D; ++a) out[1 + a] = rho * h * WSq * v[a];
+      out[E] = rho * h * WSq - P - rho * W;
+      pr[c] = rho;
+      for (int a = 0; a < D; ++a) pr[(size_t)(1 + a) * s->plane + c] = v[a];
+      pr[(size_t)E * s->plane + c] = eInt;
+    }
+    HIP_TRY(hipMemcpy(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
+    aos = conv.data();
+  }
   if (s->mixed) {
     std::vector<float> hi((size_t)s->plane * N, 0.f), lo((size_t)s->plane * N, 0.f);
     for (long long c = 0; c < V; ++c)
@@ -619,10 +721,15 @@ int hydro_sim_get_state(hydro_sim* s, double* aos, size_t n) {
 }
 
 int hydro_sim_get_primitives(hydro_sim* s, double* aos, size_t n) {
-  (void)aos;
-  (void)n;
-  if (!s) return fail(HYDRO_E_INVALID, "null argument");
-  return fail(HYDRO_E_UNSUPPORTED, "primitives exist for SRHD only");
+  if (!s || !aos) return fail(HYDRO_E_INVALID, "null argument");
+  if (!s->prim) return fail(HYDRO_E_UNSUPPORTED, "primitives exist for SRHD only");
+  if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "size must be N * volume");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<double> buf((size_t)s->plane * s->N);
+  HIP_TRY(hipMemcpy(buf.data(), s->prim, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (long long c = 0; c < s->vol; ++c)
+    for (int k = 0; k < s->N; ++k) aos[k + (size_t)s->N * c] = buf[(size_t)k * s->plane + c];
+  return HYDRO_OK;
 }
 
 int hydro_sim_boundary(hydro_sim* s) {

@@ -95,6 +95,15 @@ HYDRO_DEV T limiter(int lim_rt, T r) {
   else return limiter_eval<T>(lim_rt, r);
 }
 
+// ---------------------------------------------------------------- parameters
+// Runtime physical constants of a face solver (the reference bakes them into
+// the program source as `defs`, src/Solver/Solver.cpp:289-307).
+template <typename T>
+struct PhysParams {
+  T gamma;
+  T mx[5];  // Maxwell: se, su, ise, isu, eigenvalue (res/MaxwellRoe.cl:30-35, 139)
+};
+
 // ---------------------------------------------------------------- indexing
 struct Grid {
   int sx, sy, sz;       // sizes incl. ghosts

@@ -44,9 +44,27 @@ struct DtArgs {
   double kdx[3];
   double gamma;
   int solver;            // 0 Roe, 1 HLL, 2 HLLC
+  int eqn;               // 0 Euler, 1 SRHD, 2 Maxwell
+  const double* prim;    // SRHD primitive planes
+  double maxwell_lambda; // Maxwell: 1/sqrt(eps mu)
   int nblocks;
   double* blockmin;      // [nblocks]
 };
+
+// SRHD: constrainState on every cell, then con2prim (FP64) on cells 2..S-2
+// (res/SRHDCommon.cl:94-196); health bit 2 on non-convergence
+struct Con2PrimArgs {
+  int dim, sx, sy, sz;
+  long long plane;
+  double gamma;
+  int max_iter;
+  double stop_eps, vel_eps, pmin_eps;
+  double rho_min, rho_max, eint_max;
+  double D_min, D_max, tau_min, tau_max;
+  unsigned* health;
+};
+void launch_con2prim_f64(const Con2PrimArgs& a, double* u, double* prim, hipStream_t st);
+void launch_con2prim_f32(const Con2PrimArgs& a, float* u, float* lo, double* prim, hipStream_t st);
 void launch_dt_euler_f64(const DtArgs& a, const double* u0, hipStream_t st);
 void launch_dt_euler_f32(const DtArgs& a, const float* u0, hipStream_t st);
 

@@ -20,6 +20,7 @@ struct RoePolicy {
   using P = EulerRoe<DIM, T>;
   static constexpr int N = P::N;
   static constexpr bool kNeighbors = true;
+  static constexpr bool kFrozenPrims = false;
   using Frozen = typename P::Cell;
   static constexpr int NF = DIM + 2;  // w, v[DIM], H
 
@@ -50,7 +51,8 @@ struct RoePolicy {
 
   // res/Roe.cl:77-150 (deltaQTilde) and the L*qavg half of res/Roe.cl:264-270
   static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const T uL[N], const T uR[N], int side,
-                                bool valid, T gamma) {
+                                bool valid, const PhysParams<T>& pp) {
+    const T gamma = pp.gamma;
     FaceA a;
     a.f = P::face(cL, cR, side, gamma);
     T dlt[N], avg[N];
@@ -97,6 +99,7 @@ struct HLLPolicy {
   static constexpr int N = DIM + 2;
   static constexpr int E = DIM + 1;
   static constexpr bool kNeighbors = false;
+  static constexpr bool kFrozenPrims = false;
   struct Frozen {
     T rho, p, cs, w, H;
     T v[DIM];
@@ -167,7 +170,8 @@ struct HLLPolicy {
 
   // res/EulerHLL.cl:185-343, res/EulerHLLC.cl:178-419 (HLLC_METHOD 2)
   static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const T uL_[N], const T uR_[N], int side,
-                                bool valid, T gamma) {
+                                bool valid, const PhysParams<T>& pp) {
+    const T gamma = pp.gamma;
     (void)valid;
     FaceA a;
     T sl, sr;
@@ -224,15 +228,16 @@ struct HLLPolicy {
     } else {
       const T sStar = tdiv(dR * vR[0] * (sr - vR[0]) - dL * vL[0] * (sl - vL[0]) + pL - pR,
                            dR * (sr - vR[0]) - dL * (sl - vL[0]));
+      // branch order of res/EulerHLLC.cl:276-364
       if (T(0) <= sl) {
 #pragma unroll
         for (int s = 0; s < N; ++s) F[s] = fL[s];
-      } else if (sr <= T(0)) {
+      } else if (!(sStar <= T(0) && T(0) <= sr) && !(T(0) <= sStar) && sr <= T(0)) {
 #pragma unroll
         for (int s = 0; s < N; ++s) F[s] = fR[s];
       } else {
         const T pLR = T(.5) * (pL + pR + dL * (sl - vL[0]) * (sStar - vL[0]) + dR * (sr - vR[0]) * (sStar - vR[0]));
-        const bool left = sStar >= T(0);
+        const bool left = T(0) <= sStar;
         const T sK = left ? sl : sr;
         const T inv = trcp(sK - sStar);
 #pragma unroll

@@ -25,6 +25,8 @@
 #include "device_common.hpp"
 #include "launch.hpp"
 #include "policies.hpp"
+#include "phys_srhd.hpp"
+#include "phys_maxwell.hpp"
 #include "stage_common.hpp"
 
 namespace hydro {
@@ -54,6 +56,7 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
   const bool col_face = c >= 2 && c <= sx - 2;  // the reference writes faces here
 
   const T gamma = T(a.gamma);
+  const PhysParams<T> pp = make_params<T>(a);
   const double dtd = *a.dt;
   const T dt_dx0 = T(dtd * a.inv_dx[0]);
   const T idx0 = T(a.inv_dx[0]);
@@ -68,7 +71,7 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
 #pragma unroll
     for (int i = 0; i < Pol::NF; ++i) pkL[i] = lane_from_left(pk[i]);
     const Frozen fL = Pol::unpack(pkL);
-    const FaceA fa = Pol::face_a(fL, fz, uL, u, 0, col_face, gamma);
+    const FaceA fa = Pol::face_a(fL, fz, uL, u, 0, col_face, pp);
     T dqlo[N], dqhi[N], F[N];
     if constexpr (Pol::kNeighbors) {
 #pragma unroll
@@ -89,7 +92,14 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
     const long long cell = (long long)row * sx + cl;
     S q0[N];
     comb.load_state(cell, q0, u);
-    fz = Pol::template frozen<S>(q0, gamma);
+    if constexpr (Pol::kFrozenPrims) {
+      double pr[N];
+#pragma unroll
+      for (int s = 0; s < N; ++s) pr[s] = a.prim[s * a.plane + cell];
+      fz = Pol::template frozen<double>(pr, gamma);
+    } else {
+      fz = Pol::template frozen<S>(q0, gamma);
+    }
   };
 
   if constexpr (DIM == 1) {
@@ -137,7 +147,7 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
 
       // first pass of y-face r (between rows r-1 and r)
       const bool ok = col_face && r >= 2 && r <= sy - 2;
-      const FaceA fa = Pol::face_a(f_prev, fz, u_prev, u, 1, ok, gamma);
+      const FaceA fa = Pol::face_a(f_prev, fz, u_prev, u, 1, ok, pp);
 
       T Fy[N];
       if (r >= y0 + 1) {
@@ -175,12 +185,26 @@ static void launch_pol(const StageArgs<T, S>& a, int wpb, hipStream_t st) {
   hipLaunchKernelGGL((stage2d_kernel<Pol, DIM, T, S, KAHAN>), dim3(blocks), dim3(64 * wpb), 0, st, a);
 }
 
+// kind: 0 Euler Roe, 1 Euler HLL, 2 Euler HLLC, 3 SRHD Roe, 4 Maxwell Roe
 template <int DIM, typename T, typename S, bool KAHAN>
-static void launch_solver(int solver, int lim, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
-  if (solver == 1) {
+static void launch_solver(int kind, int lim, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
+  if (kind == 1) {
     launch_pol<HLLPolicy<DIM, T, 1>, DIM, T, S, KAHAN>(a, wpb, st);
-  } else if (solver == 2) {
+  } else if (kind == 2) {
     launch_pol<HLLPolicy<DIM, T, 2>, DIM, T, S, KAHAN>(a, wpb, st);
+  } else if (kind == 3) {
+    switch (lim) {
+    case LIM_SUPERBEE: launch_pol<SRHDRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN>(a, wpb, st); break;
+    case LIM_MONOTIZEDCENTRAL:
+      launch_pol<SRHDRoePolicy<DIM, T, LIM_MONOTIZEDCENTRAL>, DIM, T, S, KAHAN>(a, wpb, st);
+      break;
+    default: launch_pol<SRHDRoePolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN>(a, wpb, st); break;
+    }
+  } else if (kind == 4) {
+    switch (lim) {
+    case LIM_SUPERBEE: launch_pol<MaxwellRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN>(a, wpb, st); break;
+    default: launch_pol<MaxwellRoePolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN>(a, wpb, st); break;
+    }
   } else {
     switch (lim) {
     case LIM_SUPERBEE: launch_pol<RoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN>(a, wpb, st); break;

@@ -10,6 +10,8 @@
 #include "device_common.hpp"
 #include "launch.hpp"
 #include "policies.hpp"
+#include "phys_srhd.hpp"
+#include "phys_maxwell.hpp"
 #include "stage_common.hpp"
 
 namespace hydro {
@@ -50,6 +52,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   const bool row_face = y >= 2 && y <= sy - 2;
 
   const T gamma = T(a.gamma);
+  const PhysParams<T> pp = make_params<T>(a);
   const double dtd = *a.dt;
   const T dt_dx = T(dtd * a.inv_dx[0]), dt_dy = T(dtd * a.inv_dx[1]), dt_dz = T(dtd * a.inv_dx[2]);
   const T idx = T(a.inv_dx[0]), idy = T(a.inv_dx[1]), idz = T(a.inv_dx[2]);
@@ -60,7 +63,14 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
     const long long cell = (long long)z * psz + (long long)yl * sx + cl;
     S q0[N];
     comb.load_state(cell, q0, u);
-    fz = Pol::template frozen<S>(q0, gamma);
+    if constexpr (Pol::kFrozenPrims) {
+      double pr[N];
+#pragma unroll
+      for (int s = 0; s < N; ++s) pr[s] = a.prim[s * a.plane + cell];
+      fz = Pol::template frozen<double>(pr, gamma);
+    } else {
+      fz = Pol::template frozen<S>(q0, gamma);
+    }
   };
 
   const int z0 = 2 + zch * a.rows;
@@ -97,7 +107,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
       Pol::pack(fz, pk);
 #pragma unroll
       for (int i = 0; i < NF; ++i) pkL[i] = lane_from_left(pk[i]);
-      const FaceA fa = Pol::face_a(Pol::unpack(pkL), fz, uL, u, 0, col_face, gamma);
+      const FaceA fa = Pol::face_a(Pol::unpack(pkL), fz, uL, u, 0, col_face, pp);
       T dqlo[N], dqhi[N], F[N];
       if constexpr (Pol::kNeighbors) {
 #pragma unroll
@@ -130,7 +140,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
 #pragma unroll
       for (int i = 0; i < NF; ++i) pkS[i] = shA[buf][ws][N + i][lane];
       const bool ok = w > 0 && col_face && row_face && zn >= 2 && zn <= sz - 2;
-      fy = Pol::face_a(Pol::unpack(pkS), fz, uS, u, 1, ok, gamma);
+      fy = Pol::face_a(Pol::unpack(pkS), fz, uS, u, 1, ok, pp);
     }
     T Fy[N];
     if constexpr (Pol::kNeighbors) {
@@ -161,7 +171,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
     // z-direction in registers: first pass of face zn, flux of face zn-1,
     // then cells of plane zn-2
     const bool okz = col_face && row_face && zn >= 2 && zn <= sz - 2;
-    const FaceA fa = Pol::face_a(f_prev, fz, u_prev, u, 2, okz, gamma);
+    const FaceA fa = Pol::face_a(f_prev, fz, u_prev, u, 2, okz, pp);
     T Fz[N];
     if (zn >= z0 + 1) {
       Pol::face_b(fa_m1, fa_m2.dq, fa.dq, dt_dz, 2, a.lim, Fz);
@@ -195,11 +205,15 @@ static void launch3(const StageArgs<T, S>& a, hipStream_t st) {
 }
 
 template <typename T, typename S, bool KAHAN, int W>
-static void launch3_solver(int solver, int lim, const StageArgs<T, S>& a, hipStream_t st) {
-  if (solver == 1) {
+static void launch3_solver(int kind, int lim, const StageArgs<T, S>& a, hipStream_t st) {
+  if (kind == 1) {
     launch3<HLLPolicy<3, T, 1>, T, S, KAHAN, W>(a, st);
-  } else if (solver == 2) {
+  } else if (kind == 2) {
     launch3<HLLPolicy<3, T, 2>, T, S, KAHAN, W>(a, st);
+  } else if (kind == 3) {
+    launch3<SRHDRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(a, st);
+  } else if (kind == 4) {
+    launch3<MaxwellRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(a, st);
   } else {
     switch (lim) {
     case LIM_SUPERBEE: launch3<RoePolicy<3, T, LIM_SUPERBEE>, T, S, KAHAN, W>(a, st); break;
@@ -210,7 +224,8 @@ static void launch3_solver(int solver, int lim, const StageArgs<T, S>& a, hipStr
   }
 }
 
-int stage3d_rows_per_block(int solver) { return kStage3dWaves - 2 * (solver == 0 ? 2 : 1); }
+// halo rows: 2 for the Roe family (limiter neighbours), 1 for HLL/HLLC
+int stage3d_rows_per_block(int kind) { return kStage3dWaves - 2 * ((kind == 1 || kind == 2) ? 1 : 2); }
 
 void launch_stage3d_f64(int solver, int lim, const StageArgs<double, double>& a, hipStream_t st) {
   launch3_solver<double, double, false, kStage3dWaves>(solver, lim, a, st);

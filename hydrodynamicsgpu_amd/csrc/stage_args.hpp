@@ -15,6 +15,7 @@ constexpr int kMaxPartials = 3;
 template <typename T, typename S>
 struct StageArgs {
   const S* u0;          // start-of-step state planes (FP64, or FP32 high words)
+  const double* prim;   // SRHD: start-of-step primitive planes (FP64), else null
   const float* u0lo;    // FP32 low words (compensated storage), or null
   const T* dk;          // d_k planes, null for k == 0
   T* dnext;             // d_{k+1} planes (non-final stage)
@@ -34,6 +35,7 @@ struct StageArgs {
   long long plane;           // elements between state planes
   double inv_dx[3];          // 1 / kernel dx per axis
   double gamma;
+  double mx[5];              // Maxwell constants: se, su, ise, isu, eigenvalue
   int lim;                   // runtime limiter id (generic instantiation)
   int nstrips, rows, nchunks;  // wave -> (strip, chunk) decomposition
   int nwaves;

@@ -8,6 +8,15 @@ namespace hydro {
 
 constexpr int kStripOut = 60;  // output columns per wave (64 lanes - 2*2 halo)
 
+template <typename T, typename S>
+HYDRO_DEV PhysParams<T> make_params(const StageArgs<T, S>& a) {
+  PhysParams<T> p;
+  p.gamma = T(a.gamma);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) p.mx[i] = T(a.mx[i]);
+  return p;
+}
+
 template <typename T, typename S, bool KAHAN, int N>
 struct StageCombine {
   const StageArgs<T, S>& a;
