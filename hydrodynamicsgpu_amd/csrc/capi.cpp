@@ -97,7 +97,49 @@ struct StagePlan {
   double bm[kMaxPartials] = {0, 0, 0};
   double a_self = 0, b_self = 0;
   bool final_stage = false;
+  int din = -1, dout = -1;  // increment buffers read (stencil) and written
+  int nrep = 0;
+  int rep[kMaxPartials] = {0, 0, 0};
+  double rc[kMaxPartials] = {0, 0, 0};
 };
+
+// Replay plan: valid when every row before the last is d_{k+1} = beta[k][k]
+// dt L_k + (alpha[k][0] u0 part), i.e. the rows do not mix earlier stages
+// (classic RK4, RK2 midpoint, Heun, Ralston).  Then dt L_k = d_{k+1} /
+// beta[k][k] exactly up to one rounding, the last row is a combination of the
+// stored increments, and no partial-sum buffer is read or written.
+bool plan_replay(const Tableau& t, std::vector<StagePlan>& out) {
+  const int S = t.order;
+  if (S < 2 || S - 1 > kMaxPartials) return false;
+  for (int k = 0; k < S; ++k) {
+    for (int j = 1; j < S; ++j)
+      if (t.a[k][j] != 0.0) return false;
+    if (k < S - 1) {
+      if (t.b[k][k] == 0.0) return false;
+      for (int j = 0; j < k; ++j)
+        if (t.b[k][j] != 0.0) return false;
+    }
+  }
+  out.assign(S, StagePlan());
+  for (int k = 0; k < S; ++k) {
+    StagePlan& p = out[k];
+    p.din = k - 1;
+    p.dout = k < S - 1 ? k : -1;
+    p.b_self = t.b[k][k];
+    p.final_stage = k == S - 1;
+    if (p.final_stage) {
+      // d_S = sum_{j<S-1} beta[S-1][j] / beta[j][j] d_{j+1} + beta[S-1][S-1] dt L_{S-1}
+      p.a_self = t.b[S - 1][S - 2] / t.b[S - 2][S - 2];  // coefficient of the stencil input d_{S-1}
+      for (int j = 0; j < S - 2; ++j) {
+        if (t.b[S - 1][j] == 0.0) continue;
+        p.rep[p.nrep] = j;  // buffer holding d_{j+1}
+        p.rc[p.nrep] = t.b[S - 1][j] / t.b[j][j];
+        p.nrep++;
+      }
+    }
+  }
+  return true;
+}
 
 bool plan_stages(const Tableau& t, std::vector<StagePlan>& out, int& nslots) {
   const int S = t.order;
@@ -112,6 +154,8 @@ bool plan_stages(const Tableau& t, std::vector<StagePlan>& out, int& nslots) {
     p.a_self = t.a[k][k];
     p.b_self = t.b[k][k];
     p.final_stage = (k == S - 1);
+    p.din = k > 0 ? (k - 1) & 1 : -1;  // ping-pong increment buffers
+    p.dout = k < S - 1 ? k & 1 : -1;
     // row k's slot is consumed in this stage; the same thread reads it before
     // any write below, so the slot may be reused at once
     if (live[k]) used[slot_of[k]] = false;
@@ -180,7 +224,9 @@ struct hydro_sim {
   // device buffers (element type double for FP64, float for FP32-mixed)
   void* U[2] = {nullptr, nullptr};
   float* Ulo[2] = {nullptr, nullptr};
-  void* D[2] = {nullptr, nullptr};
+  void* D[3] = {nullptr, nullptr, nullptr};  // stage increments
+  int ndbuf = 0;
+  bool replay = false;
   void* P[kMaxPartials] = {nullptr, nullptr, nullptr};
   double* prim = nullptr;  // SRHD primitive planes (FP64 in both precisions)
   double* blockmin = nullptr;
@@ -335,8 +381,13 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   const StagePlan& p = s->plan[k];
   a.u0 = (const T*)s->U[s->cur];
   a.u0lo = s->kahan ? s->Ulo[s->cur] : nullptr;
-  a.dk = (k == 0) ? nullptr : (const T*)s->D[(k - 1) & 1];
-  a.dnext = (T*)s->D[k & 1];
+  a.dk = p.din < 0 ? nullptr : (const T*)s->D[p.din];
+  a.dnext = p.dout < 0 ? nullptr : (T*)s->D[p.dout];
+  a.nrep = p.nrep;
+  for (int j = 0; j < kMaxPartials; ++j) {
+    a.rep[j] = j < p.nrep ? (const T*)s->D[p.rep[j]] : nullptr;
+    a.rc[j] = p.rc[j];
+  }
   a.unew = (T*)s->U[s->cur ^ 1];
   a.unewlo = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
   for (int j = 0; j < kMaxPartials; ++j) {
@@ -529,8 +580,9 @@ void hydro_sim_destroy(hydro_sim* s) {
   for (int i = 0; i < 2; ++i) {
     if (s->U[i]) hipFree(s->U[i]);
     if (s->Ulo[i]) hipFree(s->Ulo[i]);
-    if (s->D[i]) hipFree(s->D[i]);
   }
+  for (int i = 0; i < 3; ++i)
+    if (s->D[i]) hipFree(s->D[i]);
   for (int j = 0; j < kMaxPartials; ++j)
     if (s->P[j]) hipFree(s->P[j]);
   if (s->prim) hipFree(s->prim);
@@ -564,7 +616,15 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     s->kdx[a] = kernel_dx(dx, (d->flags & HYDRO_FLAG_REF_DX_6DIGITS) != 0);
   }
   s->cfl = (double)(float)d->cfl;  // HydroGPUApp.h:84 stores cfl as float
-  if (!get_tableau(d->integrator, s->tab) || !plan_stages(s->tab, s->plan, s->nslots)) {
+  const char* plan_env = getenv("HYDRO_RK_PLAN");
+  const bool allow_replay = !(plan_env && std::string(plan_env) == "partial");
+  if (get_tableau(d->integrator, s->tab) && allow_replay && plan_replay(s->tab, s->plan)) {
+    s->replay = true;
+    s->nslots = 0;
+    s->ndbuf = s->tab.order - 1;
+  } else if (get_tableau(d->integrator, s->tab) && plan_stages(s->tab, s->plan, s->nslots)) {
+    s->ndbuf = s->tab.order > 1 ? 2 : 0;
+  } else {
     delete s;
     return fail(HYDRO_E_INVALID, "integrator not supported");
   }
@@ -602,10 +662,11 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
         return cleanup(HYDRO_E_HIP, "out of device memory (state low words)");
       hipMemset(s->Ulo[i], 0, sizeof(float) * s->plane * s->N);
     }
-    if (s->tab.order > 1) {
-      if (hipMalloc(&s->D[i], bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (stages)");
-      if (hipMemset(s->D[i], 0, bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "memset");
-    }
+  }
+  // increment buffers: their ghost cells stay 0 (stages write interior cells only)
+  for (int i = 0; i < s->ndbuf; ++i) {
+    if (hipMalloc(&s->D[i], bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (stages)");
+    if (hipMemset(s->D[i], 0, bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "memset");
   }
   for (int j = 0; j < s->nslots; ++j) {
     if (hipMalloc(&s->P[j], bytes) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (partials)");

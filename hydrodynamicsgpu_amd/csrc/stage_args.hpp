@@ -29,6 +29,11 @@ struct StageArgs {
   double am[kMaxPartials];   // alpha[m][k] per future row
   double bm[kMaxPartials];   // beta[m][k] per future row
   double a_self, b_self;     // alpha[k][k], beta[k][k]
+  // replayed increments: acc += rc[j] * rep[j] (tableaus whose earlier rows
+  // each depend on one L_k only, so dt L_k = d_{k+1} / beta[k][k])
+  int nrep;
+  const T* rep[kMaxPartials];
+  double rc[kMaxPartials];
   int final_stage;
   const double* dt;          // device-resident step size
   int sx, sy, sz;

@@ -46,6 +46,9 @@ struct StageCombine {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       T v = (a.pk >= 0) ? a.part[a.pk][s * pl + cell] : T(0);
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j)
+        if (j < a.nrep) v += T(a.rc[j]) * a.rep[j][s * pl + cell];
       if (a.dk) v += a_self * dk[s];
       acc[s] = v + bdt_self * deriv[s];
     }

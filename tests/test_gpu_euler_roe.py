@@ -87,7 +87,7 @@ CASES = [
     ([128], "RungeKutta2TVD", "Superbee", "FREEFLOW"),
     ([128], "ForwardEuler", "MinMod", "MIRROR"),
     ([100], "RungeKutta4", "MonotizedCentral", "PERIODIC"),
-    ([100], "RungeKutta3TVD", "VanAlbada1", "FREEFLOW"),
+    ([100], "RungeKutta3TVD", "Smart", "FREEFLOW"),
     ([70, 40], "RungeKutta4", "MonotizedCentral", "PERIODIC"),
     ([130, 37], "RungeKutta2", "Superbee", "FREEFLOW"),
     ([66, 70], "RungeKutta3TVD", "MinMod", "MIRROR"),
@@ -116,6 +116,50 @@ def test_fp64_parity(gpu_lib, size, integ, lim, bm):
         do = o.update()
         assert dg == pytest.approx(do, rel=1e-11)
     assert _relerr(g.get_state(), o.get_state()) < 1e-10
+
+
+@pytest.mark.parametrize("integ", ["RungeKutta4", "RungeKutta2", "RungeKutta2Heun", "RungeKutta2Ralston"])
+def test_rk_plans_agree(gpu_lib, integ, monkeypatch):
+    # the increment-replay plan (no partial sums) and the generic partial-sum
+    # plan evaluate the same tableau; both match the oracle
+    d = config.make_desc([70, 40], integratorName=integ, slopeLimiter="MonotizedCentral",
+                         boundaryMethods=[{"min": "PERIODIC", "max": "PERIODIC"}] * 2)
+    q = ics.shear_kh(d)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    out = {}
+    for plan in ("replay", "partial"):
+        monkeypatch.setenv("HYDRO_RK_PLAN", plan)
+        g = _mk(d)
+        g.set_state(q)
+        for _ in range(4):
+            g.update()
+        out[plan] = g.get_state()
+    for _ in range(4):
+        o.update()
+    ref = o.get_state()
+    assert _relerr(out["replay"], ref) < 1e-10
+    assert _relerr(out["partial"], ref) < 1e-10
+
+
+@pytest.mark.parametrize("lim", ["VanAlbada1", "VanLeer"])
+def test_nan_unsafe_limiters_fail_like_the_reference(gpu_lib, lim):
+    # rational limiters return NaN for r = 0/0 (every uniform region), so the
+    # reference's flux goes NaN; the oracle and the GPU agree on that, and the
+    # GPU raises its health flag
+    d = config.make_desc([100], integratorName="RungeKutta2TVD", slopeLimiter=lim, cfl=0.4)
+    q = ics.sod(d)
+    g = _mk(d)
+    g.set_state(q)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    for _ in range(3):
+        g.update()
+        o.update()
+    a, b = g.get_state(), o.get_state()
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    assert np.isnan(b).any()
+    assert g.health() & 1
 
 
 def test_run_matches_update(gpu_lib):
