@@ -40,13 +40,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--kahan", type=int, default=1, help="compensated FP32 state storage (1) or plain FP32 (0)")
     return ap.parse_args()
 
 
-def make_desc(n, precision):
+def make_desc(n, precision, kahan=True):
     from hydrodynamicsgpu_amd import config
 
-    flags = config.FLAG_REFERENCE | (config.FLAG_KAHAN if precision == "fp32_mixed" else 0)
+    flags = config.FLAG_REFERENCE | (config.FLAG_KAHAN if precision == "fp32_mixed" and kahan else 0)
     return config.make_desc([n, n], xmin=[-0.5, -0.5], xmax=[0.5, 0.5], integratorName="RungeKutta4",
                             slopeLimiter="MonotizedCentral",
                             boundaryMethods=[{"min": "PERIODIC", "max": "PERIODIC"}] * 2, cfl=0.5,
@@ -64,10 +65,10 @@ def sync_all(sim):
         pass
 
 
-def time_gpu(precision, n, steps, warmup, q0):
+def time_gpu(precision, n, steps, warmup, q0, kahan=True):
     from hydrodynamicsgpu_amd import capi
 
-    d = make_desc(n, precision)
+    d = make_desc(n, precision, kahan)
     sim = capi.Sim(d)
     sim.set_state(q0)
     sim.run(warmup)
@@ -130,7 +131,7 @@ def main():
     cells = (n - 4) ** 2
     q0 = ics.shear_kh(make_desc(n, "fp64"))
 
-    mixed = time_gpu("fp32_mixed", n, a.steps, a.warmup, q0)
+    mixed = time_gpu("fp32_mixed", n, a.steps, a.warmup, q0, kahan=bool(a.kahan))
     res = {}
     if not a.no_fp64:
         res["fp64"] = time_gpu("fp64", n, a.steps, a.warmup, q0)
@@ -154,10 +155,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "euler2d_roe_kh", "grid": [n, n], "interior_cells": cells, "states": N,
                    "integrator": "RungeKutta4", "limiter": "MonotizedCentral", "boundary": "periodic",
-                   "precision": "fp32_mixed+kahan", "parallelism": "single_gpu"},
+                   "precision": "fp32_mixed+kahan" if a.kahan else "fp32_mixed",
+                   "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "roe_stage_kernel<2,float,float,true,MonotizedCentral>",
+                     "kernel": "stage2d_kernel<RoePolicy<2,float,MonotizedCentral>,2,float,float,%s,mode>"
+                               % ("true" if a.kahan else "false"),
                      "kernel_avg_ms": mixed["stage_ms"], "launches_timed": mixed["launches"],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "health": mixed["health"],

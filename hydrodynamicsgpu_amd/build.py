@@ -22,7 +22,7 @@ COMMON = [
     "-I", CSRC, "-I", INCLUDE, "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["stage2d.hip", "stage3d.hip", "aux_kernels.hip", "capi.cpp"]
+SOURCES = ["stage2d_f64.hip", "stage2d_f32.hip", "stage2d_f32k.hip", "stage3d.hip", "aux_kernels.hip", "capi.cpp"]
 
 
 def _hipcc():

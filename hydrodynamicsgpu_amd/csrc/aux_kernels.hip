@@ -224,6 +224,7 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
         const auto fc = P::template frozen<double>(pc, a.gamma);
         const auto mL = P::mean(P::template frozen<double>(pm, a.gamma), fc, side, a.gamma);
         const auto mR = P::mean(fc, P::template frozen<double>(pp, a.gamma), side, a.gamma);
+        bad |= !(mL.lmax == mL.lmax) || !(mR.lmin == mR.lmin);
         const double lmax = fmax(0.0, mL.lmax);
         const double lmin = fmin(0.0, mR.lmin);
         d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
@@ -235,8 +236,10 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
         const auto cc = P::template cell<S>(q, gamma);
         const auto fL = P::face(P::template cell<S>(qm, gamma), cc, side, gamma);
         const auto fR = P::face(cc, P::template cell<S>(qp, gamma), side, gamma);
-        const double lmax = fmax(0.0, double(fL.v[0] + fL.c));
-        const double lmin = fmin(0.0, double(fR.v[0] - fR.c));
+        const double smax = double(fL.v[0] + fL.c), smin = double(fR.v[0] - fR.c);
+        bad |= !(smax == smax) || !(smin == smin);
+        const double lmax = fmax(0.0, smax);
+        const double lmin = fmin(0.0, smin);
         d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
       } else {
         S slL, srL, slR, srR;
@@ -251,6 +254,7 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
           P::speeds(P::template frozen<S>(qm, gamma), cc, side, gamma, slL, srL);
           P::speeds(cc, P::template frozen<S>(qp, gamma), side, gamma, slR, srR);
         }
+        bad |= !(srL == srL) || !(slR == slR);
         const double minL = fmin(0.0, right_ok ? double(slR) : 0.0);
         const double maxL = fmax(0.0, double(srL));
         d = a.kdx[side] / (maxL - minL);

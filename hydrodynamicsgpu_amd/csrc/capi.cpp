@@ -101,6 +101,7 @@ struct StagePlan {
   int nrep = 0;
   int rep[kMaxPartials] = {0, 0, 0};
   double rc[kMaxPartials] = {0, 0, 0};
+  int mode = CM_GENERIC;  // CombineMode the stage kernel runs
 };
 
 // Replay plan: valid when every row before the last is d_{k+1} = beta[k][k]
@@ -184,6 +185,42 @@ bool plan_stages(const Tableau& t, std::vector<StagePlan>& out, int& nslots) {
     }
   }
   return true;
+}
+
+// Fixed-function combination of a stage, or CM_GENERIC when the stage reads
+// or writes partial sums.
+int combine_mode(const StagePlan& p) {
+  if (p.pk >= 0 || p.nf > 0) return CM_GENERIC;
+  if (p.final_stage) {
+    if (p.din < 0) return p.nrep == 0 ? CM_SINGLE : CM_GENERIC;
+    return CM_FINAL;
+  }
+  if (p.nrep != 0) return CM_GENERIC;
+  return p.din < 0 ? CM_FIRST : CM_MID;
+}
+
+// Rows per wave of the 2D stage kernel.  The kernel is issue-bound, so a
+// stage takes about (waves on the busiest SIMD) x (rows + halo cost) wave
+// iterations: pick the chunk height that minimises that over the grid's
+// strips and rows (a chunk of R output rows costs about R + 3 rows of work:
+// its two halo rows on each side load and run the first face pass).  Chunks
+// stay small enough for `min_waves` waves per SIMD where the grid allows,
+// so that memory latency is hidden behind other waves.
+int pick_rows_2d(long long strips, int ny, int simds, int min_waves) {
+  int best = 32;
+  double best_cost = 1e300;
+  const bool enough = strips * ((ny + 3) / 4) >= (long long)min_waves * simds;
+  for (int rows = 4; rows <= 256; ++rows) {
+    const long long waves = strips * ((ny + rows - 1) / rows);
+    if (enough && waves < (long long)min_waves * simds) break;
+    const long long per_simd = (waves + simds - 1) / simds;
+    const double cost = (double)per_simd * (std::min(rows, ny) + 3.0);
+    if (cost < best_cost * (1.0 - 1e-9)) {
+      best_cost = cost;
+      best = rows;
+    }
+  }
+  return best;
 }
 
 // include/HydroGPU/toNumericString.h:12-22 (6 significant digits)
@@ -390,14 +427,13 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   }
   a.unew = (T*)s->U[s->cur ^ 1];
   a.unewlo = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
+  a.pkp = p.pk < 0 ? nullptr : (const T*)s->P[p.pk];
   for (int j = 0; j < kMaxPartials; ++j) {
-    a.part[j] = (T*)s->P[j];
-    a.fslot[j] = p.fslot[j] < 0 ? 0 : p.fslot[j];
+    a.fp[j] = j < p.nf ? (T*)s->P[p.fslot[j]] : nullptr;
     a.fload[j] = p.fload[j];
     a.am[j] = p.am[j];
     a.bm[j] = p.bm[j];
   }
-  a.pk = p.pk;
   a.nf = p.nf;
   a.a_self = p.a_self;
   a.b_self = p.b_self;
@@ -441,16 +477,19 @@ int enqueue_stages(hydro_sim* s) {
       HIP_TRY(hipEventCreate(&e1));
       HIP_TRY(hipEventRecord(e0, s->stream));
     }
+    const int mode = s->plan[k].mode;
+    const int kind = kernel_kind(s->d);
     if (s->mixed) {
       StageArgs<float, float> a{};
       fill_stage_args<float>(s, k, a);
-      if (s->dim == 3) launch_stage3d_mixed(kernel_kind(s->d), s->d.limiter, s->kahan, a, s->stream);
-      else launch_stage2d_mixed(s->dim, kernel_kind(s->d), s->d.limiter, s->kahan, a, s->wpb, s->stream);
+      if (s->dim == 3) launch_stage3d_mixed(kind, s->d.limiter, mode, s->kahan, a, s->stream);
+      else if (s->kahan) launch_stage2d_f32k(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
+      else launch_stage2d_f32(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
     } else {
       StageArgs<double, double> a{};
       fill_stage_args<double>(s, k, a);
-      if (s->dim == 3) launch_stage3d_f64(kernel_kind(s->d), s->d.limiter, a, s->stream);
-      else launch_stage2d_f64(s->dim, kernel_kind(s->d), s->d.limiter, a, s->wpb, s->stream);
+      if (s->dim == 3) launch_stage3d_f64(kind, s->d.limiter, mode, a, s->stream);
+      else launch_stage2d_f64(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
     }
     HIP_TRY(hipGetLastError());
     if (s->profiling) {
@@ -628,13 +667,17 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     delete s;
     return fail(HYDRO_E_INVALID, "integrator not supported");
   }
-  // rows (2D) or z-planes (3D) per wave: enough waves to fill 256 CUs
-  // several times over
+  // HYDRO_COMBINE=generic runs every stage through the generic combination
+  const char* comb_env = getenv("HYDRO_COMBINE");
+  const bool force_generic = comb_env && std::string(comb_env) == "generic";
+  for (auto& p : s->plan) p.mode = force_generic ? CM_GENERIC : combine_mode(p);
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu <= 0)
+    ncu = 256;
   if (s->dim == 2) {
-    const long long strips = (s->sz[0] - 4 + 59) / 60;
-    int rows = 64;
-    while (rows > 8 && strips * ((s->sz[1] - 4 + rows - 1) / rows) < 4096) rows >>= 1;
-    s->rows = rows;
+    int min_waves = 3;
+    if (const char* e = getenv("HYDRO_MIN_WAVES_PER_SIMD")) min_waves = std::max(1, atoi(e));
+    s->rows = pick_rows_2d((s->sz[0] - 4 + 59) / 60, s->sz[1] - 4, 4 * ncu, min_waves);
   } else if (s->dim == 3) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
     const int ry = stage3d_rows_per_block(kernel_kind(*d));

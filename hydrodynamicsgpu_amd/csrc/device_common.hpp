@@ -13,21 +13,21 @@ namespace hydro {
 // lane l-1 (shr) or lane l+1 (shl).  Edge lanes receive 0; callers only use
 // lanes whose neighbour exists.
 HYDRO_DEV float lane_from_left(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
 }
 HYDRO_DEV float lane_from_right(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
 }
 HYDRO_DEV double lane_from_left(double v) {
   int2 p = __builtin_bit_cast(int2, v);
-  p.x = __builtin_amdgcn_update_dpp(0, p.x, 0x138, 0xF, 0xF, false);
-  p.y = __builtin_amdgcn_update_dpp(0, p.y, 0x138, 0xF, 0xF, false);
+  p.x = __builtin_amdgcn_update_dpp(0, p.x, 0x138, 0xF, 0xF, true);
+  p.y = __builtin_amdgcn_update_dpp(0, p.y, 0x138, 0xF, 0xF, true);
   return __builtin_bit_cast(double, p);
 }
 HYDRO_DEV double lane_from_right(double v) {
   int2 p = __builtin_bit_cast(int2, v);
-  p.x = __builtin_amdgcn_update_dpp(0, p.x, 0x130, 0xF, 0xF, false);
-  p.y = __builtin_amdgcn_update_dpp(0, p.y, 0x130, 0xF, 0xF, false);
+  p.x = __builtin_amdgcn_update_dpp(0, p.x, 0x130, 0xF, 0xF, true);
+  p.y = __builtin_amdgcn_update_dpp(0, p.y, 0x130, 0xF, 0xF, true);
   return __builtin_bit_cast(double, p);
 }
 

@@ -6,17 +6,22 @@
 
 namespace hydro {
 
-// fused RK stage, 1D/2D (stage2d.hip).  solver: 0 Roe, 1 HLL, 2 HLLC
-void launch_stage2d_f64(int dim, int solver, int lim, const StageArgs<double, double>& a, int wpb,
+// fused RK stage, 1D/2D (stage2d_*.hip).  solver: 0 Euler Roe, 1 HLL,
+// 2 HLLC, 3 SRHD Roe, 4 Maxwell Roe; mode: CombineMode of the stage.
+// f32: FP32 storage and arithmetic; f32k: the same with compensated storage.
+void launch_stage2d_f64(int dim, int solver, int lim, int mode, const StageArgs<double, double>& a, int wpb,
                         hipStream_t st);
-void launch_stage2d_mixed(int dim, int solver, int lim, bool kahan, const StageArgs<float, float>& a, int wpb,
-                          hipStream_t st);
+void launch_stage2d_f32(int dim, int solver, int lim, int mode, const StageArgs<float, float>& a, int wpb,
+                        hipStream_t st);
+void launch_stage2d_f32k(int dim, int solver, int lim, int mode, const StageArgs<float, float>& a, int wpb,
+                         hipStream_t st);
 
 // fused RK stage, 3D (stage3d.hip): workgroups of kStage3dWaves rows
 constexpr int kStage3dWaves = 8;
 int stage3d_rows_per_block(int solver);
-void launch_stage3d_f64(int solver, int lim, const StageArgs<double, double>& a, hipStream_t st);
-void launch_stage3d_mixed(int solver, int lim, bool kahan, const StageArgs<float, float>& a, hipStream_t st);
+void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, double>& a, hipStream_t st);
+void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,
+                          hipStream_t st);
 
 // ghost fill along one axis (res/Common.cl:49-194), SoA planes.
 // kinds[s][minmax]: 0 periodic, 1 mirror, 2 reflect, 3 freeflow, -1 none

@@ -6,6 +6,10 @@
 // rotated into x.  The eigenvector matrices are not materialised: L*x and R*y
 // are applied in closed form from (v, H, c), which is algebraically the dense
 // product res/RoeEigenfieldLinear.cl:16-74 performs with the stored matrices.
+//
+// FP32: one v_rsq_f32 gives both sqrt(rho) and 1/rho per cell, and both c and
+// 1/c^2 per face (rsq is accurate to ~1 ulp).  FP64 keeps correctly rounded
+// sqrt and division, as the reference's FP64 kernels have them.
 #pragma once
 #include "device_common.hpp"
 
@@ -15,6 +19,7 @@ template <int DIM, typename T>
 struct EulerRoe {
   static constexpr int N = DIM + 2;
   static constexpr int E = DIM + 1;
+  static constexpr bool kFast = sizeof(T) == 4;
 
   // frozen per-cell data from the start-of-step state
   struct Cell {
@@ -27,24 +32,35 @@ struct EulerRoe {
   struct Face {
     T v[DIM];   // v[0] normal, v[1..] tangential (rotated order)
     T H, c, q2; // enthalpy, sound speed, |v|^2
+    T invD;     // 1 / (2 c^2)
   };
 
   template <typename S>
   static HYDRO_DEV Cell cell(const S q[N], T gamma) {
     Cell c;
     const T rho = T(q[0]);
-    const T inv = trcp(rho);
-    T vsq = T(0);
+    T inv, vsq = T(0);
+    if constexpr (kFast) {
+      const T rs = __builtin_amdgcn_rsqf(rho);
+      c.w = rho * rs;
+      inv = rs * rs;
 #pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      c.v[a] = tdiv(T(q[1 + a]), rho);
-      vsq += c.v[a] * c.v[a];
+      for (int a = 0; a < DIM; ++a) {
+        c.v[a] = T(q[1 + a]) * inv;
+        vsq += c.v[a] * c.v[a];
+      }
+    } else {
+      c.w = tsqrt(rho);
+      inv = T(1) / rho;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        c.v[a] = T(q[1 + a]) / rho;
+        vsq += c.v[a] * c.v[a];
+      }
     }
+    // H = E/rho + p/rho with p = (gamma-1) rho (E/rho - v^2/2)
     const T eT = T(q[E]) * inv;
-    const T eInt = eT - T(.5) * vsq;
-    const T p = (gamma - T(1)) * rho * eInt;
-    c.H = eT + p * inv;
-    c.w = tsqrt(rho);
+    c.H = eT + (gamma - T(1)) * (eT - T(.5) * vsq);
     return c;
   }
 
@@ -60,9 +76,16 @@ struct EulerRoe {
     }
     f.H = (L.w * L.H + R.w * R.H) * norm;
     f.q2 = q2;
-    f.c = tsqrt((f.H - T(.5) * q2) * (gamma - T(1)));
-    // rotate: normal first, then the others in their original order with the
-    // normal slot swapped (EulerRoe.cl:101-110 swaps x<->side)
+    const T c2 = (f.H - T(.5) * q2) * (gamma - T(1));
+    if constexpr (kFast) {
+      const T rs = __builtin_amdgcn_rsqf(c2);
+      f.c = c2 * rs;
+      f.invD = T(.5) * rs * rs;
+    } else {
+      f.c = tsqrt(c2);
+      f.invD = T(.5) / (f.c * f.c);
+    }
+    // rotate: normal first, the normal slot swapped with x (EulerRoe.cl:101-110)
 #pragma unroll
     for (int a = 0; a < DIM; ++a) f.v[a] = v[a];
     if (side != 0) {
@@ -82,12 +105,11 @@ struct EulerRoe {
   // x in the rotated frame -> characteristic amplitudes (L * x)
   static HYDRO_DEV void left(const Face& f, const T x[N], T out[N], T gamma) {
     const T g1 = gamma - T(1);
-    const T invD = T(.5) * trcp(f.c * f.c);
     T s = T(.5) * f.q2 * x[0] - f.v[0] * x[1] + x[E];
 #pragma unroll
     for (int a = 1; a < DIM; ++a) s -= f.v[a] * x[1 + a];
-    const T A = g1 * invD * s;
-    const T B = f.c * invD * (f.v[0] * x[0] - x[1]);
+    const T A = g1 * f.invD * s;
+    const T B = f.c * f.invD * (f.v[0] * x[0] - x[1]);
     out[0] = A + B;
     out[1] = x[0] - T(2) * A;
 #pragma unroll

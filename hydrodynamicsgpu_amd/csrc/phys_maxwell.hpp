@@ -100,8 +100,9 @@ struct MaxwellRoePolicy {
       const T theta = up ? T(1) : T(-1);
       const T r = tdiv(num, a.dq[k]);
       const T phi = limiter<LIM, T>(lim, r);
-      const T eps = l * dt_dx;
-      ft[k] = l * a.lq[k] - T(.5) * (l * a.dq[k]) * (theta + phi * (eps - theta));
+      // l qavg - 1/2 l dq (theta + phi (eps - theta)), eps = l dt/dx
+      const T x = theta + phi * (l * dt_dx - theta);
+      ft[k] = l * (a.lq[k] - T(.5) * (a.dq[k] * x));
     }
     right(ft, side, a.se, a.su, F);
   }

@@ -1,0 +1,252 @@
+// stage2d_impl.hpp -- fused RK stage for 1D and 2D grids on gfx950.
+//
+// One kernel evaluates L(u^(k)) (Roe: res/Roe.cl calcDeltaQTilde + calcFlux;
+// HLL/HLLC: res/EulerHLL.cl, res/EulerHLLC.cl calcFlux; then
+// res/CalcFluxDeriv.cl) and applies the Shu-Osher combination of
+// include/HydroGPU/Integrator/RungeKutta.h:84-99 for a block of cells.
+//
+// Work decomposition: one wavefront owns a strip of 64 consecutive columns
+// (60 output columns plus a 2-column halo on each side) and marches up a
+// chunk of rows.  x-neighbours are exchanged through full-wave DPP shifts, so
+// the x-direction stencil never touches LDS; the y-direction stencil lives in
+// registers as a window over the last rows, so every state value is read
+// from HBM once per stage (plus the halo overlap).  The march is unrolled by
+// three so the window rotates by register renaming instead of copies.  The
+// frozen start-of-step data (Roe eigenbasis, HLL wave speeds) are recomputed
+// from u0 per interface instead of being stored: O(N) bytes per cell instead
+// of O(N^2).
+//
+// Reference semantics kept:
+//  * eigen data frozen at the start of the step (src/Solver/Roe.cpp:82-99,
+//    src/Solver/HLL.cpp:225-227);
+//  * the Roe limiter ratio uses the neighbour interface's own basis
+//    (res/Roe.cl:218-220, 278-290) and the full-step dt (res/Roe.cl:206, 292);
+//  * interfaces the reference never writes (index 1 and size-1 along an
+//    axis, or any coordinate outside 2..size-2) contribute deltaQTilde = 0.
+#pragma once
+#include "device_common.hpp"
+#include "launch.hpp"
+#include "policies.hpp"
+#include "phys_srhd.hpp"
+#include "phys_maxwell.hpp"
+#include "stage_common.hpp"
+
+namespace hydro {
+
+template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>
+__global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
+  constexpr int N = Pol::N;
+  using Frozen = typename Pol::Frozen;
+  using FaceA = typename Pol::FaceA;
+  static_assert(sizeof(T) == sizeof(S), "stage state and increments share one layout");
+
+  // XCD-aware block order: consecutive logical blocks (neighbouring strips)
+  // land on the same XCD and share its L2.  The wave index is wave-uniform;
+  // readfirstlane lets the compiler keep the row arithmetic in scalars.
+  const int nb = gridDim.x;
+  const int per = nb >> 3;
+  const int b = blockIdx.x;
+  const int lb = (b & 7) * per + (b >> 3);
+  const int wave = __builtin_amdgcn_readfirstlane(lb * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+  if (wave >= a.nwaves) return;
+  const int lane = threadIdx.x & 63;
+  const int strip = wave % a.nstrips;
+  const int chunk = wave / a.nstrips;
+
+  const int sx = a.sx, sy = a.sy;
+  const long long pl = a.plane;
+  const int c = strip * kStripOut + lane;       // column (x0 - 2 + lane)
+  const uint32_t cl = c < sx ? c : sx - 1;      // in-bounds load column
+  const bool out_lane = lane >= 2 && lane < 62 && c <= sx - 3;
+  const bool col_face = c >= 2 && c <= sx - 2;  // the reference writes faces here
+
+  const PhysParams<T> pp = make_params<T>(a);
+  const T gamma = pp.gamma;
+  const double dtd = *a.dt;
+  const T dt_dx0 = T(dtd * a.inv_dx[0]);
+  const T idx0 = T(a.inv_dx[0]);
+  const StageCombine<T, S, KAHAN, N, MODE> comb(a, dtd);
+
+  struct Row {
+    T u[N];     // stage state
+    Frozen fz;  // frozen start-of-step data
+  };
+
+  auto load_row = [&](int row, Row& R) {
+    const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;  // wave-uniform
+    S q0[N];
+    comb.load_u0(e, cl, q0);
+    comb.stage_state(e, cl, q0, R.u);
+    if constexpr (Pol::kFrozenPrims) {
+      double pr[N];
+#pragma unroll
+      for (int s = 0; s < N; ++s) pr[s] = ldp(a.prim, pl, s, e, cl);
+      R.fz = Pol::template frozen<double>(pr, gamma);
+    } else {
+      R.fz = Pol::template frozen<S>(q0, gamma);
+    }
+  };
+
+  // x-direction flux divergence of one row: -(F(c+1) - F(c)) / dx
+  auto xdiv = [&](const Row& R, T div[N]) {
+    T uL[N], pk[Pol::NF], pkL[Pol::NF];
+#pragma unroll
+    for (int s = 0; s < N; ++s) uL[s] = lane_from_left(R.u[s]);
+    Pol::pack(R.fz, pk);
+#pragma unroll
+    for (int i = 0; i < Pol::NF; ++i) pkL[i] = lane_from_left(pk[i]);
+    const Frozen fL = Pol::unpack(pkL);
+    const FaceA fa = Pol::face_a(fL, R.fz, uL, R.u, 0, col_face, pp);
+    T dqlo[N], dqhi[N], F[N];
+    if constexpr (Pol::kNeighbors) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        dqlo[s] = lane_from_left(fa.dq[s]);
+        dqhi[s] = lane_from_right(fa.dq[s]);
+      }
+    }
+    Pol::face_b(fa, dqlo, dqhi, dt_dx0, 0, a.lim, F);
+#pragma unroll
+    for (int s = 0; s < N; ++s) div[s] = (F[s] - lane_from_right(F[s])) * idx0;
+  };
+
+  if constexpr (DIM == 1) {
+    Row R;
+    load_row(0, R);
+    T div[N];
+    xdiv(R, div);
+    if (out_lane) comb.finish(0, c, div);
+  } else {
+    const T dt_dy = T(dtd * a.inv_dx[1]);
+    const T idy = T(a.inv_dx[1]);
+    const int y0 = 2 + chunk * a.rows;
+    int y1 = y0 + a.rows;
+    if (y1 > sy - 2) y1 = sy - 2;
+    // face r lies between rows r-1 and r; the reference writes faces 2..sy-2
+    auto yok = [&](int r) { return col_face && r >= 2 && r <= sy - 2; };
+
+    // Window at the top of step(r): P = row r-1, FA1 = first pass of face
+    // r-1, dq2 = deltaQTilde of face r-2, acc = partial derivative of row r-2
+    // (its x-divergence plus the flux through its lower face).
+    Row P;
+    FaceA FA1;
+    T dq2[N], acc[N];
+    {
+      Row C;
+      load_row(y0 - 2, P);
+      load_row(y0 - 1, C);
+      const FaceA Fm = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 - 1), pp);
+      load_row(y0, P);
+      FA1 = Pol::face_a(C.fz, P.fz, C.u, P.u, 1, yok(y0), pp);
+#pragma unroll
+      for (int s = 0; s < N; ++s) dq2[s] = Fm.dq[s];
+      load_row(y0 + 1, C);
+      const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 + 1), pp);
+      T G[N], dx[N];
+      Pol::face_b(FA1, dq2, F0.dq, dt_dy, 1, a.lim, G);  // flux through face y0
+      xdiv(P, dx);
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        acc[s] = dx[s] + G[s] * idy;
+        dq2[s] = FA1.dq[s];
+      }
+      FA1 = F0;
+      P = C;
+    }
+
+    auto step = [&](int r, bool more) {
+      Row C;
+      load_row(r, C);
+      const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(r), pp);
+      T G[N];
+      Pol::face_b(FA1, dq2, F0.dq, dt_dy, 1, a.lim, G);  // flux through face r-1
+      if (out_lane) {
+        T deriv[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) deriv[s] = acc[s] - G[s] * idy;
+        comb.finish((long long)__builtin_amdgcn_readfirstlane(r - 2) * sx, c, deriv);
+      }
+      if (more) {
+        T dx[N];
+        xdiv(P, dx);
+#pragma unroll
+        for (int s = 0; s < N; ++s) acc[s] = dx[s] + G[s] * idy;
+      }
+#pragma unroll
+      for (int s = 0; s < N; ++s) dq2[s] = FA1.dq[s];
+      FA1 = F0;
+      P = C;
+    };
+
+    int r = y0 + 2;
+#pragma unroll 1
+    for (; r + 2 <= y1; r += 3) {
+      step(r, true);
+      step(r + 1, true);
+      step(r + 2, true);
+    }
+#pragma unroll 1
+    for (; r <= y1; ++r) step(r, true);
+    step(y1 + 1, false);
+  }
+}
+
+template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>
+static void launch_pol_mode(const StageArgs<T, S>& a, int wpb, hipStream_t st) {
+  int blocks = (a.nwaves + wpb - 1) / wpb;
+  blocks = (blocks + 7) & ~7;  // multiple of 8 for the XCD remap
+  hipLaunchKernelGGL((stage2d_kernel<Pol, DIM, T, S, KAHAN, MODE>), dim3(blocks), dim3(64 * wpb), 0, st, a);
+}
+
+// Fixed-function combination modes are instantiated for the 2D Euler
+// policies with compile-time limiters; everything else runs CM_GENERIC.
+template <class Pol, int DIM, typename T, typename S, bool KAHAN, bool MODES>
+static void launch_pol(int mode, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
+  if constexpr (MODES) {
+    switch (mode) {
+    case CM_FIRST: launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_FIRST>(a, wpb, st); return;
+    case CM_MID: launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_MID>(a, wpb, st); return;
+    case CM_FINAL: launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_FINAL>(a, wpb, st); return;
+    case CM_SINGLE: launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_SINGLE>(a, wpb, st); return;
+    default: break;
+    }
+  }
+  launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_GENERIC>(a, wpb, st);
+}
+
+// kind: 0 Euler Roe, 1 Euler HLL, 2 Euler HLLC, 3 SRHD Roe, 4 Maxwell Roe
+template <int DIM, typename T, typename S, bool KAHAN>
+static void launch_solver(int kind, int lim, int mode, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
+  constexpr bool M = DIM == 2;
+  if (kind == 1) {
+    launch_pol<HLLPolicy<DIM, T, 1>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
+  } else if (kind == 2) {
+    launch_pol<HLLPolicy<DIM, T, 2>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
+  } else if (kind == 3) {
+    switch (lim) {
+    case LIM_SUPERBEE: launch_pol<SRHDRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, false>(mode, a, wpb, st); break;
+    case LIM_MONOTIZEDCENTRAL:
+      launch_pol<SRHDRoePolicy<DIM, T, LIM_MONOTIZEDCENTRAL>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+      break;
+    default: launch_pol<SRHDRoePolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN, false>(mode, a, wpb, st); break;
+    }
+  } else if (kind == 4) {
+    switch (lim) {
+    case LIM_SUPERBEE:
+      launch_pol<MaxwellRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+      break;
+    default: launch_pol<MaxwellRoePolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN, false>(mode, a, wpb, st); break;
+    }
+  } else {
+    switch (lim) {
+    case LIM_SUPERBEE: launch_pol<RoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, M>(mode, a, wpb, st); break;
+    case LIM_MONOTIZEDCENTRAL:
+      launch_pol<RoePolicy<DIM, T, LIM_MONOTIZEDCENTRAL>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
+      break;
+    case LIM_MINMOD: launch_pol<RoePolicy<DIM, T, LIM_MINMOD>, DIM, T, S, KAHAN, M>(mode, a, wpb, st); break;
+    default: launch_pol<RoePolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN, false>(mode, a, wpb, st); break;
+    }
+  }
+}
+
+}  // namespace hydro

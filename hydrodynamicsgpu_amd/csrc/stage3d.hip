@@ -34,15 +34,16 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (lb >= a.nwaves) return;  // nwaves counts workgroups here (uniform per block)
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform row
   const int strip = lb % a.nstrips;
   const int rest = lb / a.nstrips;
   const int ych = rest % a.nchunks;
   const int zch = rest / a.nchunks;
 
   const int sx = a.sx, sy = a.sy, sz = a.sz;
+  const long long pl = a.plane;
   const int c = strip * kStripOut + lane;
-  const int cl = c < sx ? c : sx - 1;
+  const uint32_t cl = c < sx ? c : sx - 1;
   const int yb = 2 + ych * (W - 2 * H);
   const int y = yb - H + w;
   const int yl = y < sy ? y : sy - 1;
@@ -56,17 +57,18 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   const double dtd = *a.dt;
   const T dt_dx = T(dtd * a.inv_dx[0]), dt_dy = T(dtd * a.inv_dx[1]), dt_dz = T(dtd * a.inv_dx[2]);
   const T idx = T(a.inv_dx[0]), idy = T(a.inv_dx[1]), idz = T(a.inv_dx[2]);
-  StageCombine<T, S, KAHAN, N> comb(a, dtd);
+  const StageCombine<T, S, KAHAN, N, CM_GENERIC> comb(a, dtd);
   const long long psz = (long long)sx * sy;
 
   auto load = [&](int z, T u[N], Frozen& fz) {
-    const long long cell = (long long)z * psz + (long long)yl * sx + cl;
+    const long long e = (long long)z * psz + (long long)yl * sx;  // wave-uniform
     S q0[N];
-    comb.load_state(cell, q0, u);
+    comb.load_u0(e, cl, q0);
+    comb.stage_state(e, cl, q0, u);
     if constexpr (Pol::kFrozenPrims) {
       double pr[N];
 #pragma unroll
-      for (int s = 0; s < N; ++s) pr[s] = a.prim[s * a.plane + cell];
+      for (int s = 0; s < N; ++s) pr[s] = ldp(a.prim, pl, s, e, cl);
       fz = Pol::template frozen<double>(pr, gamma);
     } else {
       fz = Pol::template frozen<S>(q0, gamma);
@@ -179,7 +181,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
         T deriv[N];
 #pragma unroll
         for (int s = 0; s < N; ++s) deriv[s] = div_m2[s] - (Fz[s] - Fz_prev[s]) * idz;
-        comb.finish((long long)(zn - 2) * psz + (long long)y * sx + c, deriv);
+        comb.finish((long long)(zn - 2) * psz + (long long)y * sx, c, deriv);
       }
     } else {
 #pragma unroll
@@ -227,11 +229,14 @@ static void launch3_solver(int kind, int lim, const StageArgs<T, S>& a, hipStrea
 // halo rows: 2 for the Roe family (limiter neighbours), 1 for HLL/HLLC
 int stage3d_rows_per_block(int kind) { return kStage3dWaves - 2 * ((kind == 1 || kind == 2) ? 1 : 2); }
 
-void launch_stage3d_f64(int solver, int lim, const StageArgs<double, double>& a, hipStream_t st) {
+void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, double>& a, hipStream_t st) {
+  (void)mode;  // 3D stages use the generic combination
   launch3_solver<double, double, false, kStage3dWaves>(solver, lim, a, st);
 }
 
-void launch_stage3d_mixed(int solver, int lim, bool kahan, const StageArgs<float, float>& a, hipStream_t st) {
+void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,
+                          hipStream_t st) {
+  (void)mode;
   if (kahan) launch3_solver<float, float, true, kStage3dWaves>(solver, lim, a, st);
   else launch3_solver<float, float, false, kStage3dWaves>(solver, lim, a, st);
 }

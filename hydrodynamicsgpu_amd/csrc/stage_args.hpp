@@ -8,6 +8,17 @@ namespace hydro {
 // Maximum number of Shu-Osher rows whose partial sums can be in flight.
 constexpr int kMaxPartials = 3;
 
+// How a stage combines its derivative with the stored increments.  The host
+// picks the mode from the stage plan; all but GENERIC are fixed-function
+// forms without runtime branches in the kernel.
+enum CombineMode {
+  CM_GENERIC = 0,  // partial sums of later rows (pkp / fp[] below)
+  CM_FIRST = 1,    // k = 0, not last:   d_1 = b dt L_0
+  CM_MID = 2,      // 0 < k < S-1:       d_{k+1} = a d_k + b dt L_k
+  CM_FINAL = 3,    // k = S-1 > 0:       u_new = u0 + a d_k + sum_j rc_j d_{j+1} + b dt L_k
+  CM_SINGLE = 4,   // single stage:      u_new = u0 + b dt L_0
+};
+
 // One fused RK stage: L(u^(k)) with the frozen start-of-step data, then the
 // Shu-Osher combination.  States are carried as increments over the
 // start-of-step state u0:  u^(k) = u0 + d_k,  d_0 = 0, and row m of the
@@ -21,16 +32,16 @@ struct StageArgs {
   T* dnext;             // d_{k+1} planes (non-final stage)
   S* unew;              // final stage: u0 + d_S
   float* unewlo;        // final stage low words (compensated), or null
-  T* part[kMaxPartials];     // partial-sum buffers (slots)
-  int pk;                    // slot holding row k's partial sum, -1 none
-  int nf;                    // future rows carried
-  int fslot[kMaxPartials];   // slot per future row
-  int fload[kMaxPartials];   // 1 if that slot already holds a partial sum
-  double am[kMaxPartials];   // alpha[m][k] per future row
-  double bm[kMaxPartials];   // beta[m][k] per future row
-  double a_self, b_self;     // alpha[k][k], beta[k][k]
-  // replayed increments: acc += rc[j] * rep[j] (tableaus whose earlier rows
-  // each depend on one L_k only, so dt L_k = d_{k+1} / beta[k][k])
+  // CM_GENERIC: partial sum of this row (read), and of later rows (updated)
+  const T* pkp;                    // null when row k has no partial sum
+  int nf;                          // later rows carried
+  T* fp[kMaxPartials];             // partial-sum buffer per later row
+  int fload[kMaxPartials];         // 1 if that buffer already holds a partial sum
+  double am[kMaxPartials];         // alpha[m][k] per later row
+  double bm[kMaxPartials];         // beta[m][k] per later row
+  double a_self, b_self;           // coefficient of d_k, and beta[k][k]
+  // CM_FINAL: replayed increments, acc += rc[j] * rep[j] (tableaus whose
+  // earlier rows each depend on one L_k only, so dt L_k = d_{k+1} / beta[k][k])
   int nrep;
   const T* rep[kMaxPartials];
   double rc[kMaxPartials];

@@ -1,5 +1,6 @@
 // stage_common.hpp -- the Shu-Osher combination shared by the fused stage
-// kernels (include/HydroGPU/Integrator/RungeKutta.h:84-99 in increment form).
+// kernels (include/HydroGPU/Integrator/RungeKutta.h:84-99 in increment form),
+// and the addressing helpers they use.
 #pragma once
 #include "device_common.hpp"
 #include "stage_args.hpp"
@@ -7,6 +8,43 @@
 namespace hydro {
 
 constexpr int kStripOut = 60;  // output columns per wave (64 lanes - 2*2 halo)
+
+// Addressing: element (plane s, uniform offset e, column col) of a plane
+// set.  base + s*plane + e is wave-uniform: it becomes a raw buffer resource
+// in scalar registers (readfirstlane pins it there), and the column enters
+// as a 32-bit byte offset, so each access is one buffer_load/store with no
+// 64-bit vector address arithmetic.
+constexpr int kBufferWord3 = 0x00020000;  // 32-bit data format, raw access (gfx9 family)
+
+template <typename V>
+HYDRO_DEV __amdgpu_buffer_rsrc_t row_rsrc(const V* p) {
+  const uint64_t x = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+  void* q = reinterpret_cast<void*>((uint64_t(hi) << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(q, 0, -1, kBufferWord3);
+}
+template <typename V>
+HYDRO_DEV V ldp(const V* base, long long plane, int s, long long e, uint32_t col) {
+  const __amdgpu_buffer_rsrc_t r = row_rsrc(base + (s * plane + e));
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, col * 4u, 0, 0));
+  } else {
+    static_assert(sizeof(V) == 8, "4- or 8-byte elements");
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, col * 8u, 0, 0));
+  }
+}
+template <typename V>
+HYDRO_DEV void stp(V* base, long long plane, int s, long long e, uint32_t col, V v) {
+  const __amdgpu_buffer_rsrc_t r = row_rsrc(base + (s * plane + e));
+  if constexpr (sizeof(V) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, col * 4u, 0, 0);
+  } else {
+    static_assert(sizeof(V) == 8, "4- or 8-byte elements");
+    using U2 = __attribute__((ext_vector_type(2))) uint32_t;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, col * 8u, 0, 0);
+  }
+}
 
 template <typename T, typename S>
 HYDRO_DEV PhysParams<T> make_params(const StageArgs<T, S>& a) {
@@ -17,73 +55,127 @@ HYDRO_DEV PhysParams<T> make_params(const StageArgs<T, S>& a) {
   return p;
 }
 
-template <typename T, typename S, bool KAHAN, int N>
+// MODE is a CombineMode.  Coefficients are converted to the arithmetic type
+// once per wave; a cell is (wave-uniform element offset e, column col).
+template <typename T, typename S, bool KAHAN, int N, int MODE>
 struct StageCombine {
+  static constexpr bool kReadsDk = MODE == CM_MID || MODE == CM_FINAL || MODE == CM_GENERIC;
   const StageArgs<T, S>& a;
-  const T bdt_self, a_self;
-  const double dtd;
+  const long long pl;
+  T bdt, aself;
+  T rc[kMaxPartials], am[kMaxPartials], bm[kMaxPartials];
 
-  HYDRO_DEV StageCombine(const StageArgs<T, S>& args, double dt)
-      : a(args), bdt_self(T(args.b_self * dt)), a_self(T(args.a_self)), dtd(dt) {}
-
-  // u0 (storage precision) and the stage state u = u0 + d_k (arithmetic precision)
-  HYDRO_DEV void load_state(long long cell, S q0[N], T u[N]) const {
-    const long long pl = a.plane;
-#pragma unroll
-    for (int s = 0; s < N; ++s) q0[s] = a.u0[s * pl + cell];
-#pragma unroll
-    for (int s = 0; s < N; ++s) u[s] = T(q0[s]) + (a.dk ? a.dk[s * pl + cell] : T(0));
-  }
-
-  // d_{k+1} = P_k + alpha[k][k] d_k + dt beta[k][k] L_k, the partial sums of
-  // later rows, and on the last stage u0 + d_S (compensated when KAHAN).
-  HYDRO_DEV void finish(long long cell, const T deriv[N]) const {
-    const long long pl = a.plane;
-    T dk[N];
-#pragma unroll
-    for (int s = 0; s < N; ++s) dk[s] = a.dk ? a.dk[s * pl + cell] : T(0);
-    T acc[N];
-#pragma unroll
-    for (int s = 0; s < N; ++s) {
-      T v = (a.pk >= 0) ? a.part[a.pk][s * pl + cell] : T(0);
-#pragma unroll
-      for (int j = 0; j < kMaxPartials; ++j)
-        if (j < a.nrep) v += T(a.rc[j]) * a.rep[j][s * pl + cell];
-      if (a.dk) v += a_self * dk[s];
-      acc[s] = v + bdt_self * deriv[s];
-    }
+  HYDRO_DEV StageCombine(const StageArgs<T, S>& args, double dt) : a(args), pl(args.plane) {
+    bdt = T(args.b_self * dt);
+    aself = T(args.a_self);
 #pragma unroll
     for (int j = 0; j < kMaxPartials; ++j) {
-      if (j >= a.nf) break;
-      T* pp = a.part[a.fslot[j]];
-      const T am = T(a.am[j]);
-      const T bm = T(a.bm[j] * dtd);
-#pragma unroll
-      for (int s = 0; s < N; ++s) {
-        T v = a.fload[j] ? pp[s * pl + cell] : T(0);
-        if (a.dk) v += am * dk[s];
-        pp[s * pl + cell] = v + bm * deriv[s];
-      }
+      rc[j] = T(args.rc[j]);
+      am[j] = T(args.am[j]);
+      bm[j] = T(args.bm[j] * dt);
     }
-    if (a.final_stage) {
+  }
+
+  // u0 of a cell, in storage precision
+  HYDRO_DEV void load_u0(long long e, uint32_t col, S q0[N]) const {
 #pragma unroll
-      for (int s = 0; s < N; ++s) {
-        const S hi = a.u0[s * pl + cell];
-        if constexpr (KAHAN) {
-          // compensated update of (hi, lo) by acc (Fast2Sum: |hi| >= |lo + acc|)
-          const float lo = a.u0lo[s * pl + cell];
-          const float y = float(acc[s]) + lo;
-          const float t = float(hi) + y;
-          const float e = y - (t - float(hi));
-          a.unew[s * pl + cell] = S(t);
-          a.unewlo[s * pl + cell] = e;
-        } else {
-          a.unew[s * pl + cell] = S(hi + S(acc[s]));
-        }
+    for (int s = 0; s < N; ++s) q0[s] = ldp(a.u0, pl, s, e, col);
+  }
+
+  // stage state u = u0 + d_k (arithmetic precision) from u0
+  HYDRO_DEV void stage_state(long long e, uint32_t col, const S q0[N], T u[N]) const {
+    if constexpr (MODE == CM_FIRST || MODE == CM_SINGLE) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) u[s] = T(q0[s]);
+    } else if constexpr (MODE == CM_GENERIC) {
+      if (a.dk) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) u[s] = T(q0[s]) + ldp(a.dk, pl, s, e, col);
+      } else {
+#pragma unroll
+        for (int s = 0; s < N; ++s) u[s] = T(q0[s]);
       }
     } else {
 #pragma unroll
-      for (int s = 0; s < N; ++s) a.dnext[s * pl + cell] = acc[s];
+      for (int s = 0; s < N; ++s) u[s] = T(q0[s]) + ldp(a.dk, pl, s, e, col);
+    }
+  }
+
+  HYDRO_DEV void store_final(long long e, uint32_t col, const T acc[N]) const {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const S hi = ldp(a.u0, pl, s, e, col);
+      if constexpr (KAHAN) {
+        // compensated update of (hi, lo) by acc (Fast2Sum: |hi| >= |lo + acc|)
+        const float lo = ldp(a.u0lo, pl, s, e, col);
+        const float y = float(acc[s]) + lo;
+        const float t = float(hi) + y;
+        const float err = y - (t - float(hi));
+        stp(a.unew, pl, s, e, col, S(t));
+        stp(a.unewlo, pl, s, e, col, err);
+      } else {
+        stp(a.unew, pl, s, e, col, S(hi + S(acc[s])));
+      }
+    }
+  }
+
+  // combine the derivative of a cell into the increment / state
+  HYDRO_DEV void finish(long long e, uint32_t col, const T deriv[N]) const {
+    if constexpr (MODE == CM_FIRST) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, bdt * deriv[s]);
+    } else if constexpr (MODE == CM_MID) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, aself * ldp(a.dk, pl, s, e, col) + bdt * deriv[s]);
+    } else if constexpr (MODE == CM_SINGLE) {
+      T acc[N];
+#pragma unroll
+      for (int s = 0; s < N; ++s) acc[s] = bdt * deriv[s];
+      store_final(e, col, acc);
+    } else if constexpr (MODE == CM_FINAL) {
+      T acc[N];
+#pragma unroll
+      for (int s = 0; s < N; ++s) acc[s] = aself * ldp(a.dk, pl, s, e, col) + bdt * deriv[s];
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j) {
+        if (j < a.nrep) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) acc[s] += rc[j] * ldp(a.rep[j], pl, s, e, col);
+        }
+      }
+      store_final(e, col, acc);
+    } else {
+      T dk[N], acc[N];
+#pragma unroll
+      for (int s = 0; s < N; ++s) dk[s] = a.dk ? ldp(a.dk, pl, s, e, col) : T(0);
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        const T p = a.pkp ? ldp(a.pkp, pl, s, e, col) : T(0);
+        acc[s] = p + aself * dk[s] + bdt * deriv[s];
+      }
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j) {
+        if (j < a.nrep) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) acc[s] += rc[j] * ldp(a.rep[j], pl, s, e, col);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j) {
+        if (j < a.nf) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) {
+            const T v = a.fload[j] ? ldp((const T*)a.fp[j], pl, s, e, col) : T(0);
+            stp(a.fp[j], pl, s, e, col, v + am[j] * dk[s] + bm[j] * deriv[s]);
+          }
+        }
+      }
+      if (a.final_stage) {
+        store_final(e, col, acc);
+      } else {
+#pragma unroll
+        for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, acc[s]);
+      }
     }
   }
 };
