@@ -27,6 +27,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))  # exact_riemann (closed-form Sod solution)
 
 HBM_PEAK_GBS = 8000.0
 
@@ -40,7 +41,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--kahan", type=int, default=1, help="compensated FP32 state storage (1) or plain FP32 (0)")
+    ap.add_argument("--kahan", type=int, default=0,
+                    help="headline path with compensated FP32 state storage (1) or plain FP32 (0)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the Sod L1 accuracy leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (calibrates the HBM "
+                         "byte counters of a profiler run against a known transfer), then exit")
     return ap.parse_args()
 
 
@@ -95,6 +101,28 @@ def time_gpu(precision, n, steps, warmup, q0, kahan=True):
                 state=state, t=t, nsteps=nsteps)
 
 
+def sod_l1(precision, kahan, n=1024):
+    """L1 density error against the exact solution of the 1D Sod tube (Euler
+    Roe, TVD-RK2, Superbee, free-flow, CFL 0.5, t = 0.2, 1024 cells on
+    [-1, 1]) on the GPU path of the given precision."""
+    from exact_riemann import sample
+    from hydrodynamicsgpu_amd import capi, config, ics
+
+    flags = config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND | (config.FLAG_KAHAN if kahan else 0)
+    d = config.make_desc([n], xmin=[-1], xmax=[1], integratorName="RungeKutta2TVD", slopeLimiter="Superbee",
+                         cfl=0.5, t_end=0.2, precision=precision, flags=flags)
+    sim = capi.Sim(d)
+    sim.set_state(ics.sod(d))
+    while sim.time()[0] < 0.2 - 1e-12:
+        sim.run(16)
+    rho, _, _ = ics.primitives_euler(d, sim.get_state())
+    x = ics.cell_centers(d)[0]
+    er, _, _ = sample((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4, x / 0.2)
+    l1 = float(np.sum(np.abs(rho - er)[2:n - 2]) * (2.0 / n))
+    sim.close()
+    return l1
+
+
 def cpu_baseline(n, seconds, q0):
     import oracle
 
@@ -118,8 +146,23 @@ def cpu_baseline(n, seconds, q0):
                        % (n, n, steps, threads))
 
 
+def copy_probe(mib):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+
+
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
               "(one MI355X per box); rerun with --gpus 1", file=sys.stderr)
@@ -135,6 +178,8 @@ def main():
     res = {}
     if not a.no_fp64:
         res["fp64"] = time_gpu("fp64", n, a.steps, a.warmup, q0)
+        # the other FP32 storage form, for comparison
+        res["other"] = time_gpu("fp32_mixed", n, a.steps, a.warmup, q0, kahan=not a.kahan)
 
     value = cells * a.steps / mixed["elapsed"]
     B = N * 4 * (3 * S - 1)  # algorithmic bytes per cell-update, FP32 state
@@ -180,6 +225,17 @@ def main():
         out["accuracy"] = {"l1_rho_mixed_minus_fp64": float(np.mean(np.abs(rho32 - rho64))),
                            "linf_rho_mixed_minus_fp64": float(np.max(np.abs(rho32 - rho64))),
                            "t_mixed": mixed["t"], "t_fp64": f["t"]}
+        o = res["other"]
+        out["fp32_" + ("plain" if a.kahan else "kahan")] = {
+            "value": cells * a.steps / o["elapsed"], "ms_per_step": o["elapsed"] * 1e3 / a.steps,
+            "stage_kernel_avg_ms": o["stage_ms"]}
+    if not a.no_accuracy:
+        # L1 error at fixed resolution against the exact Sod solution: the
+        # mixed path must stay within 2x of the FP64 path
+        l1 = {"fp64": sod_l1("fp64", False), "fp32_mixed": sod_l1("fp32_mixed", False),
+              "fp32_mixed_kahan": sod_l1("fp32_mixed", True)}
+        out["sod_l1"] = dict(l1, workload="euler1d_roe_sod_1024_rk2tvd_superbee_t0.2",
+                             mixed_over_fp64=l1["fp32_mixed" + ("_kahan" if a.kahan else "")] / l1["fp64"])
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n, a.cpu_seconds, q0)

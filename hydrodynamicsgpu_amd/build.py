@@ -42,6 +42,27 @@ def _compile(src):
     return obj
 
 
+HOST_SRC = os.path.join(HERE, "host", "src")
+HOST_INC = os.path.join(HERE, "host", "include")
+HOST_LIB = os.path.join(HERE, "libhydrogpu.so")
+
+
+def build_host(verbose=False):
+    """The C++ host layer (HydroGPU::Solver & co.) over the C ABI: plain g++,
+    linked against libhydro_hip.so next to it."""
+    srcs = sorted(os.path.join(HOST_SRC, f) for f in os.listdir(HOST_SRC) if f.endswith(".cpp"))
+    tmp = HOST_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-I", HOST_INC, "-I", INCLUDE,
+           "-o", tmp] + srcs + ["-L", HERE, "-lhydro_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, HOST_LIB)
+    if verbose:
+        print("built", HOST_LIB)
+    return HOST_LIB
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs = min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8")))
@@ -55,6 +76,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -223,6 +223,9 @@ int pick_rows_2d(long long strips, int ny, int simds, int min_waves) {
   return best;
 }
 
+// upper bound on the per-wave minima of the strip-march CFL kernel
+constexpr int kDtMaxWaves = 65536;
+
 // include/HydroGPU/toNumericString.h:12-22 (6 significant digits)
 double kernel_dx(double dx, bool six) {
   if (!six) return dx;
@@ -271,6 +274,7 @@ struct hydro_sim {
   DevScalars* sc = nullptr;
   DevScalars* host_sc = nullptr;  // pinned
   int cur = 0;
+  bool ghosts_pending = false;  // U[cur]'s ghost cells still to be copied from U[cur ^ 1]
   int dt_blocks = 1024;
   int tot_blocks = 256;
   // stage decomposition
@@ -307,46 +311,83 @@ int validate(const hydro_sim_desc* d) {
   return HYDRO_OK;
 }
 
-// Euler::stateGetBoundaryKernelForBoundaryMethod (src/Equation/Euler.cpp:63-75)
-BoundaryArgs boundary_args(const hydro_sim* s, int axis) {
+// Euler::stateGetBoundaryKernelForBoundaryMethod (src/Equation/Euler.cpp:63-75),
+// SRHD (src/Equation/SRHD.cpp:63-75), Maxwell (src/Equation/Maxwell.cpp:73-85)
+BoundaryArgs boundary_args(const hydro_sim* s) {
   BoundaryArgs b{};
+  b.dim = s->dim;
   b.sx = s->sz[0];
   b.sy = s->sz[1];
   b.sz = s->sz[2];
   b.plane = s->plane;
   b.nstates = s->N;
-  b.axis = axis;
-  for (int st = 0; st < 8; ++st)
+  for (int axis = 0; axis < 3; ++axis)
     for (int mm = 0; mm < 2; ++mm) {
-      signed char k = -1;
-      if (st < s->N) {
+      signed char m = -1;
+      unsigned neg = 0;
+      if (axis < s->dim) {
         switch (s->d.boundary[axis][mm]) {
-        case HYDRO_BOUNDARY_PERIODIC: k = 0; break;
+        case HYDRO_BOUNDARY_PERIODIC: m = 0; break;
         case HYDRO_BOUNDARY_MIRROR:
-          if (s->d.equation == HYDRO_EQN_MAXWELL)  // src/Equation/Maxwell.cpp:80
-            k = (st == axis || st == 3 + axis) ? 2 : 1;
-          else  // src/Equation/Euler.cpp:70, src/Equation/SRHD.cpp:70
-            k = (axis + 1 == st) ? 2 : 1;
+          m = 1;
+          if (s->d.equation == HYDRO_EQN_MAXWELL)  // src/Equation/Maxwell.cpp:80: normal E and B
+            neg = (1u << axis) | (1u << (3 + axis));
+          else  // src/Equation/Euler.cpp:70, src/Equation/SRHD.cpp:70: normal momentum
+            neg = 1u << (axis + 1);
           break;
-        case HYDRO_BOUNDARY_FREEFLOW: k = 3; break;
-        default: k = -1;
+        case HYDRO_BOUNDARY_FREEFLOW: m = 2; break;
+        default: m = -1;
         }
       }
-      b.kind[st][mm] = k;
+      b.method[axis][mm] = m;
+      b.negmask[axis][mm] = neg;
     }
   return b;
 }
 
-int enqueue_boundary(hydro_sim* s) {
-  for (int axis = 0; axis < s->dim; ++axis) {
-    BoundaryArgs b = boundary_args(s, axis);
-    if (s->mixed) launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->stream);
-    else launch_boundary_f64(b, (double*)s->U[s->cur], s->stream);
+// true when the boundary rewrites every ghost cell (no side is NONE)
+bool boundary_covers_ghosts(const hydro_sim* s) {
+  for (int a = 0; a < s->dim; ++a)
+    for (int m = 0; m < 2; ++m)
+      if (s->d.boundary[a][m] == HYDRO_BOUNDARY_NONE) return false;
+  return true;
+}
+
+// The stage kernels write interior cells only; the new state's ghost cells
+// are those of u0 (their derivative is zero).  Copying them is deferred until
+// something reads them: the next boundary() rewrites every ghost cell from
+// interior values, so in a run of updates the copy is never needed.
+int materialize_ghosts(hydro_sim* s) {
+  if (!s->ghosts_pending) return HYDRO_OK;
+  const int prev = s->cur ^ 1;
+  if (s->mixed) {
+    launch_copy_ghosts_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const float*)s->U[prev],
+                           (float*)s->U[s->cur], s->stream);
+    if (s->kahan)
+      launch_copy_ghosts_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, s->Ulo[prev], s->Ulo[s->cur],
+                             s->stream);
+  } else {
+    launch_copy_ghosts_f64(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const double*)s->U[prev],
+                           (double*)s->U[s->cur], s->stream);
   }
+  HIP_TRY(hipGetLastError());
+  s->ghosts_pending = false;
+  return HYDRO_OK;
+}
+
+int enqueue_boundary(hydro_sim* s) {
+  if (boundary_covers_ghosts(s)) {
+    s->ghosts_pending = false;
+  } else {
+    const int rc = materialize_ghosts(s);
+    if (rc) return rc;
+  }
+  const BoundaryArgs b = boundary_args(s);
+  if (s->mixed) launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->stream);
+  else launch_boundary_f64(b, (double*)s->U[s->cur], s->stream);
   // SRHDRoe::boundary fills the primitives with the same kernels afterwards
   // (src/Solver/SRHDRoe.cpp:79-96)
-  if (s->prim)
-    for (int axis = 0; axis < s->dim; ++axis) launch_boundary_f64(boundary_args(s, axis), s->prim, s->stream);
+  if (s->prim) launch_boundary_f64(b, s->prim, s->stream);
   HIP_TRY(hipGetLastError());
   return HYDRO_OK;
 }
@@ -375,6 +416,56 @@ void maxwell_constants(const hydro_sim_desc& d, double mx[5]) {
 }
 
 int enqueue_dt(hydro_sim* s, bool advance_time) {
+  FinalizeArgs f{};
+  f.blockmin = s->blockmin;
+  f.nblocks = s->d.use_fixed_dt ? 0 : s->dt_blocks;
+  f.cfl = s->cfl;
+  f.use_fixed = s->d.use_fixed_dt;
+  f.fixed_dt = s->d.fixed_dt;
+  f.clip = (s->d.flags & HYDRO_FLAG_CLIP_TO_TEND) ? 1 : 0;
+  f.t_end = s->d.t_end;
+  f.dt = &s->sc->dt;
+  // without advance the time/steps land in a scratch copy
+  f.t = advance_time ? &s->sc->t : &s->sc[1].t;
+  f.steps = advance_time ? &s->sc->steps : &s->sc[1].steps;
+  f.health = &s->sc->health;
+  const int march_strips = (s->sz[0] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 2) / 60 + 1;
+  const long long march_waves =
+      (long long)march_strips * (s->dim > 1 ? (s->sz[1] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 1 + kDtRows - 1) / kDtRows : 1);
+  if (!s->d.use_fixed_dt && s->dim <= 2 && s->d.equation == HYDRO_EQN_EULER && march_waves <= kDtMaxWaves) {
+    // strip march, one minimum per wave, then the one-block tail
+    DtMarchArgs a{};
+    a.dim = s->dim;
+    a.sx = s->sz[0];
+    a.sy = s->sz[1];
+    a.plane = s->plane;
+    a.kdx[0] = s->kdx[0];
+    a.kdx[1] = s->kdx[1];
+    a.gamma = s->d.gamma;
+    a.solver = s->d.solver;
+    const int hi = s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
+    a.last_x = s->sz[0] - hi;
+    a.last_y = s->dim > 1 ? s->sz[1] - hi : 0;
+    a.nstrips = (a.last_x - 2) / 60 + 1;
+    if (s->dim > 1) {
+      const long long ny = a.last_y - 1;
+      // up to kDtRows rows per wave, fewer when that leaves SIMDs idle
+      a.rows = kDtRows;
+      while (a.rows > 2 && (long long)a.nstrips * ((ny + a.rows - 1) / a.rows) < 4096) --a.rows;
+      a.nchunks = (int)((ny + a.rows - 1) / a.rows);
+    } else {
+      a.rows = 1;
+      a.nchunks = 1;
+    }
+    a.nwaves = a.nstrips * a.nchunks;
+    a.wavemin = s->blockmin;
+    if (s->mixed) launch_dt_march_f32(a, (const float*)s->U[s->cur], s->stream);
+    else launch_dt_march_f64(a, (const double*)s->U[s->cur], s->stream);
+    f.nblocks = a.nwaves;
+    launch_finalize_dt(f, s->stream);
+    HIP_TRY(hipGetLastError());
+    return HYDRO_OK;
+  }
   if (!s->d.use_fixed_dt) {
     DtArgs a{};
     a.dim = s->dim;
@@ -395,19 +486,6 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
     if (s->mixed) launch_dt_euler_f32(a, (const float*)s->U[s->cur], s->stream);
     else launch_dt_euler_f64(a, (const double*)s->U[s->cur], s->stream);
   }
-  FinalizeArgs f{};
-  f.blockmin = s->blockmin;
-  f.nblocks = s->d.use_fixed_dt ? 0 : s->dt_blocks;
-  f.cfl = s->cfl;
-  f.use_fixed = s->d.use_fixed_dt;
-  f.fixed_dt = s->d.fixed_dt;
-  f.clip = (s->d.flags & HYDRO_FLAG_CLIP_TO_TEND) ? 1 : 0;
-  f.t_end = s->d.t_end;
-  f.dt = &s->sc->dt;
-  // without advance the time/steps land in a scratch copy
-  f.t = advance_time ? &s->sc->t : &s->sc[1].t;
-  f.steps = advance_time ? &s->sc->steps : &s->sc[1].steps;
-  f.health = &s->sc->health;
   launch_finalize_dt(f, s->stream);
   HIP_TRY(hipGetLastError());
   return HYDRO_OK;
@@ -469,6 +547,10 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
 }
 
 int enqueue_stages(hydro_sim* s) {
+  {
+    const int rc = materialize_ghosts(s);  // the stencil reads u0's ghost cells
+    if (rc) return rc;
+  }
   const int S = s->tab.order;
   for (int k = 0; k < S; ++k) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -498,20 +580,14 @@ int enqueue_stages(hydro_sim* s) {
     }
   }
   // ghost cells of the new state are those of u0 (the RK combination leaves
-  // them unchanged: their derivative is zero)
-  const int nxt = s->cur ^ 1;
-  if (s->mixed) {
-    launch_copy_ghosts_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const float*)s->U[s->cur],
-                           (float*)s->U[nxt], s->stream);
-    if (s->kahan)
-      launch_copy_ghosts_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, s->Ulo[s->cur], s->Ulo[nxt],
-                             s->stream);
-  } else {
-    launch_copy_ghosts_f64(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const double*)s->U[s->cur],
-                           (double*)s->U[nxt], s->stream);
+  // them unchanged: their derivative is zero); copied when first read
+  s->cur ^= 1;
+  s->ghosts_pending = true;
+  // SRHD's constrainState touches every cell, ghosts included
+  if (s->d.equation == HYDRO_EQN_SRHD) {
+    const int rc = materialize_ghosts(s);
+    if (rc) return rc;
   }
-  HIP_TRY(hipGetLastError());
-  s->cur = nxt;
   // ForwardEuler's zero-gravity pass refreshes the ghosts after the update
   // (src/Solver/SelfGravitation.cpp:122 + src/Integrator/ForwardEuler.cpp:29-31)
   if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER &&
@@ -720,8 +796,9 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
       return cleanup(HYDRO_E_HIP, "out of device memory (primitives)");
     hipMemset(s->prim, 0, sizeof(double) * s->plane * s->N);
   }
-  if (hipMalloc((void**)&s->blockmin, sizeof(double) * s->dt_blocks) != hipSuccess)
+  if (hipMalloc((void**)&s->blockmin, sizeof(double) * std::max(s->dt_blocks, kDtMaxWaves)) != hipSuccess)
     return cleanup(HYDRO_E_HIP, "alloc");
+
   if (hipMalloc((void**)&s->partial, sizeof(double) * s->tot_blocks * 8) != hipSuccess)
     return cleanup(HYDRO_E_HIP, "alloc");
   if (hipMalloc((void**)&s->sc, sizeof(DevScalars) * 2) != hipSuccess) return cleanup(HYDRO_E_HIP, "alloc");
@@ -790,6 +867,7 @@ int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
       for (int k = 0; k < N; ++k) buf[(size_t)k * s->plane + c] = aos[k + (size_t)N * c];
     HIP_TRY(hipMemcpy(s->U[s->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  s->ghosts_pending = false;
   // a new initial state restarts the clock
   DevScalars z{};
   HIP_TRY(hipMemcpy(s->sc, &z, sizeof z, hipMemcpyHostToDevice));
@@ -799,6 +877,10 @@ int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
 int hydro_sim_get_state(hydro_sim* s, double* aos, size_t n) {
   if (!s || !aos) return fail(HYDRO_E_INVALID, "null argument");
   if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "state size must be N * volume");
+  {
+    const int rc = materialize_ghosts(s);
+    if (rc) return rc;
+  }
   HIP_TRY(hipStreamSynchronize(s->stream));
   const int N = s->N;
   const long long V = s->vol;
@@ -843,7 +925,9 @@ int hydro_sim_boundary(hydro_sim* s) {
 
 int hydro_sim_calc_timestep(hydro_sim* s, double* dt_out) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
-  int rc = enqueue_dt(s, false);
+  int rc = materialize_ghosts(s);  // the CFL reads the ghost neighbours of edge cells
+  if (rc) return rc;
+  rc = enqueue_dt(s, false);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(s->host_sc, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -853,8 +937,18 @@ int hydro_sim_calc_timestep(hydro_sim* s, double* dt_out) {
 
 int hydro_sim_step(hydro_sim* s, double dt) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
-  s->host_sc->dt = dt;
-  HIP_TRY(hipMemcpyAsync(&s->sc->dt, &s->host_sc->dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  // dt onto the device and the clock advanced by it (the tail of
+  // findMinTimestep with a given dt), then the stages
+  FinalizeArgs f{};
+  f.nblocks = 0;
+  f.use_fixed = 1;
+  f.fixed_dt = dt;
+  f.dt = &s->sc->dt;
+  f.t = &s->sc->t;
+  f.steps = &s->sc->steps;
+  f.health = &s->sc->health;
+  launch_finalize_dt(f, s->stream);
+  HIP_TRY(hipGetLastError());
   return enqueue_stages(s);
 }
 

@@ -23,14 +23,16 @@ void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, d
 void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,
                           hipStream_t st);
 
-// ghost fill along one axis (res/Common.cl:49-194), SoA planes.
-// kinds[s][minmax]: 0 periodic, 1 mirror, 2 reflect, 3 freeflow, -1 none
+// ghost fill of every axis in one launch (res/Common.cl:49-194 composed),
+// SoA planes.  method[axis][minmax]: 0 periodic, 1 mirror, 2 freeflow,
+// -1 none; negmask[axis][minmax]: bit s set when state s is reflected
+// (negated) rather than mirrored
 struct BoundaryArgs {
-  int sx, sy, sz;
+  int dim, sx, sy, sz;
   long long plane;
   int nstates;
-  int axis;
-  signed char kind[8][2];
+  signed char method[3][2];
+  unsigned negmask[3][2];
 };
 void launch_boundary_f64(const BoundaryArgs& b, double* planes, hipStream_t st);
 void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, hipStream_t st);
@@ -87,6 +89,23 @@ struct FinalizeArgs {
   unsigned* health; // in/out
 };
 void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st);
+
+// CFL of Euler Roe/HLL/HLLC on 1D/2D grids (strip march): one minimum per
+// wave into wavemin[nwaves] (NaN where a speed was not finite), reduced by
+// launch_finalize_dt over nblocks = nwaves
+constexpr int kDtRows = 8;  // rows per wave of the strip-march CFL kernel (at most)
+struct DtMarchArgs {
+  int dim, sx, sy;
+  long long plane;
+  double kdx[2];
+  double gamma;
+  int solver;                   // 0 Roe, 1 HLL, 2 HLLC
+  int last_x, last_y;           // last cell with a dt: S-3 (Roe), S-2 (HLL)
+  int nstrips, rows, nchunks, nwaves;
+  double* wavemin;
+};
+void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
+void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
 
 // FP64 interior sums, per block: partial[b * n + s]
 void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u,

@@ -67,24 +67,36 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
   const T idx0 = T(a.inv_dx[0]);
   const StageCombine<T, S, KAHAN, N, MODE> comb(a, dtd);
 
+  using Comb = StageCombine<T, S, KAHAN, N, MODE>;
   struct Row {
     T u[N];     // stage state
     Frozen fz;  // frozen start-of-step data
   };
-
-  auto load_row = [&](int row, Row& R) {
-    const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;  // wave-uniform
+  // global loads of one row, issued one march step ahead of their use
+  struct Raw {
     S q0[N];
-    comb.load_u0(e, cl, q0);
-    comb.stage_state(e, cl, q0, R.u);
+    T dk[N];
+    double pr[Pol::kFrozenPrims ? N : 1];
+  };
+
+  auto fetch = [&](int row, Raw& w) {
+    const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;  // wave-uniform
+    comb.load_u0(e, cl, w.q0);
+    comb.load_dk(e, cl, w.dk);
     if constexpr (Pol::kFrozenPrims) {
-      double pr[N];
 #pragma unroll
-      for (int s = 0; s < N; ++s) pr[s] = ldp(a.prim, pl, s, e, cl);
-      R.fz = Pol::template frozen<double>(pr, gamma);
-    } else {
-      R.fz = Pol::template frozen<S>(q0, gamma);
+      for (int s = 0; s < N; ++s) w.pr[s] = ldp(a.prim, pl, s, e, cl);
     }
+  };
+  auto build = [&](const Raw& w, Row& R) {
+    comb.stage_state(w.q0, w.dk, R.u);
+    if constexpr (Pol::kFrozenPrims) R.fz = Pol::template frozen<double>(w.pr, gamma);
+    else R.fz = Pol::template frozen<S>(w.q0, gamma);
+  };
+  auto load_row = [&](int row, Row& R) {
+    Raw w;
+    fetch(row, w);
+    build(w, R);
   };
 
   // x-direction flux divergence of one row: -(F(c+1) - F(c)) / dx
@@ -127,20 +139,28 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
 
     // Window at the top of step(r): P = row r-1, FA1 = first pass of face
     // r-1, dq2 = deltaQTilde of face r-2, acc = partial derivative of row r-2
-    // (its x-divergence plus the flux through its lower face).
+    // (its x-divergence plus the flux through its lower face), nx = the
+    // loads of row r (in flight).
     Row P;
     FaceA FA1;
     T dq2[N], acc[N];
+    Raw nx;
     {
       Row C;
-      load_row(y0 - 2, P);
-      load_row(y0 - 1, C);
+      Raw w0, w1;
+      fetch(y0 - 2, w0);
+      fetch(y0 - 1, w1);
+      build(w0, P);
+      fetch(y0, w0);
+      build(w1, C);
       const FaceA Fm = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 - 1), pp);
-      load_row(y0, P);
+      fetch(y0 + 1, w1);
+      build(w0, P);
       FA1 = Pol::face_a(C.fz, P.fz, C.u, P.u, 1, yok(y0), pp);
 #pragma unroll
       for (int s = 0; s < N; ++s) dq2[s] = Fm.dq[s];
-      load_row(y0 + 1, C);
+      fetch(y0 + 2, nx);
+      build(w1, C);
       const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 + 1), pp);
       T G[N], dx[N];
       Pol::face_b(FA1, dq2, F0.dq, dt_dy, 1, a.lim, G);  // flux through face y0
@@ -155,8 +175,15 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
     }
 
     auto step = [&](int r, bool more) {
+      const Raw cur = nx;
+      // prefetch the next row (clamped: the last step reloads its own row)
+      fetch(r + 1 <= y1 + 1 ? r + 1 : y1 + 1, nx);
+      // operands of this step's output row r-2
+      const long long eo = (long long)__builtin_amdgcn_readfirstlane(r - 2) * sx;
+      typename Comb::Fin fin;
+      comb.fetch_fin(eo, cl, fin);
       Row C;
-      load_row(r, C);
+      build(cur, C);
       const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(r), pp);
       T G[N];
       Pol::face_b(FA1, dq2, F0.dq, dt_dy, 1, a.lim, G);  // flux through face r-1
@@ -164,7 +191,7 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
         T deriv[N];
 #pragma unroll
         for (int s = 0; s < N; ++s) deriv[s] = acc[s] - G[s] * idy;
-        comb.finish((long long)__builtin_amdgcn_readfirstlane(r - 2) * sx, c, deriv);
+        comb.finish(eo, c, fin, deriv);
       }
       if (more) {
         T dx[N];

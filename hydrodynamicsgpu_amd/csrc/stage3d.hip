@@ -63,8 +63,10 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   auto load = [&](int z, T u[N], Frozen& fz) {
     const long long e = (long long)z * psz + (long long)yl * sx;  // wave-uniform
     S q0[N];
+    T dk[N];
     comb.load_u0(e, cl, q0);
-    comb.stage_state(e, cl, q0, u);
+    comb.load_dk(e, cl, dk);
+    comb.stage_state(q0, dk, u);
     if constexpr (Pol::kFrozenPrims) {
       double pr[N];
 #pragma unroll

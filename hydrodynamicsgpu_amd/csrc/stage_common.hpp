@@ -76,39 +76,82 @@ struct StageCombine {
     }
   }
 
-  // u0 of a cell, in storage precision
+  // Loads of one cell that the stage state needs: u0 (storage precision)
+  // and d_k.  Issued ahead of use so their latency overlaps compute.
   HYDRO_DEV void load_u0(long long e, uint32_t col, S q0[N]) const {
 #pragma unroll
     for (int s = 0; s < N; ++s) q0[s] = ldp(a.u0, pl, s, e, col);
   }
-
-  // stage state u = u0 + d_k (arithmetic precision) from u0
-  HYDRO_DEV void stage_state(long long e, uint32_t col, const S q0[N], T u[N]) const {
-    if constexpr (MODE == CM_FIRST || MODE == CM_SINGLE) {
+  HYDRO_DEV void load_dk(long long e, uint32_t col, T dk[N]) const {
+    if constexpr (MODE == CM_MID || MODE == CM_FINAL) {
 #pragma unroll
-      for (int s = 0; s < N; ++s) u[s] = T(q0[s]);
+      for (int s = 0; s < N; ++s) dk[s] = ldp(a.dk, pl, s, e, col);
     } else if constexpr (MODE == CM_GENERIC) {
       if (a.dk) {
 #pragma unroll
-        for (int s = 0; s < N; ++s) u[s] = T(q0[s]) + ldp(a.dk, pl, s, e, col);
+        for (int s = 0; s < N; ++s) dk[s] = ldp(a.dk, pl, s, e, col);
       } else {
 #pragma unroll
-        for (int s = 0; s < N; ++s) u[s] = T(q0[s]);
+        for (int s = 0; s < N; ++s) dk[s] = T(0);
       }
-    } else {
+    }
+  }
+  // stage state u = u0 + d_k in arithmetic precision
+  HYDRO_DEV void stage_state(const S q0[N], const T dk[N], T u[N]) const {
 #pragma unroll
-      for (int s = 0; s < N; ++s) u[s] = T(q0[s]) + ldp(a.dk, pl, s, e, col);
+    for (int s = 0; s < N; ++s) {
+      if constexpr (kReadsDk) u[s] = T(q0[s]) + dk[s];
+      else u[s] = T(q0[s]);
     }
   }
 
-  HYDRO_DEV void store_final(long long e, uint32_t col, const T acc[N]) const {
+  // Operands of the combination of one cell, loaded ahead of the flux work.
+  struct Fin {
+    T dk[N];
+    T rep[kMaxPartials][N];
+    S hi[N];
+    float lo[N];
+    T pk[N];
+    T fv[kMaxPartials][N];
+  };
+
+  HYDRO_DEV void fetch_fin(long long e, uint32_t col, Fin& f) const {
+    constexpr bool kFinal = MODE == CM_FINAL || MODE == CM_SINGLE;
+    if constexpr (kReadsDk) load_dk(e, col, f.dk);
+    if constexpr (MODE == CM_FINAL || MODE == CM_GENERIC) {
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j)
+        if (j < a.nrep) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) f.rep[j][s] = ldp(a.rep[j], pl, s, e, col);
+        }
+    }
+    if constexpr (MODE == CM_GENERIC) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) f.pk[s] = a.pkp ? ldp(a.pkp, pl, s, e, col) : T(0);
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j)
+        if (j < a.nf) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) f.fv[j][s] = a.fload[j] ? ldp((const T*)a.fp[j], pl, s, e, col) : T(0);
+        }
+    }
+    if (kFinal || (MODE == CM_GENERIC && a.final_stage)) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        f.hi[s] = ldp(a.u0, pl, s, e, col);
+        if constexpr (KAHAN) f.lo[s] = ldp(a.u0lo, pl, s, e, col);
+      }
+    }
+  }
+
+  HYDRO_DEV void store_final(long long e, uint32_t col, const Fin& f, const T acc[N]) const {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-      const S hi = ldp(a.u0, pl, s, e, col);
+      const S hi = f.hi[s];
       if constexpr (KAHAN) {
         // compensated update of (hi, lo) by acc (Fast2Sum: |hi| >= |lo + acc|)
-        const float lo = ldp(a.u0lo, pl, s, e, col);
-        const float y = float(acc[s]) + lo;
+        const float y = float(acc[s]) + f.lo[s];
         const float t = float(hi) + y;
         const float err = y - (t - float(hi));
         stp(a.unew, pl, s, e, col, S(t));
@@ -120,63 +163,59 @@ struct StageCombine {
   }
 
   // combine the derivative of a cell into the increment / state
-  HYDRO_DEV void finish(long long e, uint32_t col, const T deriv[N]) const {
+  HYDRO_DEV void finish(long long e, uint32_t col, const Fin& f, const T deriv[N]) const {
     if constexpr (MODE == CM_FIRST) {
 #pragma unroll
       for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, bdt * deriv[s]);
     } else if constexpr (MODE == CM_MID) {
 #pragma unroll
-      for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, aself * ldp(a.dk, pl, s, e, col) + bdt * deriv[s]);
+      for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, aself * f.dk[s] + bdt * deriv[s]);
     } else if constexpr (MODE == CM_SINGLE) {
       T acc[N];
 #pragma unroll
       for (int s = 0; s < N; ++s) acc[s] = bdt * deriv[s];
-      store_final(e, col, acc);
+      store_final(e, col, f, acc);
     } else if constexpr (MODE == CM_FINAL) {
       T acc[N];
 #pragma unroll
-      for (int s = 0; s < N; ++s) acc[s] = aself * ldp(a.dk, pl, s, e, col) + bdt * deriv[s];
+      for (int s = 0; s < N; ++s) acc[s] = aself * f.dk[s] + bdt * deriv[s];
 #pragma unroll
-      for (int j = 0; j < kMaxPartials; ++j) {
+      for (int j = 0; j < kMaxPartials; ++j)
         if (j < a.nrep) {
 #pragma unroll
-          for (int s = 0; s < N; ++s) acc[s] += rc[j] * ldp(a.rep[j], pl, s, e, col);
+          for (int s = 0; s < N; ++s) acc[s] += rc[j] * f.rep[j][s];
         }
-      }
-      store_final(e, col, acc);
+      store_final(e, col, f, acc);
     } else {
-      T dk[N], acc[N];
+      T acc[N];
 #pragma unroll
-      for (int s = 0; s < N; ++s) dk[s] = a.dk ? ldp(a.dk, pl, s, e, col) : T(0);
+      for (int s = 0; s < N; ++s) acc[s] = f.pk[s] + aself * f.dk[s] + bdt * deriv[s];
 #pragma unroll
-      for (int s = 0; s < N; ++s) {
-        const T p = a.pkp ? ldp(a.pkp, pl, s, e, col) : T(0);
-        acc[s] = p + aself * dk[s] + bdt * deriv[s];
-      }
-#pragma unroll
-      for (int j = 0; j < kMaxPartials; ++j) {
+      for (int j = 0; j < kMaxPartials; ++j)
         if (j < a.nrep) {
 #pragma unroll
-          for (int s = 0; s < N; ++s) acc[s] += rc[j] * ldp(a.rep[j], pl, s, e, col);
+          for (int s = 0; s < N; ++s) acc[s] += rc[j] * f.rep[j][s];
         }
-      }
 #pragma unroll
-      for (int j = 0; j < kMaxPartials; ++j) {
+      for (int j = 0; j < kMaxPartials; ++j)
         if (j < a.nf) {
 #pragma unroll
-          for (int s = 0; s < N; ++s) {
-            const T v = a.fload[j] ? ldp((const T*)a.fp[j], pl, s, e, col) : T(0);
-            stp(a.fp[j], pl, s, e, col, v + am[j] * dk[s] + bm[j] * deriv[s]);
-          }
+          for (int s = 0; s < N; ++s) stp(a.fp[j], pl, s, e, col, f.fv[j][s] + am[j] * f.dk[s] + bm[j] * deriv[s]);
         }
-      }
       if (a.final_stage) {
-        store_final(e, col, acc);
+        store_final(e, col, f, acc);
       } else {
 #pragma unroll
         for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, acc[s]);
       }
     }
+  }
+
+  // load the operands, then combine (for callers that do not prefetch)
+  HYDRO_DEV void finish(long long e, uint32_t col, const T deriv[N]) const {
+    Fin f;
+    fetch_fin(e, col, f);
+    finish(e, col, f, deriv);
   }
 };
 

@@ -1,0 +1,262 @@
+// Solver base and the concrete solvers: the reference's virtual interface
+// (src/Solver/Solver.cpp, Roe.cpp, HLL.cpp, EulerRoe.cpp, EulerHLL.cpp,
+// EulerHLLC.cpp, SRHDRoe.cpp, MaxwellRoe.cpp) with bodies that call the C ABI
+// of libhydro_hip.so.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+
+#include "HydroGPU/Equation/Euler.h"
+#include "HydroGPU/Equation/Maxwell.h"
+#include "HydroGPU/Equation/SRHD.h"
+#include "HydroGPU/HydroGPUApp.h"
+#include "HydroGPU/Integrator/ForwardEuler.h"
+#include "HydroGPU/Integrator/RungeKutta.h"
+#include "HydroGPU/Solver/EulerHLL.h"
+#include "HydroGPU/Solver/EulerHLLC.h"
+#include "HydroGPU/Solver/EulerRoe.h"
+#include "HydroGPU/Solver/MaxwellRoe.h"
+#include "HydroGPU/Solver/SRHDRoe.h"
+
+namespace HydroGPU {
+
+void checkStatus(int status, const char* where) {
+  if (status != HYDRO_OK) throw Exception() << where << ": " << hydro_last_error();
+}
+
+namespace {
+
+// res/SlopeLimiter.cl:5-68, in the order of the HYDRO_LIMITER_* ids
+const char* const kLimiterNames[HYDRO_NUM_LIMITERS] = {
+    "DonorCell", "LaxWendroff", "BeamWarming", "Fromm", "CHARM", "HCUS", "HQUICK",
+    "Koren", "MinMod", "Oshker", "Ospre", "Smart", "Sweby", "UMIST", "VanAlbada1",
+    "VanAlbada2", "VanLeer", "MonotizedCentral", "Superbee", "BarthJespersen"};
+
+int limiterId(const std::string& name) {
+  for (int i = 0; i < HYDRO_NUM_LIMITERS; ++i)
+    if (name == kLimiterNames[i]) return i;
+  throw Exception() << "unknown slope limiter " << name;
+}
+
+// a `defs` entry as the kernels see it: Lua's tostring ("%.14g"), which is
+// how the reference renders defs into the program source
+// (src/Solver/Solver.cpp:289-307)
+real def(const HydroGPUApp* app, const char* key, real fallback) {
+  auto it = app->defs.find(key);
+  if (it == app->defs.end()) return fallback;
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.14g", it->second);
+  return std::strtod(buf, nullptr);
+}
+
+}  // namespace
+
+namespace Integrator {
+
+void Integrator::integrate(real dt) { solver->step(dt); }
+
+}  // namespace Integrator
+
+namespace Solver {
+
+Solver::Solver(HydroGPUApp* app_) : app(app_) {}
+
+Solver::~Solver() {
+  if (sim) hydro_sim_destroy(sim);
+  if (ctx) hydro_ctx_destroy(ctx);
+}
+
+void Solver::init() {
+  if (app->useGravity) throw Exception() << "self-gravity is not part of this build (useGravity must be false)";
+  createEquation();
+  if (!equation) throw Exception() << name() << " has no equation";
+
+  // src/Solver/Solver.cpp:178-204
+  typedef std::map<std::string, std::function<std::shared_ptr<Integrator::Integrator>()>> gensMap_t;
+  gensMap_t gens;
+#define MAKE_INTEGRATOR(I) gens[#I] = [this]() { return std::make_shared<Integrator::I>(this); }
+  MAKE_INTEGRATOR(ForwardEuler);
+  MAKE_INTEGRATOR(RungeKutta2);
+  MAKE_INTEGRATOR(RungeKutta2Heun);
+  MAKE_INTEGRATOR(RungeKutta2Ralston);
+  MAKE_INTEGRATOR(RungeKutta3);
+  MAKE_INTEGRATOR(RungeKutta4);
+  MAKE_INTEGRATOR(RungeKutta4_3_8thsRule);
+  MAKE_INTEGRATOR(RungeKutta2TVD);
+  MAKE_INTEGRATOR(RungeKutta2NonTVD);
+  MAKE_INTEGRATOR(RungeKutta3TVD);
+  MAKE_INTEGRATOR(RungeKutta4TVD);
+  MAKE_INTEGRATOR(RungeKutta4NonTVD);
+#undef MAKE_INTEGRATOR
+  auto i = gens.find(app->integratorName);
+  if (i == gens.end()) throw Exception() << "failed to find integrator named " << app->integratorName;
+  integrator = i->second();
+
+  hydro_sim_desc d;
+  hydro_sim_desc_default(&d);
+  d.dim = app->dim;
+  for (int a = 0; a < 3; ++a) {
+    d.size[a] = a < app->dim ? app->size[a] : 1;
+    d.xmin[a] = app->xmin[a];
+    d.xmax[a] = app->xmax[a];
+    for (int m = 0; m < 2; ++m) d.boundary[a][m] = app->boundaryMethods[a][m];
+  }
+  d.equation = equation->equationId();
+  d.solver = getSolverId();
+  d.limiter = limiterId(app->slopeLimiter);
+  d.integrator = integrator->integratorId();
+  d.precision = app->precision == PRECISION_FP32_MIXED ? HYDRO_PRECISION_FP32_MIXED : HYDRO_PRECISION_FP64;
+  d.cfl = app->cfl;
+  d.use_fixed_dt = app->useFixedDT ? 1 : 0;
+  d.fixed_dt = app->fixedDT;
+  d.gamma = def(app, "idealGas_heatCapacityRatio", d.gamma);
+  d.srhd_solve_prim_max_iter = (int)def(app, "srhd_solvePrimMaxIter", d.srhd_solve_prim_max_iter);
+  d.srhd_solve_prim_stop_epsilon = def(app, "srhd_solvePrimStopEpsilon", d.srhd_solve_prim_stop_epsilon);
+  d.srhd_solve_prim_vel_epsilon = def(app, "srhd_solvePrimVelEpsilon", d.srhd_solve_prim_vel_epsilon);
+  d.srhd_solve_prim_pmin_epsilon = def(app, "srhd_solvePrimPMinEpsilon", d.srhd_solve_prim_pmin_epsilon);
+  d.srhd_rho_min = def(app, "srhd_rhoMin", d.srhd_rho_min);
+  d.srhd_rho_max = def(app, "srhd_rhoMax", d.srhd_rho_max);
+  d.srhd_eint_max = def(app, "srhd_eIntMax", d.srhd_eint_max);
+  d.srhd_D_min = def(app, "srhd_DMin", d.srhd_D_min);
+  d.srhd_D_max = def(app, "srhd_DMax", d.srhd_D_max);
+  d.srhd_tau_min = def(app, "srhd_tauMin", d.srhd_tau_min);
+  d.srhd_tau_max = def(app, "srhd_tauMax", d.srhd_tau_max);
+  d.maxwell_permittivity = def(app, "maxwell_permittivity", d.maxwell_permittivity);
+  d.maxwell_permeability = def(app, "maxwell_permeability", d.maxwell_permeability);
+  d.maxwell_conductivity = def(app, "maxwell_conductivity", d.maxwell_conductivity);
+  d.flags = app->flags;
+
+  if (!ctx) checkStatus(hydro_ctx_create(app->device, &ctx), "hydro_ctx_create");
+  if (sim) {
+    hydro_sim_destroy(sim);
+    sim = nullptr;
+  }
+  checkStatus(hydro_sim_create(ctx, &d, &sim), "hydro_sim_create");
+  desc_ = d;
+  frame = 0;
+}
+
+int Solver::numStates() { return (int)equation->states.size(); }
+
+int Solver::getVolume() { return app->size[0] * app->size[1] * app->size[2]; }
+
+void Solver::boundary() { checkStatus(hydro_sim_boundary(sim), "boundary"); }
+
+real Solver::calcTimestep() {
+  double dt = 0;
+  checkStatus(hydro_sim_calc_timestep(sim, &dt), "calcTimestep");
+  return dt;
+}
+
+void Solver::step(real dt) { checkStatus(hydro_sim_step(sim, dt), "step"); }
+
+void Solver::update() {
+  boundary();
+  initStep();
+  const real dt = app->useFixedDT ? app->fixedDT : calcTimestep();
+  integrator->integrate(dt);
+  ++frame;
+}
+
+void Solver::resetState() {
+  if (!app->initState) throw Exception() << "expected initState to be defined in config file";
+  if (!sim) throw Exception() << "resetState before init";
+  const int n = numStates();
+  std::vector<real> stateVec((size_t)getVolume() * n);
+  std::vector<real> cellResults;
+  const int nch = equation->numReadStateChannels();
+  int flattenedIndex = 0;
+  int index[3];
+  for (index[2] = 0; index[2] < app->size[2]; ++index[2]) {
+    for (index[1] = 0; index[1] < app->size[1]; ++index[1]) {
+      for (index[0] = 0; index[0] < app->size[0]; ++index[0], ++flattenedIndex) {
+        real pos[3];
+        for (int i = 0; i < 3; ++i)
+          pos[i] = real(app->xmax[i] - app->xmin[i]) * (real(index[i]) + .5) / real(app->size[i]) + real(app->xmin[i]);
+        cellResults = app->initState(pos[0], pos[1], pos[2]);
+        if ((int)cellResults.size() < nch)
+          throw Exception() << "initState returned " << cellResults.size() << " values, expected " << nch;
+        equation->readStateCell(stateVec.data() + (size_t)flattenedIndex * n, cellResults.data());
+      }
+    }
+  }
+  setState(stateVec);
+}
+
+std::vector<real> Solver::getState() {
+  std::vector<real> v((size_t)getVolume() * numStates());
+  checkStatus(hydro_sim_get_state(sim, v.data(), v.size()), "getState");
+  return v;
+}
+
+void Solver::setState(const std::vector<real>& state) {
+  checkStatus(hydro_sim_set_state(sim, state.data(), state.size()), "setState");
+}
+
+void Solver::run(int nsteps) {
+  checkStatus(hydro_sim_run(sim, nsteps), "run");
+  frame += nsteps;
+}
+
+real Solver::getTime() {
+  double t = 0;
+  checkStatus(hydro_sim_time(sim, &t, nullptr, nullptr), "getTime");
+  return t;
+}
+
+long long Solver::getSteps() {
+  long long n = 0;
+  checkStatus(hydro_sim_time(sim, nullptr, &n, nullptr), "getSteps");
+  return n;
+}
+
+std::vector<real> Solver::totals() {
+  std::vector<real> v(numStates());
+  checkStatus(hydro_sim_totals(sim, v.data()), "totals");
+  return v;
+}
+
+unsigned Solver::health() {
+  unsigned h = 0;
+  checkStatus(hydro_sim_health(sim, &h), "health");
+  return h;
+}
+
+void EulerRoe::createEquation() { equation = std::make_shared<Equation::Euler>(app); }
+void EulerHLL::createEquation() { equation = std::make_shared<Equation::Euler>(app); }
+void SRHDRoe::createEquation() { equation = std::make_shared<Equation::SRHD>(app); }
+void MaxwellRoe::createEquation() { equation = std::make_shared<Equation::Maxwell>(app); }
+
+std::vector<real> SRHDRoe::getPrimitives() {
+  std::vector<real> v((size_t)getVolume() * numStates());
+  checkStatus(hydro_sim_get_primitives(handle(), v.data(), v.size()), "getPrimitives");
+  return v;
+}
+
+}  // namespace Solver
+
+HydroGPUApp::HydroGPUApp() {
+#define MAKE_SOLVER(S) SolverGenPair(#S, [this]() -> SolverPtr { return std::make_shared<Solver::S>(this); })
+  solverGensForEqns = {
+      SolverEqnsPair("Euler", {MAKE_SOLVER(EulerHLL), MAKE_SOLVER(EulerHLLC), MAKE_SOLVER(EulerRoe)}),
+      SolverEqnsPair("SRHD", {MAKE_SOLVER(SRHDRoe)}),
+      SolverEqnsPair("Maxwell", {MAKE_SOLVER(MaxwellRoe)}),
+  };
+#undef MAKE_SOLVER
+}
+
+HydroGPUApp::SolverPtr HydroGPUApp::createSolver() {
+  for (const SolverEqnsPair& p : solverGensForEqns)
+    for (const SolverGenPair& q : p.generators)
+      if (q.name == solverName) {
+        solver = q.func();
+        solver->init();
+        solver->resetState();
+        return solver;
+      }
+  throw Exception() << "unknown solver " << solverName;
+}
+
+}  // namespace HydroGPU

@@ -201,8 +201,11 @@ int hydro_sim_calc_timestep(hydro_sim* sim, double* dt_out);
  * src/Solver/HLL.cpp:234-241) including the integrator stages
  * (include/HydroGPU/Integrator/RungeKutta.h:59-125,
  * src/Integrator/ForwardEuler.cpp:8-32) and the per-solver tail
- * (gravity/source pass, SRHD constrainState + updatePrimitives).
- * Must follow hydro_sim_calc_timestep (the frozen eigen data). */
+ * (gravity/source pass, SRHD constrainState + updatePrimitives), and
+ * advances the simulated time by dt.  The frozen eigen data (Roe basis, HLL
+ * wave speeds) are those of the state the step starts from; the reference
+ * reuses the basis of its last calcTimestep call, which is the same state
+ * whenever each step follows a calcTimestep, as in Solver::update. */
 int hydro_sim_step(hydro_sim* sim, double dt);
 
 /* Replaces Solver::update (src/Solver/Solver.cpp:521-545):
@@ -226,7 +229,8 @@ int hydro_sim_time(hydro_sim* sim, double* t, long long* steps, double* last_dt)
 int hydro_sim_totals(hydro_sim* sim, double* totals_out);
 
 /* Non-zero per-step health flags accumulated since creation:
- * bit0 non-finite dt, bit1 non-positive density, bit2 con2prim not converged. */
+ * bit0 non-finite or non-positive dt, or a non-finite wave speed in the CFL;
+ * bit2 con2prim not converged. */
 int hydro_sim_health(hydro_sim* sim, unsigned* flags_out);
 
 /* Device HIP stream of the sim (hipStream_t as void*), for callers that want

@@ -41,6 +41,7 @@ def lib():
             getattr(L, n).argtypes = [vp, dp, sz]
             getattr(L, n).restype = ip
         L.oracle_boundary.argtypes = [vp]
+        L.oracle_init_step.argtypes = [vp]
         L.oracle_calc_timestep.argtypes = [vp]
         L.oracle_calc_timestep.restype = ctypes.c_double
         L.oracle_step.argtypes = [vp, ctypes.c_double]
@@ -119,6 +120,9 @@ class OracleSim:
 
     def boundary(self):
         lib().oracle_boundary(self._h)
+
+    def init_step(self):
+        lib().oracle_init_step(self._h)
 
     def calc_timestep(self):
         return lib().oracle_calc_timestep(self._h)

@@ -1212,6 +1212,12 @@ static void integrate(oracle_sim* s, double dt, callback_t cb) {
 
 /* ------------------------------------------------ solver driver */
 
+/* Solver::initStep: HLL freezes its wave speeds (src/Solver/HLL.cpp:225-227);
+ * Roe has nothing to do before calcTimestep. */
+void oracle_init_step(oracle_sim* s) {
+  if (s->d.solver != HYDRO_SOLVER_ROE) hll_calc_eigenvalues(s);
+}
+
 double oracle_calc_timestep(oracle_sim* s) {
   if (s->d.solver == HYDRO_SOLVER_ROE) {
     /* Roe::calcTimestep (src/Solver/Roe.cpp:88-92) */
@@ -1240,7 +1246,7 @@ void oracle_step(oracle_sim* s, double dt) {
 /* Solver::update (src/Solver/Solver.cpp:521-545) */
 double oracle_update(oracle_sim* s) {
   oracle_boundary(s);
-  if (s->d.solver != HYDRO_SOLVER_ROE) hll_calc_eigenvalues(s); /* HLL::initStep */
+  oracle_init_step(s); /* Solver::initStep */
   double dt;
   if (s->d.use_fixed_dt) {
     dt = s->d.fixed_dt;

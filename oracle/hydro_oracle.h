@@ -32,6 +32,7 @@ int oracle_get_state(const oracle_sim* s, double* aos, size_t n);
 int oracle_get_primitives(const oracle_sim* s, double* aos, size_t n);
 int oracle_set_potential(oracle_sim* s, const double* pot, size_t n);
 void oracle_boundary(oracle_sim* s);
+void oracle_init_step(oracle_sim* s);
 double oracle_calc_timestep(oracle_sim* s);
 void oracle_step(oracle_sim* s, double dt);
 double oracle_update(oracle_sim* s);

@@ -1,0 +1,205 @@
+// host_api_test.cpp -- the C++ host layer (HydroGPU::Solver & co.) driven
+// the way the reference's HydroGPUApp drives its solvers, checked against the
+// oracle (the FP64 CPU restatement, linked as test infrastructure).
+//
+//   host_api_test cpu   configuration errors that throw before any device work
+//   host_api_test gpu   Sod (Euler Roe, TVD-RK2, Superbee, free-flow, CFL 0.5,
+//                       to t = 0.2) and short runs of every solver class,
+//                       each compared with the oracle on the same descriptor
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "HydroGPU/HydroGPUApp.h"
+#include "HydroGPU/Solver/Solver.h"
+#include "../../oracle/hydro_oracle.h"
+
+using HydroGPU::real;
+
+static int failures = 0;
+
+#define CHECK(cond, ...)                    \
+  do {                                      \
+    if (!(cond)) {                          \
+      std::printf("FAIL %s:%d: ", __FILE__, __LINE__); \
+      std::printf(__VA_ARGS__);             \
+      std::printf("\n");                    \
+      ++failures;                           \
+    }                                       \
+  } while (0)
+
+template <typename F>
+static bool throws(F f, const char* expect) {
+  try {
+    f();
+  } catch (const HydroGPU::Exception& e) {
+    if (std::strstr(e.what(), expect)) return true;
+    std::printf("  threw '%s', expected '%s'\n", e.what(), expect);
+    return false;
+  }
+  return false;
+}
+
+// buildStateEuler (res/util.lua:36-80): rho, rho v, B, E
+static std::vector<real> euler_cell(real rho, real vx, real vy, real p, real gamma) {
+  std::vector<real> c(10, 0.0);
+  c[0] = rho;
+  c[1] = rho * vx;
+  c[2] = rho * vy;
+  c[7] = rho * .5 * (vx * vx + vy * vy) + p / (gamma - 1.0);
+  return c;
+}
+
+static real relerr(const std::vector<real>& a, const std::vector<real>& b) {
+  real d = 0, m = 0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    d = std::fmax(d, std::fabs(a[i] - b[i]));
+    m = std::fmax(m, std::fabs(b[i]));
+  }
+  return d / m;
+}
+
+static int run_cpu() {
+  HydroGPU::HydroGPUApp app;
+  app.solverName = "EulerBurgers";  // in the reference's registry, not in this build
+  CHECK(throws([&] { app.createSolver(); }, "unknown solver EulerBurgers"), "unknown solver");
+  app.solverName = "EulerRoe";
+  app.integratorName = "BackwardEulerConjugateGradient";
+  app.size = {{64, 1, 1, 1}};
+  CHECK(throws([&] { app.createSolver(); }, "failed to find integrator named BackwardEulerConjugateGradient"),
+        "unknown integrator");
+  app.integratorName = "RungeKutta4";
+  app.slopeLimiter = "NoSuchLimiter";
+  CHECK(throws([&] { app.createSolver(); }, "unknown slope limiter NoSuchLimiter"), "unknown limiter");
+  app.slopeLimiter = "Superbee";
+  app.useGravity = true;
+  CHECK(throws([&] { app.createSolver(); }, "self-gravity"), "gravity");
+  std::printf("%s cpu checks\n", failures ? "FAILED" : "passed");
+  return failures ? 1 : 0;
+}
+
+// one solver run through the host classes, the same descriptor through the oracle
+static void compare(HydroGPU::HydroGPUApp& app, int updates, real tol, const char* what) {
+  auto solver = app.createSolver();
+  oracle_sim* o = oracle_create(&solver->desc(), 0);
+  std::vector<real> q0 = solver->getState();
+  oracle_set_state(o, q0.data(), q0.size());
+  real t = 0;
+  for (int i = 0; i < updates; ++i) {
+    solver->update();
+    t += oracle_update(o);
+  }
+  std::vector<real> ref(q0.size());
+  oracle_get_state(o, ref.data(), ref.size());
+  const real e = relerr(solver->getState(), ref);
+  CHECK(e < tol, "%s: rel err %.3e >= %.1e", what, e, tol);
+  CHECK(std::fabs(solver->getTime() - t) <= 1e-12 * t, "%s: time %.17g vs oracle %.17g", what, solver->getTime(), t);
+  CHECK(solver->health() == 0, "%s: health %u", what, solver->health());
+  std::printf("%-34s %4d updates  rel err %.2e  t %.6f\n", what, updates, e, solver->getTime());
+  oracle_destroy(o);
+}
+
+static int run_gpu() {
+  const real gamma = 1.4;
+  {
+    // Sod: the 1D CPU-runnable configuration
+    HydroGPU::HydroGPUApp app;
+    app.solverName = "EulerRoe";
+    app.integratorName = "RungeKutta2TVD";
+    app.slopeLimiter = "Superbee";
+    app.dim = 1;
+    app.size = {{1024, 1, 1, 1}};
+    app.xmin[0] = -1;
+    app.xmax[0] = 1;
+    app.cfl = .5f;
+    app.defs["idealGas_heatCapacityRatio"] = gamma;
+    app.initState = [&](real x, real, real) {
+      return x < 0 ? euler_cell(1.0, 0, 0, 1.0, gamma) : euler_cell(.125, 0, 0, .1, gamma);
+    };
+    auto solver = app.createSolver();
+    oracle_sim* o = oracle_create(&solver->desc(), 0);
+    std::vector<real> q0 = solver->getState();
+    oracle_set_state(o, q0.data(), q0.size());
+    int steps = 0;
+    while (solver->getTime() < 0.2) {
+      solver->update();
+      oracle_update(o);
+      ++steps;
+    }
+    std::vector<real> ref(q0.size());
+    oracle_get_state(o, ref.data(), ref.size());
+    const real e = relerr(solver->getState(), ref);
+    CHECK(e < 1e-10, "Sod: rel err %.3e", e);
+    CHECK(std::fabs(oracle_time(o) - solver->getTime()) < 1e-12, "Sod: time");
+    std::printf("%-34s %4d updates  rel err %.2e  t %.6f\n", "EulerRoe Sod 1024 RK2TVD", steps, e, solver->getTime());
+    // run(n) (no per-step synchronisation) continues where update() left off
+    solver->run(5);
+    for (int i = 0; i < 5; ++i) oracle_update(o);
+    oracle_get_state(o, ref.data(), ref.size());
+    CHECK(relerr(solver->getState(), ref) < 1e-10, "Sod run(5)");
+    CHECK(solver->getSteps() == steps + 5, "step count %lld", solver->getSteps());
+    oracle_destroy(o);
+  }
+  auto kh = [&](real x, real y, real) {
+    const bool in = std::fabs(y) < .25;
+    return euler_cell(in ? 2.0 : 1.0, in ? .5 : -.5, .01 * std::sin(4 * M_PI * x), 2.5, gamma);
+  };
+  for (const char* name : {"EulerRoe", "EulerHLL", "EulerHLLC"}) {
+    HydroGPU::HydroGPUApp app;
+    app.solverName = name;
+    app.integratorName = "RungeKutta3TVD";
+    app.slopeLimiter = "MonotizedCentral";
+    app.dim = 2;
+    app.size = {{72, 40, 1, 1}};
+    for (int a = 0; a < 2; ++a)
+      for (int m = 0; m < 2; ++m) app.boundaryMethods[a][m] = HydroGPU::BOUNDARY_METHOD_PERIODIC;
+    app.initState = kh;
+    compare(app, 6, 1e-10, (std::string(name) + " KH 72x40 RK3TVD").c_str());
+  }
+  {
+    HydroGPU::HydroGPUApp app;
+    app.solverName = "SRHDRoe";
+    app.integratorName = "RungeKutta2";
+    app.dim = 2;
+    app.size = {{40, 36, 1, 1}};
+    app.xmin = {{0, 0, 0, 0}};
+    app.xmax = {{1, 1, 1, 0}};
+    app.defs["idealGas_heatCapacityRatio"] = 5. / 3.;
+    app.initState = [&](real x, real y, real) {
+      const bool r = x > .5, u = y > .5;
+      return euler_cell(r == u ? 1.0 : .5, r ? .3 : -.2, u ? .1 : -.3, r ? 1.0 : .4, 5. / 3.);
+    };
+    compare(app, 4, 1e-9, "SRHDRoe quadrants 40x36 RK2");
+  }
+  {
+    HydroGPU::HydroGPUApp app;
+    app.solverName = "MaxwellRoe";
+    app.integratorName = "RungeKutta4";
+    app.dim = 1;
+    app.size = {{128, 1, 1, 1}};
+    for (int m = 0; m < 2; ++m) app.boundaryMethods[0][m] = HydroGPU::BOUNDARY_METHOD_PERIODIC;
+    app.initState = [&](real x, real, real) {
+      std::vector<real> c(6, 0.0);
+      c[1] = std::sin(2 * M_PI * x);  // E_y
+      c[5] = std::sin(2 * M_PI * x);  // B_z
+      return c;
+    };
+    compare(app, 5, 1e-10, "MaxwellRoe wave 128 RK4");
+  }
+  std::printf("%s gpu checks\n", failures ? "FAILED" : "passed");
+  return failures ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+  const std::string mode = argc > 1 ? argv[1] : "cpu";
+  try {
+    if (mode == "gpu") return run_gpu();
+    return run_cpu();
+  } catch (const std::exception& e) {
+    std::printf("FAIL: exception %s\n", e.what());
+    return 1;
+  }
+}
