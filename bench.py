@@ -27,7 +27,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-sys.path.insert(0, os.path.join(ROOT, "tests"))  # exact_riemann (closed-form Sod solution)
 
 HBM_PEAK_GBS = 8000.0
 
@@ -44,6 +43,9 @@ def parse():
     ap.add_argument("--kahan", type=int, default=0,
                     help="headline path with compensated FP32 state storage (1) or plain FP32 (0)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the Sod L1 accuracy leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 512-grid leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (Sod 1D, 3D HLLC blast, SRHD quadrants)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the HBM "
                          "byte counters of a profiler run against a known transfer), then exit")
@@ -105,7 +107,7 @@ def sod_l1(precision, kahan, n=1024):
     """L1 density error against the exact solution of the 1D Sod tube (Euler
     Roe, TVD-RK2, Superbee, free-flow, CFL 0.5, t = 0.2, 1024 cells on
     [-1, 1]) on the GPU path of the given precision."""
-    from exact_riemann import sample
+    from hydrodynamicsgpu_amd.riemann import sample
     from hydrodynamicsgpu_amd import capi, config, ics
 
     flags = config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND | (config.FLAG_KAHAN if kahan else 0)
@@ -121,6 +123,182 @@ def sod_l1(precision, kahan, n=1024):
     l1 = float(np.sum(np.abs(rho - er)[2:n - 2]) * (2.0 / n))
     sim.close()
     return l1
+
+
+def batch_leg(steps, warmup):
+    """The uneven 512-grid mix (1D lines of 256..65536 cells and 2D squares of
+    64^2..512^2, Riemann problems, Euler Roe, Superbee, RK2, FP32-mixed) in
+    one batch, after the autotuner picked its decomposition, against the
+    single 2048^2 grid with the same scheme (the large-grid throughput)."""
+    from hydrodynamicsgpu_amd import capi, config, ics
+
+    kw = dict(integratorName="RungeKutta2", slopeLimiter="Superbee", precision="fp32_mixed")
+    mix = ics.riemann_batch(512, **kw)
+    b = capi.Batch([d for d, _ in mix])
+    for m, (_, q) in enumerate(mix):
+        b.set_state(m, q)
+    rows, strips, _ = b.autotune()
+    b.run(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    cells = b.interior_cells()
+    health = max(b.health(m) for m in range(len(mix)))
+    b.close()
+    # the large grid, same scheme
+    n = 2048
+    d = config.make_desc([n, n], xmin=[-0.5, -0.5], xmax=[0.5, 0.5],
+                         boundaryMethods=[{"min": "PERIODIC", "max": "PERIODIC"}] * 2, cfl=0.5, **kw)
+    sim = capi.Sim(d)
+    sim.set_state(ics.shear_kh(d))
+    sim.run(warmup)
+    sim.sync()
+    t1 = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    el1 = time.perf_counter() - t1
+    sim.close()
+    big = (n - 4) ** 2 * steps / el1
+    v = cells * steps / el
+    return {"workload": "512 uneven Euler Roe grids (1D 256..65536, 2D 64^2..512^2), Superbee, RK2, fp32_mixed",
+            "members": len(mix), "interior_cells": cells, "value": v, "unit": "cell-updates/s",
+            "ms_per_step": el * 1e3 / steps, "decomposition": {"rows": rows, "strips_1d": strips},
+            "large_grid_value": big, "large_grid": "2048^2 KH, same scheme", "fraction_of_large_grid": v / big,
+            "health": health}
+
+
+def time_desc(d, q0, steps, warmup):
+    """Timed run of one configuration on the GPU: `steps` updates after
+    `warmup`, bracketed by synchronisation, then the same with HIP events on
+    the stage launches."""
+    from hydrodynamicsgpu_amd import capi
+
+    sim = capi.Sim(d)
+    sim.set_state(q0)
+    sim.run(warmup)
+    sync_all(sim)
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sync_all(sim)
+    el = time.perf_counter() - t0
+    sim.set_profiling(True)
+    sim.run(max(2, steps // 4))
+    stage_ms, launches = sim.stage_timing()
+    sim.set_profiling(False)
+    out = dict(elapsed=el, stage_ms=stage_ms, launches=launches, health=sim.health(), state=sim.get_state())
+    sim.close()
+    return out
+
+
+def cpu_steps(d, q0, seconds, max_steps):
+    """The FP64 oracle on the host cores: cell-updates/s over a bounded number of steps."""
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.lib().oracle_set_threads(threads)
+    o = oracle.OracleSim(d)
+    o.set_state(q0)
+    o.update()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.update()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= max_steps:
+            break
+    return n, el, threads
+
+
+def config_legs(steps, warmup, with_cpu, cpu_seconds):
+    """The other configurations: Sod 1D (FP64, to t = 0.2, L1 vs the exact
+    solution), the 3D HLLC blast (256^3, TVD-RK3, mixed with Kahan vs FP64)
+    and the SRHD quadrants (1024^2, RK2, FP32 fluxes + FP64 con2prim vs FP64).
+    B per cell-update: N b (3S - 1), SRHD + 4 N 8 for the FP64 primitives."""
+    from hydrodynamicsgpu_amd import capi, config, ics
+    from hydrodynamicsgpu_amd.riemann import sample
+
+    res = {}
+    # C1: Sod, 1024 cells incl. ghosts on [-1, 1], TVD-RK2, Superbee, CFL 0.5, t = 0.2
+    n = 1024
+    kw = dict(xmin=[-1], xmax=[1], integratorName="RungeKutta2TVD", slopeLimiter="Superbee", cfl=0.5, t_end=0.2)
+    d = config.make_desc([n], flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND, **kw)
+    q = ics.sod(d)
+    sim = capi.Sim(d)
+    sim.set_state(q)
+    sim.run(2)
+    sim.set_state(q)
+    sync_all(sim)
+    t0 = time.perf_counter()
+    while sim.time()[0] < 0.2 - 1e-12:
+        sim.run(64)
+    el = time.perf_counter() - t0
+    _, nsteps, _ = sim.time()
+    rho, _, _ = ics.primitives_euler(d, sim.get_state())
+    x = ics.cell_centers(d)[0]
+    er, _, _ = sample((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4, x / 0.2)
+    l1 = float(np.sum(np.abs(rho - er)[2:n - 2]) * (2.0 / n))
+    sim.close()
+    c1 = {"workload": "euler1d_roe_sod_1024_rk2tvd_superbee_t0.2", "precision": "fp64", "steps": nsteps,
+          "gpu_value": (n - 4) * nsteps / el, "unit": "cell-updates/s", "gpu_seconds_to_t": el, "l1_rho_vs_exact": l1,
+          "note": "latency-bound: 1020 cells per step, the host waits for t each 64 steps"}
+    if with_cpu:
+        import oracle
+
+        o = oracle.OracleSim(d)
+        o.set_state(q)
+        ocount = 0
+        t0 = time.perf_counter()
+        while o.time < 0.2 - 1e-14:
+            o.update()
+            ocount += 1
+        c1["cpu_seconds_to_t"] = time.perf_counter() - t0
+        c1["cpu_value"] = (n - 4) * ocount / c1["cpu_seconds_to_t"]
+        c1["cpu_cores"] = 1
+    res["C1"] = c1
+
+    def pair(name, workload, make, ic, N, S, b_extra=0, mixed_flags=0):
+        d64 = make("fp64", 0)
+        q0 = ic(d64)
+        cells = int(np.prod([sz - 4 for sz in list(d64.size)[:d64.dim]]))
+        g64 = time_desc(d64, q0, steps, warmup)
+        g32 = time_desc(make("fp32_mixed", mixed_flags), q0, steps, warmup)
+        v64 = cells * steps / g64["elapsed"]
+        v32 = cells * steps / g32["elapsed"]
+        B32 = N * 4 * (3 * S - 1) + b_extra
+        B64 = N * 8 * (3 * S - 1) + b_extra
+        a32 = cells * B32 / S / (g32["stage_ms"] * 1e-3) / 1e9
+        a64 = cells * B64 / S / (g64["stage_ms"] * 1e-3) / 1e9
+        diff = np.abs(g32["state"] - g64["state"]).reshape(-1, N)[:, 0]
+        out = {"workload": workload, "interior_cells": cells, "gpu_fp64_value": v64, "gpu_mixed_value": v32,
+               "unit": "cell-updates/s", "mixed_over_fp64": v32 / v64,
+               "stage_kernel_avg_ms": {"fp64": g64["stage_ms"], "mixed": g32["stage_ms"]},
+               "roofline_frac_mixed": a32 / HBM_PEAK_GBS, "roofline_frac_fp64": a64 / HBM_PEAK_GBS,
+               "B_mixed": B32, "B_fp64": B64, "l1_rho_mixed_minus_fp64": float(np.mean(diff)),
+               "health": {"fp64": g64["health"], "mixed": g32["health"]}}
+        if with_cpu:
+            ns, el, th = cpu_steps(d64, q0, cpu_seconds, 3)
+            out["cpu_value"] = cells * ns / el
+            out["cpu_cores"] = th
+        res[name] = out
+
+    def c3(prec, flags):
+        return config.make_desc([256, 256, 256], xmin=[-1] * 3, xmax=[1] * 3, solverName="EulerHLLC",
+                                integratorName="RungeKutta3TVD", slopeLimiter="MinMod",
+                                boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4, precision=prec,
+                                flags=config.FLAG_REFERENCE | flags)
+
+    pair("C3", "euler3d_hllc_blast_256^3_rk3tvd", c3, ics.blast, 5, 3, mixed_flags=config.FLAG_KAHAN)
+
+    def c4(prec, flags):
+        return config.make_desc([1024, 1024], xmin=[0, 0], xmax=[1, 1], solverName="SRHDRoe",
+                                integratorName="RungeKutta2", defs={"idealGas_heatCapacityRatio": 5.0 / 3.0},
+                                precision=prec, flags=config.FLAG_REFERENCE | flags)
+
+    pair("C4", "srhd2d_roe_quadrants_1024^2_rk2", c4, ics.srhd_quadrants, 4, 2, b_extra=4 * 8 * 4)
+    return res
 
 
 def cpu_baseline(n, seconds, q0):
@@ -229,6 +407,10 @@ def main():
         out["fp32_" + ("plain" if a.kahan else "kahan")] = {
             "value": cells * a.steps / o["elapsed"], "ms_per_step": o["elapsed"] * 1e3 / a.steps,
             "stage_kernel_avg_ms": o["stage_ms"]}
+    if not a.no_batch:
+        out["batch"] = batch_leg(a.steps, a.warmup)
+    if not a.no_configs:
+        out["configs"] = config_legs(a.steps, a.warmup, not a.no_cpu, min(a.cpu_seconds, 10.0))
     if not a.no_accuracy:
         # L1 error at fixed resolution against the exact Sod solution: the
         # mixed path must stay within 2x of the FP64 path

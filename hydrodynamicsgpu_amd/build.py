@@ -16,13 +16,18 @@ LIB = os.path.join(HERE, "libhydro_hip.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 ARCH = os.environ.get("HYDRO_OFFLOAD_ARCH", "gfx950")
+# -ffp-contract=on: a*b+c is fused where one expression says so, never across
+# statements, so a cell's arithmetic does not depend on where the compiler
+# inlined it: results are the same for every work decomposition (chunk
+# height, strips per task) and for a grid alone or inside a batch.
 COMMON = [
-    "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", CSRC, "-I", INCLUDE, "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["stage2d_f64.hip", "stage2d_f32.hip", "stage2d_f32k.hip", "stage3d.hip", "aux_kernels.hip", "capi.cpp"]
+SOURCES = ["stage2d_f64.hip", "stage2d_f32.hip", "stage2d_f32k.hip", "stage3d.hip", "aux_kernels.hip",
+           "batch_f64.hip", "batch_f32.hip", "batch_f32k.hip", "batch_aux.hip", "batch.cpp", "capi.cpp"]
 
 
 def _hipcc():

@@ -61,6 +61,10 @@ def lib():
         L.hydro_batch_member_dt.argtypes = [vp, ip, dp]
         L.hydro_batch_interior_cells.argtypes = [vp]
         L.hydro_batch_interior_cells.restype = ctypes.c_longlong
+        L.hydro_batch_member_time.argtypes = [vp, ip, dp, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.hydro_batch_member_health.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_uint)]
+        L.hydro_batch_set_decomposition.argtypes = [vp, ip, ip]
+        L.hydro_batch_autotune.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), dp]
         _lib = L
     return _lib
 
@@ -184,7 +188,68 @@ EXPORTED_SYMBOLS = [
     "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
     "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_batch_create", "hydro_batch_destroy", "hydro_batch_set_state",
     "hydro_batch_get_state", "hydro_batch_run", "hydro_batch_sync", "hydro_batch_member_dt",
-    "hydro_batch_interior_cells",
+    "hydro_batch_interior_cells", "hydro_batch_member_time", "hydro_batch_member_health",
+    "hydro_batch_set_decomposition", "hydro_batch_autotune",
 ]
 
-__all__ = ["Sim", "HydroError", "lib", "check", "context", "config", "EXPORTED_SYMBOLS", "LIB_PATH"]
+__all__ = ["Sim", "Batch", "HydroError", "lib", "check", "context", "config", "EXPORTED_SYMBOLS", "LIB_PATH"]
+
+
+class Batch:
+    """Many independent Euler grids advanced together (hydro_batch_*)."""
+
+    def __init__(self, descs, device=0):
+        self.descs = list(descs)
+        arr = (config.HydroSimDesc * len(self.descs))(*self.descs)
+        self._h = ctypes.c_void_p()
+        check(lib().hydro_batch_create(context(device), ctypes.byref(arr), len(self.descs), ctypes.byref(self._h)))
+        self.sizes = [int(np.prod(list(d.size)[:d.dim])) * (d.dim + 2) for d in self.descs]
+
+    def close(self):
+        if self._h:
+            lib().hydro_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, member, aos):
+        aos = np.ascontiguousarray(aos, dtype=np.float64)
+        check(lib().hydro_batch_set_state(self._h, member, _dp(aos), aos.size))
+
+    def get_state(self, member):
+        out = np.empty(self.sizes[member])
+        check(lib().hydro_batch_get_state(self._h, member, _dp(out), out.size))
+        return out
+
+    def run(self, nsteps, graph=True):
+        check(lib().hydro_batch_run(self._h, nsteps, 1 if graph else 0))
+
+    def sync(self):
+        check(lib().hydro_batch_sync(self._h))
+
+    def time(self, member):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_longlong()
+        check(lib().hydro_batch_member_time(self._h, member, ctypes.byref(t), ctypes.byref(n), ctypes.byref(dt)))
+        return t.value, n.value, dt.value
+
+    def health(self, member):
+        h = ctypes.c_uint()
+        check(lib().hydro_batch_member_health(self._h, member, ctypes.byref(h)))
+        return h.value
+
+    def interior_cells(self):
+        return lib().hydro_batch_interior_cells(self._h)
+
+    def set_decomposition(self, rows, strips):
+        check(lib().hydro_batch_set_decomposition(self._h, rows, strips))
+
+    def autotune(self):
+        r, k = ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        check(lib().hydro_batch_autotune(self._h, ctypes.byref(r), ctypes.byref(k), ctypes.byref(ms)))
+        return r.value, k.value, ms.value

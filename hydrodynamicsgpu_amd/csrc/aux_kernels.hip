@@ -7,131 +7,15 @@
 #include "policies.hpp"
 #include "launch.hpp"
 #include "stage_common.hpp"
+#include "aux_device.hpp"
 
 namespace hydro {
-
-// ---------------------------------------------------------------- boundary
-// res/Common.cl:49-194 applies, per axis in order x -> y -> z, per state, the
-// min side then the max side, each pass covering every line of the grid
-// (ghost lines of the other axes included).  Every source cell of those maps
-// is interior along the pass's axis, so the value a ghost cell ends with is
-// the composition of the per-axis index maps: for each axis on which the
-// cell is a ghost, replace that coordinate by its source and multiply by -1
-// for a reflect.  One thread per ghost cell evaluates that composition
-// directly, so the whole boundary is one launch with no ordering between
-// threads (sources are never written by the launch).
-// method: 0 periodic, 1 mirror (a reflect is a mirror whose state is
-// negated), 2 freeflow (chained copy: both ghosts take the first interior
-// cell), -1 none.
-HYDRO_DEV int ghost_source(int method, int k, int n) {
-  if (k < 2) {
-    if (method == 0) return n - 4 + k;   // 0 <- n-4, 1 <- n-3
-    if (method == 2) return 2;           // 1 <- 2, 0 <- 1
-    return 3 - k;                        // 0 <- 3, 1 <- 2
-  }
-  if (method == 0) return k - (n - 4);   // n-2 <- 2, n-1 <- 3
-  if (method == 2) return n - 3;         // n-2 <- n-3, n-1 <- n-2
-  return 2 * n - 5 - k;                  // n-1 <- n-4, n-2 <- n-3
-}
-
-// ---------------------------------------------------------------- ghosts
-// Enumerates exactly the ghost cells (some coordinate in {0,1,S-2,S-1}).
-HYDRO_DEV bool ghost_cell(int g, int dim, int sx, int sy, int sz, long long& cell) {
-  auto pick = [](int k, int n) { return k < 2 ? k : n - 4 + k; };  // k in 0..3
-  if (dim == 1) {
-    if (g >= 4) return false;
-    cell = pick(g, sx);
-    return true;
-  }
-  if (dim == 2) {
-    const int rows = 4 * sx;
-    if (g < rows) {
-      const int y = pick(g / sx, sy), x = g % sx;
-      cell = (long long)y * sx + x;
-      return true;
-    }
-    g -= rows;
-    if (g >= 4 * (sy - 4)) return false;
-    const int y = 2 + g / 4, x = pick(g % 4, sx);
-    cell = (long long)y * sx + x;
-    return true;
-  }
-  const long long planes = 4LL * sx * sy;
-  if (g < planes) {
-    const int z = pick((int)(g / ((long long)sx * sy)), sz);
-    const long long r = g % ((long long)sx * sy);
-    cell = (long long)z * sx * sy + r;
-    return true;
-  }
-  long long h = g - planes;
-  const long long rows = 4LL * sx * (sz - 4);
-  if (h < rows) {
-    const int z = 2 + (int)(h / (4LL * sx));
-    const long long r = h % (4LL * sx);
-    const int y = pick((int)(r / sx), sy), x = (int)(r % sx);
-    cell = ((long long)z * sy + y) * sx + x;
-    return true;
-  }
-  h -= rows;
-  if (h >= 4LL * (sy - 4) * (sz - 4)) return false;
-  const int z = 2 + (int)(h / (4LL * (sy - 4)));
-  const long long r = h % (4LL * (sy - 4));
-  const int y = 2 + (int)(r / 4), x = pick((int)(r % 4), sx);
-  cell = ((long long)z * sy + y) * sx + x;
-  return true;
-}
-
-HYDRO_DEV long long ghost_count(int dim, int sx, int sy, int sz) {
-  if (dim == 1) return 4;
-  if (dim == 2) return 4LL * sx + 4LL * (sy - 4);
-  return 4LL * sx * sy + 4LL * sx * (sz - 4) + 4LL * (sy - 4) * (sz - 4);
-}
 
 template <typename S>
 __global__ void __launch_bounds__(256) boundary_kernel(BoundaryArgs b, S* planes, float* lo) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long cell;
   if (g >= ghost_count(b.dim, b.sx, b.sy, b.sz)) return;
-  if (!ghost_cell((int)g, b.dim, b.sx, b.sy, b.sz, cell)) return;
-  const int x = (int)(cell % b.sx);
-  const long long yz = cell / b.sx;
-  const int y = (int)(yz % b.sy);
-  const int z = (int)(yz / b.sy);
-  int sc[3] = {x, y, z};
-  const int n[3] = {b.sx, b.sy, b.sz};
-  // one source cell for every state (mirror and reflect share the index
-  // map); per-state signs from the reflect masks of the ghost axes
-  unsigned neg = 0;
-  bool write = false;
-  for (int a = 0; a < b.dim; ++a) {
-    const int k = sc[a];
-    if (k >= 2 && k < n[a] - 2) continue;
-    const int side = k < 2 ? 0 : 1;
-    const int m = b.method[a][side];
-    if (m < 0) continue;
-    write = true;
-    neg ^= b.negmask[a][side];
-    sc[a] = ghost_source(m, k, n[a]);
-  }
-  if (!write) return;
-  const long long src = sc[0] + (long long)b.sx * (sc[1] + (long long)b.sy * sc[2]);
-  S v[8];
-  float w[8];
-#pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    if (s < b.nstates) {
-      v[s] = planes[s * b.plane + src];
-      if (lo) w[s] = lo[s * b.plane + src];
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    if (s < b.nstates) {
-      const bool ng = (neg >> s) & 1u;
-      planes[s * b.plane + cell] = ng ? -v[s] : v[s];
-      if (lo) lo[s * b.plane + cell] = ng ? -w[s] : w[s];
-    }
-  }
+  boundary_cell<S>(b, planes, lo, g);
 }
 
 template <typename S>
@@ -174,20 +58,6 @@ void launch_copy_ghosts_f32(int dim, int sx, int sy, int sz, long long plane, in
 }
 
 // ---------------------------------------------------------------- CFL
-HYDRO_DEV double block_min(double v, double* sh) {
-  // wave min via shuffles, then across the block's waves through LDS
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) sh[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = blockDim.x >> 6;
-    for (int i = 1; i < nw; ++i) v = fmin(v, sh[i]);
-  }
-  return v;
-}
-
 template <typename S, int DIM>
 HYDRO_DEV void load_q(const S* u, long long plane, long long cell, S q[DIM + 2]) {
 #pragma unroll
@@ -307,18 +177,6 @@ void launch_dt_euler_f32(const DtArgs& a, const float* u0, hipStream_t st) {
 
 // Solver::findMinTimestep tail (src/Solver/Solver.cpp:510-515): dt = min * cfl;
 // plus fixed dt, clipping to t_end, time/step bookkeeping and health flags.
-HYDRO_DEV void finalize_dt(const FinalizeArgs& f, double m, bool anynan) {
-  double dt = f.use_fixed ? f.fixed_dt : m * f.cfl;
-  const double t = *f.t;
-  if (f.clip && t + dt > f.t_end) dt = f.t_end - t;
-  unsigned h = *f.health;
-  if (anynan || !(dt > 0.0) || dt == INFINITY) h |= 1u;
-  *f.health = h;
-  *f.dt = dt;
-  *f.t = t + dt;
-  *f.steps = *f.steps + 1;
-}
-
 __global__ void __launch_bounds__(1024) finalize_dt_kernel(FinalizeArgs f) {
   __shared__ double sh[16];
   double m = INFINITY;
@@ -349,128 +207,12 @@ void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st) {
   hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, st, f);
 }
 
-// ---------------------------------------------------------------- CFL, 1D/2D
-// The CFL of the Euler Roe / HLL / HLLC solvers for 1D and 2D grids as a
-// strip march (the decomposition of the fused stage kernels): a wave owns 64
-// columns and a chunk of rows, computes each cell's frozen data once, the
-// x-face between lane-1 and lane through a DPP shift and the y-face against
-// the previous row held in registers; the next row's loads are issued before
-// the current row's arithmetic.  Per cell: the left face's maximal and the
-// right face's minimal wave speed give dx / (|max(0, lmax) - min(0, lmin)|
-// + 1e-9) for Roe (res/Roe.cl:4-65) and dx / (max(0, sR) - min(0, sL)) with
-// the never-written right face of cell S-2 read as 0 for HLL/HLLC
-// (res/EulerHLL.cl:142-175).  Each wave writes its minimum (NaN when a
-// speed was not finite) to its own slot; finalize_dt_kernel reduces them.
 template <typename S, int DIM, int SOLVER>
 __global__ void __launch_bounds__(256) dt_march_kernel(DtMarchArgs a, const S* u0) {
-  using Roe = EulerRoe<DIM, S>;
-  using HLL = HLLPolicy<DIM, S, SOLVER == 0 ? 1 : SOLVER>;
-  using Frozen = typename std::conditional<SOLVER == 0, typename Roe::Cell, typename HLL::Frozen>::type;
-  constexpr int N = DIM + 2;
-  constexpr int NF = SOLVER == 0 ? DIM + 2 : DIM + 5;
-
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (wave >= a.nwaves) return;
-  const int lane = threadIdx.x & 63;
-  const int strip = wave % a.nstrips;
-  const int chunk = wave / a.nstrips;
-  const int sx = a.sx, sy = a.sy;
-  const int c = strip * 60 + lane;
-  const uint32_t cl = c < sx ? c : sx - 1;
-  const bool out_col = lane >= 2 && lane < 62 && c <= a.last_x;
-  const bool right_ok_x = c + 1 <= sx - 2;
-  const S gamma = S(a.gamma);
-
-  auto fetch = [&](int row, S q[N]) {
-    const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;
-#pragma unroll
-    for (int s = 0; s < N; ++s) q[s] = ldp(u0, a.plane, s, e, cl);
-  };
-  auto frozen = [&](const S q[N]) {
-    if constexpr (SOLVER == 0) return Roe::template cell<S>(q, gamma);
-    else return HLL::template frozen<S>(q, gamma);
-  };
-  auto pack = [&](const Frozen& f, S o[NF]) {
-    if constexpr (SOLVER == 0) RoePolicy<DIM, S, 0>::pack(f, o);
-    else HLL::pack(f, o);
-  };
-  auto unpack = [&](const S o[NF]) {
-    if constexpr (SOLVER == 0) return RoePolicy<DIM, S, 0>::unpack(o);
-    else return HLL::unpack(o);
-  };
-  // (min, max) wave speed of the face between L and R, normal `side`
-  auto speeds = [&](const Frozen& L, const Frozen& R, int side, S& lmin, S& lmax) {
-    if constexpr (SOLVER == 0) {
-      const auto f = Roe::face(L, R, side, gamma);
-      lmin = f.v[0] - f.c;
-      lmax = f.v[0] + f.c;
-    } else {
-      HLL::speeds(L, R, side, gamma, lmin, lmax);
-    }
-  };
-  bool bad = false;
-  // dt of one cell along one side from its left face's max speed and its
-  // right face's min speed
-  auto side_dt = [&](S lmaxL, S lminR, bool right_ok, double kdx) {
-    bad |= !(lmaxL == lmaxL) || (right_ok && !(lminR == lminR));
-    if constexpr (SOLVER == 0) {
-      const double lmax = fmax(0.0, double(lmaxL));
-      const double lmin = fmin(0.0, double(lminR));
-      return kdx / (fabs(lmax - lmin) + 1e-9);
-    } else {
-      const double maxL = fmax(0.0, double(lmaxL));
-      const double minL = fmin(0.0, right_ok ? double(lminR) : 0.0);
-      return kdx / (maxL - minL);
-    }
-  };
-  auto x_dt = [&](const Frozen& f) {
-    S pk[NF], pkL[NF];
-    pack(f, pk);
-#pragma unroll
-    for (int i = 0; i < NF; ++i) pkL[i] = lane_from_left(pk[i]);
-    S lmin, lmax;
-    speeds(unpack(pkL), f, 0, lmin, lmax);  // face c: cells c-1, c
-    const S lminR = lane_from_right(lmin);  // face c+1
-    return side_dt(lmax, lminR, right_ok_x, a.kdx[0]);
-  };
-
-  double m = INFINITY;
-  if constexpr (DIM == 1) {
-    S q[N];
-    fetch(0, q);
-    const double d = x_dt(frozen(q));
-    if (out_col) m = d;
-  } else {
-    // a chunk is at most kDtRows rows: all its rows (plus one below and one
-    // above) are loaded up front, so the wave waits for memory once
-    const int y0 = 2 + chunk * a.rows;
-    int y1 = y0 + a.rows;
-    if (y1 > a.last_y + 1) y1 = a.last_y + 1;
-    S q[kDtRows + 2][N];
-#pragma unroll
-    for (int i = 0; i < kDtRows + 2; ++i) fetch(y0 - 1 + i <= y1 ? y0 - 1 + i : y1, q[i]);
-    Frozen cur = frozen(q[1]);
-    S lminY, lmaxY_prev;
-    speeds(frozen(q[0]), cur, 1, lminY, lmaxY_prev);  // face y0: bottom face of row y0
-    double xd = x_dt(cur);
-#pragma unroll
-    for (int i = 2; i < kDtRows + 2; ++i) {
-      const int r = y0 - 1 + i;  // face r: top face of row r-1
-      if (r > y1) break;
-      const Frozen prev = cur;
-      cur = frozen(q[i]);
-      S lmaxY;
-      speeds(prev, cur, 1, lminY, lmaxY);
-      const double yd = side_dt(lmaxY_prev, lminY, r <= sy - 2, a.kdx[1]);
-      if (out_col) m = fmin(m, fmin(xd, yd));
-      if (r < y1) xd = x_dt(cur);
-      lmaxY_prev = lmaxY;
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
-  const bool wave_bad = __any(bad && out_col);
-  if (lane == 0) a.wavemin[wave] = wave_bad ? NAN : m;
+  const double m = dt_march_wave<S, DIM, SOLVER>(a, u0, wave % a.nstrips, wave / a.nstrips);
+  if ((threadIdx.x & 63) == 0) a.wavemin[wave] = m;
 }
 
 template <typename S>

@@ -33,26 +33,16 @@
 
 namespace hydro {
 
+// One wavefront's share of a stage: columns strip*60-2 .. strip*60+61 of
+// rows 2 + chunk*rows .. (1D: the single row).  strip and chunk are
+// wave-uniform.
 template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>
-__global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
+HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
   constexpr int N = Pol::N;
   using Frozen = typename Pol::Frozen;
   using FaceA = typename Pol::FaceA;
   static_assert(sizeof(T) == sizeof(S), "stage state and increments share one layout");
-
-  // XCD-aware block order: consecutive logical blocks (neighbouring strips)
-  // land on the same XCD and share its L2.  The wave index is wave-uniform;
-  // readfirstlane lets the compiler keep the row arithmetic in scalars.
-  const int nb = gridDim.x;
-  const int per = nb >> 3;
-  const int b = blockIdx.x;
-  const int lb = (b & 7) * per + (b >> 3);
-  const int wave = __builtin_amdgcn_readfirstlane(lb * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
-  if (wave >= a.nwaves) return;
   const int lane = threadIdx.x & 63;
-  const int strip = wave % a.nstrips;
-  const int chunk = wave / a.nstrips;
-
   const int sx = a.sx, sy = a.sy;
   const long long pl = a.plane;
   const int c = strip * kStripOut + lane;       // column (x0 - 2 + lane)
@@ -140,14 +130,15 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
     // Window at the top of step(r): P = row r-1, FA1 = first pass of face
     // r-1, dq2 = deltaQTilde of face r-2, acc = partial derivative of row r-2
     // (its x-divergence plus the flux through its lower face), nx = the
-    // loads of row r (in flight).
+    // loads of row r (in flight), w1 / w2 = the loads of rows r-1 / r-2 (the
+    // combination of row r-2 reuses its u0 and d_k instead of reloading).
     Row P;
     FaceA FA1;
     T dq2[N], acc[N];
-    Raw nx;
+    Raw nx, w1, w2;
     {
       Row C;
-      Raw w0, w1;
+      Raw w0;
       fetch(y0 - 2, w0);
       fetch(y0 - 1, w1);
       build(w0, P);
@@ -156,11 +147,12 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
       const FaceA Fm = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 - 1), pp);
       fetch(y0 + 1, w1);
       build(w0, P);
+      w2 = w0;  // row y0
       FA1 = Pol::face_a(C.fz, P.fz, C.u, P.u, 1, yok(y0), pp);
 #pragma unroll
       for (int s = 0; s < N; ++s) dq2[s] = Fm.dq[s];
       fetch(y0 + 2, nx);
-      build(w1, C);
+      build(w1, C);  // row y0 + 1
       const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 + 1), pp);
       T G[N], dx[N];
       Pol::face_b(FA1, dq2, F0.dq, dt_dy, 1, a.lim, G);  // flux through face y0
@@ -178,10 +170,10 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
       const Raw cur = nx;
       // prefetch the next row (clamped: the last step reloads its own row)
       fetch(r + 1 <= y1 + 1 ? r + 1 : y1 + 1, nx);
-      // operands of this step's output row r-2
+      // operands of this step's output row r-2 (u0 and d_k from the window)
       const long long eo = (long long)__builtin_amdgcn_readfirstlane(r - 2) * sx;
       typename Comb::Fin fin;
-      comb.fetch_fin(eo, cl, fin);
+      comb.fetch_fin_rest(eo, cl, w2.q0, w2.dk, fin);
       Row C;
       build(cur, C);
       const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(r), pp);
@@ -203,6 +195,8 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
       for (int s = 0; s < N; ++s) dq2[s] = FA1.dq[s];
       FA1 = F0;
       P = C;
+      w2 = w1;
+      w1 = cur;
     };
 
     int r = y0 + 2;
@@ -216,6 +210,20 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
     for (; r <= y1; ++r) step(r, true);
     step(y1 + 1, false);
   }
+}
+
+template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>
+__global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
+  // XCD-aware block order: consecutive logical blocks (neighbouring strips)
+  // land on the same XCD and share its L2.  The wave index is wave-uniform;
+  // readfirstlane lets the compiler keep the row arithmetic in scalars.
+  const int nb = gridDim.x;
+  const int per = nb >> 3;
+  const int b = blockIdx.x;
+  const int lb = (b & 7) * per + (b >> 3);
+  const int wave = __builtin_amdgcn_readfirstlane(lb * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+  if (wave >= a.nwaves) return;
+  stage2d_wave<Pol, DIM, T, S, KAHAN, MODE>(a, wave % a.nstrips, wave / a.nstrips);
 }
 
 template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>

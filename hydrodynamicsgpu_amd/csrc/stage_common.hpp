@@ -115,6 +115,41 @@ struct StageCombine {
     T fv[kMaxPartials][N];
   };
 
+  // Operands of the combination of a cell whose u0 and d_k the caller
+  // already holds (the row march keeps them): only the rest is loaded.
+  HYDRO_DEV void fetch_fin_rest(long long e, uint32_t col, const S q0[N], const T dk[N], Fin& f) const {
+    constexpr bool kFinal = MODE == CM_FINAL || MODE == CM_SINGLE;
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      if constexpr (kReadsDk) f.dk[s] = dk[s];
+      f.hi[s] = q0[s];
+    }
+    if constexpr (MODE == CM_FINAL || MODE == CM_GENERIC) {
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j)
+        if (j < a.nrep) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) f.rep[j][s] = ldp(a.rep[j], pl, s, e, col);
+        }
+    }
+    if constexpr (MODE == CM_GENERIC) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) f.pk[s] = a.pkp ? ldp(a.pkp, pl, s, e, col) : T(0);
+#pragma unroll
+      for (int j = 0; j < kMaxPartials; ++j)
+        if (j < a.nf) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) f.fv[j][s] = a.fload[j] ? ldp((const T*)a.fp[j], pl, s, e, col) : T(0);
+        }
+    }
+    if constexpr (KAHAN) {
+      if (kFinal || (MODE == CM_GENERIC && a.final_stage)) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) f.lo[s] = ldp(a.u0lo, pl, s, e, col);
+      }
+    }
+  }
+
   HYDRO_DEV void fetch_fin(long long e, uint32_t col, Fin& f) const {
     constexpr bool kFinal = MODE == CM_FINAL || MODE == CM_SINGLE;
     if constexpr (kReadsDk) load_dk(e, col, f.dk);
@@ -187,9 +222,14 @@ struct StageCombine {
         }
       store_final(e, col, f, acc);
     } else {
+      // the fixed-function forms' order (d_k and L first), then the partial
+      // sum: a stage gives the same bits whichever form runs it
       T acc[N];
 #pragma unroll
-      for (int s = 0; s < N; ++s) acc[s] = f.pk[s] + aself * f.dk[s] + bdt * deriv[s];
+      for (int s = 0; s < N; ++s) {
+        acc[s] = aself * f.dk[s] + bdt * deriv[s];
+        if (a.pkp) acc[s] += f.pk[s];
+      }
 #pragma unroll
       for (int j = 0; j < kMaxPartials; ++j)
         if (j < a.nrep) {

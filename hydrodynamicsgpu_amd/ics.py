@@ -171,3 +171,45 @@ def primitives_euler(desc, aos):
     E = q[:, desc.dim + 1]
     p = (desc.gamma - 1.0) * (E - 0.5 * rho * (v * v).sum(axis=1))
     return rho, v, p
+
+
+def riemann_batch(count=512, seed=2024, min_1d=256, max_1d=65536, min_2d=64, max_2d=512,
+                  max_shock=8.0, min_rarefaction=0.05, **desc_kw):
+    """The uneven batch mix: `count` independent Euler grids, each a 1D line
+    with length log-uniform in [min_1d, max_1d] or a 2D square with side
+    log-uniform in [min_2d, max_2d] (a fair coin per member), holding a
+    Riemann problem split at the x-centre with rho, p log-uniform in
+    [0.1, 10] and |u| < 1 on each side.  Draws whose exact solution has a
+    star pressure above `max_shock` or below `min_rarefaction` times the
+    smaller side pressure are drawn again: the strongest collisions drive the
+    Roe/Superbee/RK2 scheme (in the reference as here) to negative pressures
+    within a few steps.  Everything comes from
+    the SplitMix64 counter hash of `seed`, so the mix is the same in every
+    language.  Returns a list of (desc, AoS state); `desc_kw` go to
+    config.make_desc (solver, integrator, limiter, precision, ...)."""
+    from . import config
+    from .riemann import star_state
+
+    gamma = desc_kw.get("defs", {}).get("idealGas_heatCapacityRatio", config.DEFAULT_DEFS["idealGas_heatCapacityRatio"])
+    lg = lambda x: float(np.exp(np.log(0.1) + x * (np.log(10.0) - np.log(0.1))))  # noqa: E731
+    out = []
+    for m in range(count):
+        for attempt in range(64):
+            r = uniform01(seed, np.arange(8, dtype=np.uint64) + np.uint64(8 * m), stream=attempt)
+            left = (lg(r[2]), 2.0 * r[3] - 1.0, lg(r[4]))
+            right = (lg(r[5]), 2.0 * r[6] - 1.0, lg(r[7]))
+            ps, _ = star_state(left, right, gamma)
+            pmin = min(left[2], right[2])
+            if min_rarefaction * pmin <= ps <= max_shock * pmin:
+                break
+        if r[0] < 0.5:
+            n = int(round(np.exp(np.log(min_1d) + r[1] * (np.log(max_1d) - np.log(min_1d)))))
+            size = [n]
+        else:
+            n = int(round(np.exp(np.log(min_2d) + r[1] * (np.log(max_2d) - np.log(min_2d)))))
+            size = [n, n]
+        kw = dict(desc_kw)
+        kw.setdefault("boundaryMethods", [{"min": "FREEFLOW", "max": "FREEFLOW"}] * 3)
+        d = config.make_desc(size, xmin=[-0.5] * len(size), xmax=[0.5] * len(size), **kw)
+        out.append((d, riemann(d, left, right)))
+    return out

@@ -2,7 +2,9 @@
 
 Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics", ch. 4:
 Newton iteration for p* (eqs. 4.5-4.9), then sampling (section 4.5).
-Used to pin the oracle and the HIP path against the mathematics (Sod tube).
+Used to pin the oracle and the HIP path against the mathematics (Sod tube),
+and by the batch IC generator to keep its random Riemann problems inside the
+regime the Roe scheme survives.
 """
 import numpy as np
 

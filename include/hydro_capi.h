@@ -260,8 +260,20 @@ int hydro_batch_get_state(hydro_batch* b, int member, double* aos, size_t n);
 int hydro_batch_run(hydro_batch* b, int nsteps, int use_graph);
 int hydro_batch_sync(hydro_batch* b);
 int hydro_batch_member_dt(hydro_batch* b, int member, double* dt_out);
+/* simulated time, completed steps and last dt of one member (synchronises) */
+int hydro_batch_member_time(hydro_batch* b, int member, double* t, long long* steps, double* last_dt);
+/* health flags of one member, as hydro_sim_health */
+int hydro_batch_member_health(hydro_batch* b, int member, unsigned* flags_out);
 /* total interior cells over all members */
 long long hydro_batch_interior_cells(const hydro_batch* b);
+/* Work decomposition: row-chunk height of the 2D members' tasks and strips
+ * (60 columns each) per task of the 1D members.  No counterpart in the
+ * reference (one program per grid there). */
+int hydro_batch_set_decomposition(hydro_batch* b, int rows, int strips);
+/* Autotuner: times candidate decompositions on the batch itself (the state,
+ * clocks and buffer parity are restored afterwards), keeps the fastest and
+ * reports it with its time per step. */
+int hydro_batch_autotune(hydro_batch* b, int* rows_out, int* strips_out, double* ms_per_step_out);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "HydroGPU/Equation/Equation.h"
 #include "HydroGPU/HydroGPUApp.h"
 #include "HydroGPU/Solver/Solver.h"
 #include "../../oracle/hydro_oracle.h"
@@ -81,11 +82,31 @@ static int run_cpu() {
   return failures ? 1 : 0;
 }
 
+// the state resetState() uploads: initState at every cell centre (ghosts
+// included), through the equation's readStateCell (before any device-side
+// conversion, which the oracle applies on its own for SRHD)
+static std::vector<real> initial_state(HydroGPU::HydroGPUApp& app, HydroGPU::Solver::Solver& s) {
+  const int n = s.numStates();
+  std::vector<real> st((size_t)s.getVolume() * n);
+  int flat = 0;
+  for (int k = 0; k < app.size[2]; ++k)
+    for (int j = 0; j < app.size[1]; ++j)
+      for (int i = 0; i < app.size[0]; ++i, ++flat) {
+        const int idx[3] = {i, j, k};
+        real pos[3];
+        for (int a = 0; a < 3; ++a)
+          pos[a] = real(app.xmax[a] - app.xmin[a]) * (real(idx[a]) + .5) / real(app.size[a]) + real(app.xmin[a]);
+        const std::vector<real> c = app.initState(pos[0], pos[1], pos[2]);
+        s.getEquation()->readStateCell(st.data() + (size_t)flat * n, c.data());
+      }
+  return st;
+}
+
 // one solver run through the host classes, the same descriptor through the oracle
 static void compare(HydroGPU::HydroGPUApp& app, int updates, real tol, const char* what) {
   auto solver = app.createSolver();
   oracle_sim* o = oracle_create(&solver->desc(), 0);
-  std::vector<real> q0 = solver->getState();
+  std::vector<real> q0 = initial_state(app, *solver);
   oracle_set_state(o, q0.data(), q0.size());
   real t = 0;
   for (int i = 0; i < updates; ++i) {

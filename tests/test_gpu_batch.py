@@ -1,0 +1,162 @@
+"""The batched launch (hydro_batch_*): many independent Euler grids of uneven
+sizes, 1D and 2D mixed, advanced together.
+
+Every member runs the single-grid arithmetic on its own slice with its own
+dt, so after any number of steps a member's state, time and step count must
+equal those of the same grid run alone through hydro_sim_* -- bit for bit
+(the single-grid path is itself checked against the oracle elsewhere).  The
+same must hold for every work decomposition, with and without the hipGraph
+replay, and the autotuner must leave the state untouched.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hydrodynamicsgpu_amd import config, ics
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [[64], [100], [300], [1021], [64, 64], [70, 90], [130, 64], [61, 125], [200, 40], [8, 9]]
+
+
+def _mix(seed=5, **kw):
+    out = []
+    rng = np.random.default_rng(seed)
+    for i, size in enumerate(SIZES):
+        d = config.make_desc(size, xmin=[-0.5] * len(size), xmax=[0.5] * len(size),
+                             boundaryMethods=[{"min": "FREEFLOW", "max": "MIRROR"}, {"min": "PERIODIC", "max": "PERIODIC"}],
+                             **kw)
+        left = (rng.uniform(0.2, 5.0), rng.uniform(-0.9, 0.9), rng.uniform(0.2, 5.0))
+        right = (rng.uniform(0.2, 5.0), rng.uniform(-0.9, 0.9), rng.uniform(0.2, 5.0))
+        out.append((d, ics.riemann(d, left, right)))
+    return out
+
+
+def _single(d, q, steps):
+    from hydrodynamicsgpu_amd import capi
+
+    s = capi.Sim(d)
+    s.set_state(q)
+    s.run(steps)
+    return s.get_state(), s.time()
+
+
+def _batch(mix):
+    from hydrodynamicsgpu_amd import capi
+
+    b = capi.Batch([d for d, _ in mix])
+    for m, (_, q) in enumerate(mix):
+        b.set_state(m, q)
+    return b
+
+
+CONFIGS = [
+    dict(integratorName="RungeKutta2", slopeLimiter="Superbee", precision="fp32_mixed"),
+    dict(integratorName="RungeKutta4", slopeLimiter="MonotizedCentral", precision="fp64"),
+    dict(integratorName="RungeKutta3TVD", slopeLimiter="MinMod", precision="fp32_mixed",
+         flags=config.FLAG_REFERENCE | config.FLAG_KAHAN),
+    dict(solverName="EulerHLLC", integratorName="RungeKutta2TVD", precision="fp64"),
+    dict(solverName="EulerHLL", integratorName="ForwardEuler", precision="fp32_mixed"),
+    dict(integratorName="RungeKutta4TVD", slopeLimiter="Koren", precision="fp64"),
+]
+
+
+@pytest.mark.parametrize("kw", CONFIGS)
+def test_members_equal_lone_grids(gpu_lib, kw):
+    mix = _mix(**kw)
+    b = _batch(mix)
+    b.run(5)
+    for m, (d, q) in enumerate(mix):
+        ref, (t, n, dt) = _single(d, q, 5)
+        np.testing.assert_array_equal(b.get_state(m), ref, err_msg="member %d %s" % (m, list(d.size)[:d.dim]))
+        assert b.time(m) == (t, n, dt)
+        assert b.health(m) == 0
+
+
+def test_member_matches_oracle(gpu_lib):
+    # one member against the oracle directly (FP64, RK2, Superbee)
+    mix = _mix(integratorName="RungeKutta2", slopeLimiter="Superbee")
+    b = _batch(mix)
+    b.run(4)
+    for m in (0, 5):
+        d, q = mix[m]
+        o = oracle.OracleSim(d)
+        o.set_state(q)
+        for _ in range(4):
+            o.update()
+        ref = o.get_state()
+        assert np.max(np.abs(b.get_state(m) - ref)) / np.max(np.abs(ref)) < 1e-10
+
+
+@pytest.mark.parametrize("rows,strips", [(4, 1), (8, 3), (64, 16), (1000, 2)])
+def test_decompositions_and_graphs_agree(gpu_lib, rows, strips):
+    kw = dict(integratorName="RungeKutta2", slopeLimiter="Superbee", precision="fp32_mixed")
+    mix = _mix(**kw)
+    ref = _batch(mix)
+    ref.run(6, graph=False)
+    b = _batch(mix)
+    b.set_decomposition(rows, strips)
+    b.run(3, graph=True)
+    b.run(1, graph=False)  # eager work between replays
+    b.run(2, graph=True)
+    for m in range(len(mix)):
+        np.testing.assert_array_equal(b.get_state(m), ref.get_state(m))
+        assert b.time(m) == ref.time(m)
+
+
+def test_autotune_leaves_state_and_clock(gpu_lib):
+    kw = dict(integratorName="RungeKutta2", slopeLimiter="Superbee", precision="fp32_mixed")
+    mix = _mix(**kw)
+    b = _batch(mix)
+    b.run(2)
+    before = [b.get_state(m) for m in range(len(mix))]
+    clocks = [b.time(m) for m in range(len(mix))]
+    rows, strips, ms = b.autotune()
+    assert rows >= 1 and strips >= 1 and ms > 0
+    for m in range(len(mix)):
+        np.testing.assert_array_equal(b.get_state(m), before[m])
+        assert b.time(m) == clocks[m]
+    b.run(3)
+    ref = _batch(mix)
+    ref.run(5)
+    for m in range(len(mix)):
+        np.testing.assert_array_equal(b.get_state(m), ref.get_state(m))
+
+
+def test_fixed_dt_batch(gpu_lib):
+    mix = _mix(integratorName="RungeKutta2", slopeLimiter="Superbee", useFixedDT=True, fixedDT=1e-4)
+    b = _batch(mix)
+    b.run(3)
+    for m, (d, q) in enumerate(mix):
+        ref, tm = _single(d, q, 3)
+        np.testing.assert_array_equal(b.get_state(m), ref)
+        assert b.time(m) == tm
+        assert b.health(m) == 0
+
+
+def test_batch_rejects_mixed_configurations(gpu_lib):
+    from hydrodynamicsgpu_amd import capi
+
+    d1 = config.make_desc([64], integratorName="RungeKutta2")
+    d2 = config.make_desc([64], integratorName="RungeKutta4")
+    with pytest.raises(capi.HydroError, match="share"):
+        capi.Batch([d1, d2])
+    d3 = config.make_desc([16, 16, 16])
+    with pytest.raises(capi.HydroError, match="1D or 2D"):
+        capi.Batch([d3])
+    d4 = config.make_desc([64], boundaryMethods=[{"min": "NONE", "max": "FREEFLOW"}])
+    with pytest.raises(capi.HydroError, match="boundary method"):
+        capi.Batch([d4])
+
+
+def test_uneven_512_mix_one_step_properties(gpu_lib):
+    # the benchmark mix itself: 512 members, 1D 256..65536 and 2D 64^2..512^2;
+    # two steps, then spot members against lone grids
+    mix = ics.riemann_batch(512, integratorName="RungeKutta2", slopeLimiter="Superbee", precision="fp32_mixed")
+    b = _batch(mix)
+    b.run(2)
+    for m in (0, 1, 2, 100, 255, 511):
+        d, q = mix[m]
+        ref, tm = _single(d, q, 2)
+        np.testing.assert_array_equal(b.get_state(m), ref)
+        assert b.time(m) == tm

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from exact_riemann import sample
+from hydrodynamicsgpu_amd.riemann import sample
 from hydrodynamicsgpu_amd import config, ics
 
 pytestmark = pytest.mark.gpu

@@ -122,3 +122,30 @@ def test_state_after_run_includes_carried_ghosts(gpu_lib):
     ghost[2:-2, 2:-2] = False
     assert np.max(np.abs(a[ghost] - b[ghost])) <= 1e-10 * np.max(np.abs(b))
     assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < 1e-10
+
+
+@pytest.mark.parametrize("integ,bm,precision", [("RungeKutta4", "PERIODIC", "fp32_mixed"),
+                                                 ("RungeKutta3TVD", "MIRROR", "fp64"),
+                                                 ("ForwardEuler", "FREEFLOW", "fp64")])
+def test_graph_replays_match_direct_launches(gpu_lib, monkeypatch, integ, bm, precision):
+    # run() replays a captured pair of updates; several replays, with eager
+    # work and synchronisation in between, must equal direct launches bit
+    # for bit
+    d = config.make_desc([90, 66], integratorName=integ, slopeLimiter="MonotizedCentral",
+                         boundaryMethods=[{"min": bm, "max": bm}] * 2, cfl=0.4, precision=precision)
+    q = _rand_state(d, 8)
+    monkeypatch.setenv("HYDRO_GRAPHS", "0")
+    direct = _mk(d)
+    monkeypatch.setenv("HYDRO_GRAPHS", "1")
+    graphed = _mk(d)
+    direct.set_state(q)
+    graphed.set_state(q)
+    for n in (4, 3, 2, 5):
+        direct.run(n)
+        graphed.run(n)
+        graphed.sync()
+        a, b = graphed.get_state(), direct.get_state()
+        np.testing.assert_array_equal(a, b)
+        assert graphed.time() == direct.time()
+        graphed.update()  # eager work between replays
+        direct.update()

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from exact_riemann import sample
+from hydrodynamicsgpu_amd.riemann import sample
 from hydrodynamicsgpu_amd import config, ics
 
 INTEG = {n: i for i, n in enumerate(config.INTEGRATORS)}
