@@ -44,6 +44,8 @@ def parse():
                     help="headline path with compensated FP32 state storage (1) or plain FP32 (0)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the Sod L1 accuracy leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 512-grid leg")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the HBM traffic measurement (rocprofv3 FETCH_SIZE / WRITE_SIZE child runs)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (Sod 1D, 3D HLLC blast, SRHD quadrants)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -210,6 +212,68 @@ def cpu_steps(d, q0, seconds, max_steps):
         if el >= seconds or n >= max_steps:
             break
     return n, el, threads
+
+
+def _pmc(counter, args, outdir):
+    """One rocprofv3 counter pass over a child run of this script; returns
+    {kernel name: [value per dispatch]} (KiB for FETCH_SIZE / WRITE_SIZE)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + args
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=120)
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if r.returncode != 0 or not files:
+        return None
+    out = {}
+    for row in csv.DictReader(open(files[0])):
+        if row["Counter_Name"] == counter:
+            out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return out
+
+
+def traffic_leg(kahan):
+    """HBM bytes per stage launch: FETCH_SIZE and WRITE_SIZE of the stage
+    kernels in child runs of the headline workload, each scaled by what the
+    same counter reads for a plain 256 MiB device copy (known bytes)."""
+    import tempfile
+
+    work = tempfile.mkdtemp(prefix="hydro_pmc_", dir="/tmp")
+    base = ["--steps", "3", "--warmup", "1", "--no-cpu", "--no-fp64", "--no-accuracy", "--no-batch", "--no-configs",
+            "--no-traffic", "--kahan", str(int(kahan))]
+    probe = ["--copy-probe", "256"]
+    f = _pmc("FETCH_SIZE", base, os.path.join(work, "f"))
+    w = _pmc("WRITE_SIZE", base, os.path.join(work, "w"))
+    cf = _pmc("FETCH_SIZE", probe, os.path.join(work, "cf"))
+    cw = _pmc("WRITE_SIZE", probe, os.path.join(work, "cw"))
+    if not (f and w and cf and cw):
+        return None
+    known = 256.0 * (1 << 20)
+
+    def probe_kib(d):
+        vals = [v for k, vs in d.items() if "copyBuffer" in k or "copy" in k.lower() for v in vs]
+        return max(vals) if vals else None
+
+    cfk, cwk = probe_kib(cf), probe_kib(cw)
+    if not cfk or not cwk:
+        return None
+    ff = known / (cfk * 1024.0)
+    wf = known / (cwk * 1024.0)
+    fs = [v for k, vs in f.items() if "stage2d_kernel" in k for v in vs]
+    ws = [v for k, vs in w.items() if "stage2d_kernel" in k for v in vs]
+    if not fs or not ws:
+        return None
+    fb = sum(fs) / len(fs) * 1024.0 * ff
+    wb = sum(ws) / len(ws) * 1024.0 * wf
+    return {"bytes_per_launch": fb + wb, "fetch_bytes": fb, "write_bytes": wb, "fetch_scale": ff, "write_scale": wf,
+            "launches": len(fs)}
 
 
 def config_legs(steps, warmup, with_cpu, cpu_seconds):
@@ -407,6 +471,15 @@ def main():
         out["fp32_" + ("plain" if a.kahan else "kahan")] = {
             "value": cells * a.steps / o["elapsed"], "ms_per_step": o["elapsed"] * 1e3 / a.steps,
             "stage_kernel_avg_ms": o["stage_ms"]}
+    if not a.no_traffic:
+        try:
+            t = traffic_leg(a.kahan)
+        except Exception as e:  # reported, never fatal for the GPU number
+            t = None
+            out["roofline"]["traffic_error"] = str(e)
+        if t:
+            out["roofline"]["traffic"] = t["bytes_per_launch"]
+            out["roofline"]["traffic_detail"] = t
     if not a.no_batch:
         out["batch"] = batch_leg(a.steps, a.warmup)
     if not a.no_configs:
