@@ -174,7 +174,7 @@ def primitives_euler(desc, aos):
 
 
 def riemann_batch(count=512, seed=2024, min_1d=256, max_1d=65536, min_2d=64, max_2d=512,
-                  max_shock=8.0, min_rarefaction=0.05, **desc_kw):
+                  max_shock=3.0, min_rarefaction=0.05, **desc_kw):
     """The uneven batch mix: `count` independent Euler grids, each a 1D line
     with length log-uniform in [min_1d, max_1d] or a 2D square with side
     log-uniform in [min_2d, max_2d] (a fair coin per member), holding a
@@ -183,7 +183,9 @@ def riemann_batch(count=512, seed=2024, min_1d=256, max_1d=65536, min_2d=64, max
     star pressure above `max_shock` or below `min_rarefaction` times the
     smaller side pressure are drawn again: the strongest collisions drive the
     Roe/Superbee/RK2 scheme (in the reference as here) to negative pressures
-    within a few steps.  Everything comes from
+    (with a bound of 8, one member of the default 512 fails at step 59 in
+    FP64 and at step 40 in FP32; with 3, every member stays positive for the
+    100 FP32 steps checked).  Everything comes from
     the SplitMix64 counter hash of `seed`, so the mix is the same in every
     language.  Returns a list of (desc, AoS state); `desc_kw` go to
     config.make_desc (solver, integrator, limiter, precision, ...)."""
