@@ -100,9 +100,18 @@ def time_gpu(precision, n, steps, warmup, q0, kahan=True):
     health = sim.health()
     state = sim.get_state()
     t, nsteps, dt = sim.time()
+    # host-inclusive rates: upload of the FP64 AoS host array, K (or 1)
+    # steps, readback, i.e. what a caller that keeps its state on the host pays
+    host = {}
+    for k in (steps, 1):
+        t2 = time.perf_counter()
+        sim.set_state(q0)
+        sim.run(k)
+        sim.get_state()
+        host[k] = time.perf_counter() - t2
     sim.close()
     return dict(elapsed=el_clean, elapsed_profiled=el, stage_ms=stage_ms, launches=launches, health=health,
-                state=state, t=t, nsteps=nsteps)
+                state=state, t=t, nsteps=nsteps, host=host)
 
 
 def sod_l1(precision, kahan, n=1024):
@@ -451,6 +460,11 @@ def main():
                      "kernel_avg_ms": mixed["stage_ms"], "launches_timed": mixed["launches"],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "health": mixed["health"],
+        "host_inclusive": {
+            "note": "not the headline: set_state from a host FP64 AoS array + steps + get_state, wall clock",
+            "value_k_steps": cells * a.steps / mixed["host"][a.steps],
+            "value_1_step": cells / mixed["host"][1],
+            "ms_transfer_overhead": (mixed["host"][a.steps] - mixed["elapsed"]) * 1e3},
     }
     if "fp64" in res:
         f = res["fp64"]
