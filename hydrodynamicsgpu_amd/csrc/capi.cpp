@@ -277,8 +277,14 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
     a.rows = s->rows;
     a.nchunks = (s->sz[1] - 4 + a.rows - 1) / a.rows;
     a.nwaves = a.nstrips * a.nchunks;
+  } else if (kernel_kind(s->d) == 1 || kernel_kind(s->d) == 2) {
+    // 3D first order: rows = y rows per chunk, workgroups of W-1 z-planes
+    a.rows = s->rows;
+    a.nchunks = (s->sz[1] - 4 + a.rows - 1) / a.rows;
+    const int nzb = (s->sz[2] - 4 + kStage3dR1Waves - 2) / (kStage3dR1Waves - 1);
+    a.nwaves = a.nstrips * a.nchunks * nzb;
   } else {
-    // 3D: rows = z-planes per workgroup, nchunks = y-chunks, nwaves = workgroups
+    // 3D Roe family: rows = z-planes per workgroup, nchunks = y-chunks, nwaves = workgroups
     const int ry = stage3d_rows_per_block(kernel_kind(s->d));
     a.rows = s->rows;
     a.nchunks = (s->sz[1] - 4 + ry - 1) / ry;
@@ -497,6 +503,10 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     int min_waves = 3;
     if (const char* e = getenv("HYDRO_MIN_WAVES_PER_SIMD")) min_waves = std::max(1, atoi(e));
     s->rows = pick_rows_2d((s->sz[0] - 4 + 59) / 60, s->sz[1] - 4, 4 * ncu, min_waves);
+  } else if (s->dim == 3 && (kernel_kind(*d) == 1 || kernel_kind(*d) == 2)) {
+    const long long strips = (s->sz[0] - 4 + 59) / 60;
+    const long long nzb = (s->sz[2] - 4 + kStage3dR1Waves - 2) / (kStage3dR1Waves - 1);
+    s->rows = pick_rows_3d_r1(strips * nzb, s->sz[1] - 4, ncu);
   } else if (s->dim == 3) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
     const int ry = stage3d_rows_per_block(kernel_kind(*d));

@@ -38,7 +38,9 @@ template <> HYDRO_DEV float tmin(float a, float b) { return fminf(a, b); }
 template <> HYDRO_DEV float tmax(float a, float b) { return fmaxf(a, b); }
 template <> HYDRO_DEV double tmin(double a, double b) { return fmin(a, b); }
 template <> HYDRO_DEV double tmax(double a, double b) { return fmax(a, b); }
-HYDRO_DEV float tsqrt(float a) { return __builtin_sqrtf(a); }
+// FP32: the hardware square root (v_sqrt_f32, ~1 ulp) without the
+// correctly-rounded refinement sequence; the FP32 path's tolerance covers it.
+HYDRO_DEV float tsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 HYDRO_DEV double tsqrt(double a) { return __builtin_sqrt(a); }
 HYDRO_DEV float tabs(float a) { return fabsf(a); }
 HYDRO_DEV double tabs(double a) { return fabs(a); }

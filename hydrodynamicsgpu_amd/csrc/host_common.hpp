@@ -201,6 +201,27 @@ inline int combine_mode(const StagePlan& p) {
 // its two halo rows on each side load and run the first face pass).  Chunks
 // stay small enough for `min_waves` waves per SIMD where the grid allows,
 // so that memory latency is hidden behind other waves.
+// Rows per y-chunk of the first-order 3D kernel: `columns` = strips x
+// z-blocks workgroups per chunk.  At least two workgroups per CU where the
+// grid allows (latency hiding), then the fewest rounds x (rows + 1 prologue
+// row).
+inline int pick_rows_3d_r1(long long columns, int ny, int ncu) {
+  int best = ny;
+  double best_cost = 1e300;
+  const bool enough = columns * ((ny + 7) / 8) >= 2LL * ncu;
+  for (int rows = 8; rows <= std::max(8, ny); ++rows) {
+    const long long wg = columns * ((ny + rows - 1) / rows);
+    if (enough && wg < 2LL * ncu) break;
+    const long long rounds = (wg + 2LL * ncu - 1) / (2LL * ncu);
+    const double cost = (double)rounds * (std::min(rows, ny) + 1.0);
+    if (cost < best_cost * (1.0 - 1e-9)) {
+      best_cost = cost;
+      best = rows;
+    }
+  }
+  return best;
+}
+
 inline int pick_rows_2d(long long strips, int ny, int simds, int min_waves) {
   int best = 32;
   double best_cost = 1e300;

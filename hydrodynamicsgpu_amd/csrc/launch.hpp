@@ -18,6 +18,9 @@ void launch_stage2d_f32k(int dim, int solver, int lim, int mode, const StageArgs
 
 // fused RK stage, 3D (stage3d.hip): workgroups of kStage3dWaves rows
 constexpr int kStage3dWaves = 8;
+// first-order 3D kernel: waves (z-planes) per workgroup, one of them supplies
+// only the top z-face
+constexpr int kStage3dR1Waves = 8;
 int stage3d_rows_per_block(int solver);
 void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, double>& a, hipStream_t st);
 void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,

@@ -9,12 +9,14 @@
 // (ROE_EIGENFIELD_TRANSFORM_SEPARATE, src/Equation/Maxwell.cpp:66-70).
 #pragma once
 #include "device_common.hpp"
+#include "policies.hpp"
 
 namespace hydro {
 
 template <int DIM, typename T, int LIM>
-struct MaxwellRoePolicy {
+struct MaxwellRoePolicy : StageIsState<6, T> {
   static constexpr int N = 6;
+  using Stage = typename StageIsState<N, T>::Stage;
   static constexpr bool kNeighbors = true;
   static constexpr bool kFrozenPrims = false;
   struct Frozen {
@@ -71,12 +73,12 @@ struct MaxwellRoePolicy {
   }
 
   // res/Roe.cl:126-138 and 254-263 (separate transforms)
-  static HYDRO_DEV FaceA face_a(const Frozen&, const Frozen&, const T uL[N], const T uR[N], int side, bool valid,
+  static HYDRO_DEV FaceA face_a(const Frozen&, const Frozen&, const Stage& sL, const Stage& sR, int side, bool valid,
                                 const PhysParams<T>& pp) {
     FaceA a;
     T tl[N], tr[N];
-    left(uL, side, pp.mx[2], pp.mx[3], tl);
-    left(uR, side, pp.mx[2], pp.mx[3], tr);
+    left(sL.u, side, pp.mx[2], pp.mx[3], tl);
+    left(sR.u, side, pp.mx[2], pp.mx[3], tr);
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       a.dq[k] = valid ? tr[k] - tl[k] : T(0);

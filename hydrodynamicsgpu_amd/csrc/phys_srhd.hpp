@@ -8,12 +8,14 @@
 // back by -90 degrees (side 1: (x,y) -> (y,-x); side 2: (x,z) -> (z,-x)).
 #pragma once
 #include "device_common.hpp"
+#include "policies.hpp"
 
 namespace hydro {
 
 template <int DIM, typename T, int LIM>
-struct SRHDRoePolicy {
+struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
   static constexpr int N = DIM + 2;
+  using Stage = typename StageIsState<N, T>::Stage;
   static constexpr int E = DIM + 1;
   static constexpr bool kNeighbors = true;
   static constexpr bool kFrozenPrims = true;
@@ -193,8 +195,10 @@ struct SRHDRoePolicy {
     Lm[E][E] = scale * (-vx - tq * (vx - AMinus * lmin) + Kappa * AMinus * lmin);
   }
 
-  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const T uL[N], const T uR[N], int side,
+  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const Stage& sL, const Stage& sR, int side,
                                 bool valid, const PhysParams<T>& pp) {
+    const T* uL = sL.u;
+    const T* uR = sR.u;
     const T gamma = pp.gamma;
     FaceA a;
     const Mean m = mean(cL, cR, side, gamma);

@@ -14,11 +14,34 @@
 
 namespace hydro {
 
+// Per-cell stage data a policy's face_a reads, built once per cell by
+// stage() and moved between lanes as NS values (pack_stage/unpack_stage).
+// The Roe family reads the stage state itself.
+template <int N, typename T>
+struct StageIsState {
+  struct Stage {
+    T u[N];
+  };
+  static constexpr int NS = N;
+  static HYDRO_DEV Stage stage(const T u[N], T) {
+    Stage c;
+#pragma unroll
+    for (int s = 0; s < N; ++s) c.u[s] = u[s];
+    return c;
+  }
+  static HYDRO_DEV void pack_stage(const Stage& c, T o[N]) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) o[s] = c.u[s];
+  }
+  static HYDRO_DEV Stage unpack_stage(const T o[N]) { return stage(o, T(0)); }
+};
+
 // ------------------------------------------------------------------ Roe
 template <int DIM, typename T, int LIM>
-struct RoePolicy {
+struct RoePolicy : StageIsState<DIM + 2, T> {
   using P = EulerRoe<DIM, T>;
   static constexpr int N = P::N;
+  using Stage = typename StageIsState<N, T>::Stage;
   static constexpr bool kNeighbors = true;
   static constexpr bool kFrozenPrims = false;
   using Frozen = typename P::Cell;
@@ -50,7 +73,7 @@ struct RoePolicy {
   }
 
   // res/Roe.cl:77-150 (deltaQTilde) and the L*qavg half of res/Roe.cl:264-270
-  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const T uL[N], const T uR[N], int side,
+  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const Stage& sL, const Stage& sR, int side,
                                 bool valid, const PhysParams<T>& pp) {
     const T gamma = pp.gamma;
     FaceA a;
@@ -58,8 +81,8 @@ struct RoePolicy {
     T dlt[N], avg[N];
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-      dlt[s] = uR[s] - uL[s];
-      avg[s] = T(.5) * (uR[s] + uL[s]);
+      dlt[s] = sR.u[s] - sL.u[s];
+      avg[s] = T(.5) * (sR.u[s] + sL.u[s]);
     }
     P::rotate(dlt, side);
     P::rotate(avg, side);
@@ -110,6 +133,50 @@ struct HLLPolicy {
     T F[N];
     T dq[N];  // unused (no limiter); kept so the kernels stay generic
   };
+  // stage state with its velocity, pressure and total enthalpy, computed once
+  // per cell (each cell takes part in 2*DIM faces)
+  struct Stage {
+    T q[N];
+    T v[DIM];
+    T p, h;
+  };
+  static constexpr int NS = N + DIM + 2;
+
+  static HYDRO_DEV Stage stage(const T u[N], T gamma) {
+    Stage c;
+    const T d = u[0];
+    const T inv = trcp(d);
+    T vsq = T(0);
+#pragma unroll
+    for (int s = 0; s < N; ++s) c.q[s] = u[s];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      c.v[k] = tdiv(u[1 + k], d);
+      vsq += c.v[k] * c.v[k];
+    }
+    const T eT = u[E] * inv;
+    c.p = (gamma - T(1)) * d * (eT - T(.5) * vsq);
+    c.h = eT + c.p * inv;
+    return c;
+  }
+  static HYDRO_DEV void pack_stage(const Stage& c, T o[NS]) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) o[s] = c.q[s];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) o[N + k] = c.v[k];
+    o[N + DIM] = c.p;
+    o[N + DIM + 1] = c.h;
+  }
+  static HYDRO_DEV Stage unpack_stage(const T o[NS]) {
+    Stage c;
+#pragma unroll
+    for (int s = 0; s < N; ++s) c.q[s] = o[s];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) c.v[k] = o[N + k];
+    c.p = o[N + DIM];
+    c.h = o[N + DIM + 1];
+    return c;
+  }
 
   template <typename S>
   static HYDRO_DEV Frozen frozen(const S q0[N], T gamma) {
@@ -170,37 +237,35 @@ struct HLLPolicy {
   }
 
   // res/EulerHLL.cl:185-343, res/EulerHLLC.cl:178-419 (HLLC_METHOD 2)
-  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const T uL_[N], const T uR_[N], int side,
+  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const Stage& sL, const Stage& sR, int side,
                                 bool valid, const PhysParams<T>& pp) {
     const T gamma = pp.gamma;
     (void)valid;
     FaceA a;
     T sl, sr;
     speeds(cL, cR, side, gamma, sl, sr);
-    T qL[N], qR[N];
+    // both cells in the frame whose component 0 is the face normal
+    T qL[N], qR[N], vL[DIM], vR[DIM];
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-      qL[s] = uL_[s];
-      qR[s] = uR_[s];
+      qL[s] = sL.q[s];
+      qR[s] = sR.q[s];
       a.dq[s] = T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      vL[k] = sL.v[k];
+      vR[k] = sR.v[k];
     }
     EulerRoe<DIM, T>::rotate(qL, side);
     EulerRoe<DIM, T>::rotate(qR, side);
-    const T g1 = gamma - T(1);
-    const T dL = qL[0], dR = qR[0];
-    const T iL = trcp(dL), iR = trcp(dR);
-    T vL[DIM], vR[DIM], vsqL = T(0), vsqR = T(0);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      vL[d] = tdiv(qL[1 + d], dL);
-      vR[d] = tdiv(qR[1 + d], dR);
-      vsqL += vL[d] * vL[d];
-      vsqR += vR[d] * vR[d];
+    if (side != 0) {
+      T t = vL[0]; vL[0] = vL[side]; vL[side] = t;
+      t = vR[0]; vR[0] = vR[side]; vR[side] = t;
     }
-    const T eTL = qL[E] * iL, eTR = qR[E] * iR;
-    const T pL = g1 * dL * (eTL - T(.5) * vsqL);
-    const T pR = g1 * dR * (eTR - T(.5) * vsqR);
-    const T hL = eTL + pL * iL, hR = eTR + pR * iR;
+    const T dL = qL[0], dR = qR[0];
+    const T pL = sL.p, pR = sR.p;
+    const T hL = sL.h, hR = sR.h;
     T fL[N], fR[N];
     fL[0] = dL * vL[0];
     fR[0] = dR * vR[0];

@@ -40,6 +40,7 @@ template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>
 HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
   constexpr int N = Pol::N;
   using Frozen = typename Pol::Frozen;
+  using Stage = typename Pol::Stage;
   using FaceA = typename Pol::FaceA;
   static_assert(sizeof(T) == sizeof(S), "stage state and increments share one layout");
   const int lane = threadIdx.x & 63;
@@ -59,7 +60,7 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
 
   using Comb = StageCombine<T, S, KAHAN, N, MODE>;
   struct Row {
-    T u[N];     // stage state
+    Stage st;   // stage state (and what the policy derives from it per cell)
     Frozen fz;  // frozen start-of-step data
   };
   // global loads of one row, issued one march step ahead of their use
@@ -79,7 +80,9 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
     }
   };
   auto build = [&](const Raw& w, Row& R) {
-    comb.stage_state(w.q0, w.dk, R.u);
+    T u[N];
+    comb.stage_state(w.q0, w.dk, u);
+    R.st = Pol::stage(u, gamma);
     if constexpr (Pol::kFrozenPrims) R.fz = Pol::template frozen<double>(w.pr, gamma);
     else R.fz = Pol::template frozen<S>(w.q0, gamma);
   };
@@ -91,14 +94,15 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
 
   // x-direction flux divergence of one row: -(F(c+1) - F(c)) / dx
   auto xdiv = [&](const Row& R, T div[N]) {
-    T uL[N], pk[Pol::NF], pkL[Pol::NF];
+    T ps[Pol::NS], pk[Pol::NF], pkL[Pol::NF];
+    Pol::pack_stage(R.st, ps);
 #pragma unroll
-    for (int s = 0; s < N; ++s) uL[s] = lane_from_left(R.u[s]);
+    for (int i = 0; i < Pol::NS; ++i) ps[i] = lane_from_left(ps[i]);
     Pol::pack(R.fz, pk);
 #pragma unroll
     for (int i = 0; i < Pol::NF; ++i) pkL[i] = lane_from_left(pk[i]);
     const Frozen fL = Pol::unpack(pkL);
-    const FaceA fa = Pol::face_a(fL, R.fz, uL, R.u, 0, col_face, pp);
+    const FaceA fa = Pol::face_a(fL, R.fz, Pol::unpack_stage(ps), R.st, 0, col_face, pp);
     T dqlo[N], dqhi[N], F[N];
     if constexpr (Pol::kNeighbors) {
 #pragma unroll
@@ -144,16 +148,16 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
       build(w0, P);
       fetch(y0, w0);
       build(w1, C);
-      const FaceA Fm = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 - 1), pp);
+      const FaceA Fm = Pol::face_a(P.fz, C.fz, P.st, C.st, 1, yok(y0 - 1), pp);
       fetch(y0 + 1, w1);
       build(w0, P);
       w2 = w0;  // row y0
-      FA1 = Pol::face_a(C.fz, P.fz, C.u, P.u, 1, yok(y0), pp);
+      FA1 = Pol::face_a(C.fz, P.fz, C.st, P.st, 1, yok(y0), pp);
 #pragma unroll
       for (int s = 0; s < N; ++s) dq2[s] = Fm.dq[s];
       fetch(y0 + 2, nx);
       build(w1, C);  // row y0 + 1
-      const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(y0 + 1), pp);
+      const FaceA F0 = Pol::face_a(P.fz, C.fz, P.st, C.st, 1, yok(y0 + 1), pp);
       T G[N], dx[N];
       Pol::face_b(FA1, dq2, F0.dq, dt_dy, 1, a.lim, G);  // flux through face y0
       xdiv(P, dx);
@@ -176,7 +180,7 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
       comb.fetch_fin_rest(eo, cl, w2.q0, w2.dk, fin);
       Row C;
       build(cur, C);
-      const FaceA F0 = Pol::face_a(P.fz, C.fz, P.u, C.u, 1, yok(r), pp);
+      const FaceA F0 = Pol::face_a(P.fz, C.fz, P.st, C.st, 1, yok(r), pp);
       T G[N];
       Pol::face_b(FA1, dq2, F0.dq, dt_dy, 1, a.lim, G);  // flux through face r-1
       if (out_lane) {

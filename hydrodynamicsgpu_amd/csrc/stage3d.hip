@@ -1,12 +1,13 @@
 // stage3d.hip -- fused RK stage for 3D grids on gfx950.
 //
-// A workgroup of W wavefronts covers a 64-column strip (x, lanes; DPP
-// shifts) by W consecutive rows (y, one row per wave; LDS exchange) and
-// marches a chunk of z-planes with the z stencil held in registers.  The
-// first and last H rows of the workgroup are halo rows (H = 2 for Roe, whose
-// limiter needs the neighbouring interfaces, 1 for HLL/HLLC); they compute
-// only what their neighbours need.  Kernel arithmetic and reference
-// semantics are those of stage2d.hip.
+// Roe family (limited, stencil radius 2): a workgroup of W wavefronts covers
+// a 64-column strip (x, lanes; DPP shifts) by W consecutive rows (y, one row
+// per wave; LDS exchange) and marches a chunk of z-planes with the z stencil
+// held in registers.  The first and last 2 rows of the workgroup are halo
+// rows that compute only what their neighbours need.
+// HLL/HLLC (first order, radius 1): stage3d_r1_kernel below.
+// Kernel arithmetic and reference semantics are those of the 2D kernel
+// (stage2d_impl.hpp).
 #include "device_common.hpp"
 #include "launch.hpp"
 #include "policies.hpp"
@@ -20,9 +21,11 @@ template <class Pol, typename T, typename S, bool KAHAN, int W>
 __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   constexpr int N = Pol::N;
   constexpr int NF = Pol::NF;
-  constexpr int NA = N + NF;
+  constexpr int NS = Pol::NS;
+  constexpr int NA = NS + NF;
   constexpr int H = Pol::kNeighbors ? 2 : 1;
   using Frozen = typename Pol::Frozen;
+  using Stage = typename Pol::Stage;
   using FaceA = typename Pol::FaceA;
 
   __shared__ T shA[2][W][NA][64];  // stage state + frozen data per row, double-buffered by plane
@@ -60,13 +63,14 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   const StageCombine<T, S, KAHAN, N, CM_GENERIC> comb(a, dtd);
   const long long psz = (long long)sx * sy;
 
-  auto load = [&](int z, T u[N], Frozen& fz) {
+  auto load = [&](int z, Stage& st, Frozen& fz) {
     const long long e = (long long)z * psz + (long long)yl * sx;  // wave-uniform
     S q0[N];
-    T dk[N];
+    T dk[N], u[N];
     comb.load_u0(e, cl, q0);
     comb.load_dk(e, cl, dk);
     comb.stage_state(q0, dk, u);
+    st = Pol::stage(u, gamma);
     if constexpr (Pol::kFrozenPrims) {
       double pr[N];
 #pragma unroll
@@ -81,7 +85,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   int z1 = z0 + a.rows;
   if (z1 > sz - 2) z1 = sz - 2;
 
-  T u_prev[N];
+  Stage u_prev;
   Frozen f_prev;
   FaceA fa_m1, fa_m2;
   T Fz_prev[N], div_m1[N], div_m2[N];
@@ -94,7 +98,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
 
 #pragma unroll 1
   for (int zn = z0 - 1; zn <= z1 + 1; ++zn) {
-    T u[N];
+    Stage u;
     Frozen fz;
     load(zn, u, fz);
     const bool out_plane = zn >= z0 && zn < z1;
@@ -105,13 +109,14 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
 #pragma unroll
     for (int s = 0; s < N; ++s) div[s] = T(0);
     if (out_plane && out_row) {
-      T uL[N], pk[NF], pkL[NF];
+      T uL[NS], pk[NF], pkL[NF];
+      Pol::pack_stage(u, uL);
 #pragma unroll
-      for (int s = 0; s < N; ++s) uL[s] = lane_from_left(u[s]);
+      for (int s = 0; s < NS; ++s) uL[s] = lane_from_left(uL[s]);
       Pol::pack(fz, pk);
 #pragma unroll
       for (int i = 0; i < NF; ++i) pkL[i] = lane_from_left(pk[i]);
-      const FaceA fa = Pol::face_a(Pol::unpack(pkL), fz, uL, u, 0, col_face, pp);
+      const FaceA fa = Pol::face_a(Pol::unpack(pkL), fz, Pol::unpack_stage(uL), u, 0, col_face, pp);
       T dqlo[N], dqhi[N], F[N];
       if constexpr (Pol::kNeighbors) {
 #pragma unroll
@@ -127,24 +132,25 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
 
     // y-direction through LDS (whole workgroup, uniform barriers)
     {
-      T pk[NF];
+      T pk[NF], ps[NS];
       Pol::pack(fz, pk);
+      Pol::pack_stage(u, ps);
 #pragma unroll
-      for (int s = 0; s < N; ++s) shA[buf][w][s][lane] = u[s];
+      for (int s = 0; s < NS; ++s) shA[buf][w][s][lane] = ps[s];
 #pragma unroll
-      for (int i = 0; i < NF; ++i) shA[buf][w][N + i][lane] = pk[i];
+      for (int i = 0; i < NF; ++i) shA[buf][w][NS + i][lane] = pk[i];
     }
     __syncthreads();
     FaceA fy;
     {
       const int ws = w > 0 ? w - 1 : 0;
-      T uS[N], pkS[NF];
+      T uS[NS], pkS[NF];
 #pragma unroll
-      for (int s = 0; s < N; ++s) uS[s] = shA[buf][ws][s][lane];
+      for (int s = 0; s < NS; ++s) uS[s] = shA[buf][ws][s][lane];
 #pragma unroll
-      for (int i = 0; i < NF; ++i) pkS[i] = shA[buf][ws][N + i][lane];
+      for (int i = 0; i < NF; ++i) pkS[i] = shA[buf][ws][NS + i][lane];
       const bool ok = w > 0 && col_face && row_face && zn >= 2 && zn <= sz - 2;
-      fy = Pol::face_a(Pol::unpack(pkS), fz, uS, u, 1, ok, pp);
+      fy = Pol::face_a(Pol::unpack(pkS), fz, Pol::unpack_stage(uS), u, 1, ok, pp);
     }
     T Fy[N];
     if constexpr (Pol::kNeighbors) {
@@ -194,11 +200,196 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
       Fz_prev[s] = Fz[s];
       div_m2[s] = div_m1[s];
       div_m1[s] = div[s];
-      u_prev[s] = u[s];
     }
+    u_prev = u;
     fa_m2 = fa_m1;
     fa_m1 = fa;
     f_prev = fz;
+  }
+}
+
+// First-order face solvers (HLL, HLLC: stencil radius 1).  A workgroup of
+// W waves covers one 64-column strip (x: lanes, DPP), W consecutive z-planes
+// (one per wave) and a chunk of rows that every wave marches in y with the
+// y stencil in registers, as the 2D kernel does.  Each wave computes the
+// flux through the lower z-face of its own plane (loading the plane below,
+// which the wave beneath loads too: an L2 hit) and hands it to the wave
+// below through LDS, so every z-face is computed once; the top wave of the
+// workgroup only supplies that face (W-1 output planes per workgroup).  One
+// barrier per row.
+template <class Pol, typename T, typename S, bool KAHAN, int MODE, int W>
+__global__ void __launch_bounds__(64 * W) stage3d_r1_kernel(StageArgs<T, S> a) {
+  constexpr int N = Pol::N;
+  using Frozen = typename Pol::Frozen;
+  using Comb = StageCombine<T, S, KAHAN, N, MODE>;
+  __shared__ T shz[2][W][N][64];  // lower z-face fluxes per wave, double-buffered by row
+
+  const int nb = gridDim.x;
+  const int per = nb >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (lb >= a.nwaves) return;  // nwaves counts workgroups here (uniform per block)
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int sx = a.sx, sy = a.sy, sz = a.sz;
+  const int nzb = (sz - 4 + W - 2) / (W - 1);
+  const int zblk = lb % nzb;  // z fastest: neighbouring blocks share their boundary plane in L2
+  const int rest = lb / nzb;
+  const int strip = rest % a.nstrips;
+  const int ych = rest / a.nstrips;
+  const int p = 2 + zblk * (W - 1) + w;    // this wave's plane
+  const bool ow = w < W - 1 && p <= sz - 3;  // output plane (wave-uniform)
+  const bool zw = p <= sz - 2;               // its lower face is read by an output wave
+  const long long psz = (long long)sx * sy;
+  const int c = strip * kStripOut + lane;
+  const uint32_t cl = c < sx ? c : sx - 1;
+  const bool out_lane = lane >= 2 && lane < 62 && c <= sx - 3;
+
+  const PhysParams<T> pp = make_params<T>(a);
+  const T gamma = pp.gamma;
+  const double dtd = *a.dt;
+  const T idx = T(a.inv_dx[0]), idy = T(a.inv_dx[1]), idz = T(a.inv_dx[2]);
+  const Comb comb(a, dtd);
+
+  struct Raw {
+    S q0[N];
+    T dk[N];
+  };
+  using Stage = typename Pol::Stage;
+  struct Row {
+    Stage st;
+    Frozen fz;
+  };
+  auto fetch = [&](int plane, int row, Raw& r) {
+    const long long e = (long long)__builtin_amdgcn_readfirstlane(plane) * psz +
+                        (long long)__builtin_amdgcn_readfirstlane(row) * sx;
+    comb.load_u0(e, cl, r.q0);
+    comb.load_dk(e, cl, r.dk);
+  };
+  auto build = [&](const Raw& r, Row& R) {
+    T u[N];
+    comb.stage_state(r.q0, r.dk, u);
+    R.st = Pol::stage(u, gamma);
+    R.fz = Pol::template frozen<S>(r.q0, gamma);
+  };
+  auto flux = [&](const Row& L, const Row& R, int side, T F[N]) {
+    const auto fa = Pol::face_a(L.fz, R.fz, L.st, R.st, side, true, pp);
+    Pol::face_b(fa, nullptr, nullptr, T(0), side, 0, F);
+  };
+  auto xdiv = [&](const Row& R, T div[N]) {
+    T ps[Pol::NS], pk[Pol::NF], pkL[Pol::NF];
+    Pol::pack_stage(R.st, ps);
+#pragma unroll
+    for (int i = 0; i < Pol::NS; ++i) ps[i] = lane_from_left(ps[i]);
+    Pol::pack(R.fz, pk);
+#pragma unroll
+    for (int i = 0; i < Pol::NF; ++i) pkL[i] = lane_from_left(pk[i]);
+    Row L;
+    L.st = Pol::unpack_stage(ps);
+    L.fz = Pol::unpack(pkL);
+    T F[N];
+    flux(L, R, 0, F);
+#pragma unroll
+    for (int s = 0; s < N; ++s) div[s] = (F[s] - lane_from_right(F[s])) * idx;
+  };
+
+  const int pown = p < sz ? p : sz - 1;  // clamped load planes (idle waves past the grid)
+  const int plow = p - 1 < sz ? p - 1 : sz - 1;
+  const int y0 = 2 + ych * a.rows;
+  int y1 = y0 + a.rows;
+  if (y1 > sy - 2) y1 = sy - 2;
+
+  // window: P = own row r-1 and its loads w1, Gp = flux through face r-1
+  // (lower face of row r-1), acc = x and z divergence of row r-1
+  Row P;
+  Raw w1, nx, nz;
+  T Gp[N], acc[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) Gp[s] = acc[s] = T(0);
+  if (ow) {
+    fetch(pown, y0 - 1, w1);
+    build(w1, P);
+  }
+  if (zw) {
+    fetch(pown, y0, nx);
+    fetch(plow, y0, nz);
+  }
+  // one row step; zstep: row r is an output row (its z-face and x-face run,
+  // with the barrier), else only face r closes row r-1.  buf alternates.
+  auto step = [&](int r, bool zstep, int buf) {
+    const Raw cur = nx, curz = nz;
+    if (zw && zstep) {  // prefetch row r+1 (<= y1 <= sy-2)
+      fetch(pown, r + 1, nx);
+      fetch(plow, r + 1, nz);
+    }
+    const long long eo = (long long)__builtin_amdgcn_readfirstlane(pown) * psz +
+                         (long long)__builtin_amdgcn_readfirstlane(r - 1) * sx;
+    typename Comb::Fin fin;
+    if (ow && r > y0) comb.fetch_fin_rest(eo, cl, w1.q0, w1.dk, fin);
+    Row C;
+    if (zw) build(cur, C);
+    T G[N], Fz[N], xd[N];
+    if (ow) flux(P, C, 1, G);  // face r: rows r-1 | r
+    if (zstep) {
+      if (zw) {
+        Row Z;
+        build(curz, Z);
+        flux(Z, C, 2, Fz);  // lower z-face of row r
+#pragma unroll
+        for (int s = 0; s < N; ++s) shz[buf][w][s][lane] = Fz[s];
+      }
+      if (ow) xdiv(C, xd);
+      __syncthreads();
+      if (ow) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) xd[s] += (Fz[s] - shz[buf][w + 1][s][lane]) * idz;
+      }
+    }
+    if (ow) {
+      if (r > y0 && out_lane) {
+        T deriv[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) deriv[s] = acc[s] + (Gp[s] - G[s]) * idy;
+        comb.finish(eo, c, fin, deriv);
+      }
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        Gp[s] = G[s];
+        acc[s] = xd[s];
+      }
+      P = C;
+      w1 = cur;
+    }
+  };
+  // unrolled by two so the window rotates by renaming and the LDS buffer
+  // index is static
+  int r = y0;
+#pragma unroll 1
+  for (; r + 1 < y1; r += 2) {
+    step(r, true, 0);
+    step(r + 1, true, 1);
+  }
+  if (r < y1) {
+    step(r, true, 0);
+    ++r;
+  }
+  step(y1, false, 0);
+}
+
+template <class Pol, typename T, typename S, bool KAHAN, int MODE>
+static void launch3_r1_mode(const StageArgs<T, S>& a, hipStream_t st) {
+  const int blocks = (a.nwaves + 7) & ~7;  // multiple of 8 for the XCD remap
+  hipLaunchKernelGGL((stage3d_r1_kernel<Pol, T, S, KAHAN, MODE, kStage3dR1Waves>), dim3(blocks),
+                     dim3(64 * kStage3dR1Waves), 0, st, a);
+}
+
+template <class Pol, typename T, typename S, bool KAHAN>
+static void launch3_r1(int mode, const StageArgs<T, S>& a, hipStream_t st) {
+  switch (mode) {
+  case CM_FIRST: launch3_r1_mode<Pol, T, S, KAHAN, CM_FIRST>(a, st); return;
+  case CM_MID: launch3_r1_mode<Pol, T, S, KAHAN, CM_MID>(a, st); return;
+  case CM_FINAL: launch3_r1_mode<Pol, T, S, KAHAN, CM_FINAL>(a, st); return;
+  case CM_SINGLE: launch3_r1_mode<Pol, T, S, KAHAN, CM_SINGLE>(a, st); return;
+  default: launch3_r1_mode<Pol, T, S, KAHAN, CM_GENERIC>(a, st); return;
   }
 }
 
@@ -209,11 +400,11 @@ static void launch3(const StageArgs<T, S>& a, hipStream_t st) {
 }
 
 template <typename T, typename S, bool KAHAN, int W>
-static void launch3_solver(int kind, int lim, const StageArgs<T, S>& a, hipStream_t st) {
+static void launch3_solver(int kind, int lim, int mode, const StageArgs<T, S>& a, hipStream_t st) {
   if (kind == 1) {
-    launch3<HLLPolicy<3, T, 1>, T, S, KAHAN, W>(a, st);
+    launch3_r1<HLLPolicy<3, T, 1>, T, S, KAHAN>(mode, a, st);
   } else if (kind == 2) {
-    launch3<HLLPolicy<3, T, 2>, T, S, KAHAN, W>(a, st);
+    launch3_r1<HLLPolicy<3, T, 2>, T, S, KAHAN>(mode, a, st);
   } else if (kind == 3) {
     launch3<SRHDRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(a, st);
   } else if (kind == 4) {
@@ -228,19 +419,20 @@ static void launch3_solver(int kind, int lim, const StageArgs<T, S>& a, hipStrea
   }
 }
 
-// halo rows: 2 for the Roe family (limiter neighbours), 1 for HLL/HLLC
-int stage3d_rows_per_block(int kind) { return kStage3dWaves - 2 * ((kind == 1 || kind == 2) ? 1 : 2); }
+// output rows per workgroup of the Roe-family kernel: 2 halo rows per side
+// (limiter neighbours)
+int stage3d_rows_per_block(int kind) { return kStage3dWaves - 2 * 2; }
 
+// the Roe-family kernel runs the generic combination (mode ignored); the
+// first-order kernel takes every mode
 void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, double>& a, hipStream_t st) {
-  (void)mode;  // 3D stages use the generic combination
-  launch3_solver<double, double, false, kStage3dWaves>(solver, lim, a, st);
+  launch3_solver<double, double, false, kStage3dWaves>(solver, lim, mode, a, st);
 }
 
 void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,
                           hipStream_t st) {
-  (void)mode;
-  if (kahan) launch3_solver<float, float, true, kStage3dWaves>(solver, lim, a, st);
-  else launch3_solver<float, float, false, kStage3dWaves>(solver, lim, a, st);
+  if (kahan) launch3_solver<float, float, true, kStage3dWaves>(solver, lim, mode, a, st);
+  else launch3_solver<float, float, false, kStage3dWaves>(solver, lim, mode, a, st);
 }
 
 }  // namespace hydro

@@ -43,6 +43,11 @@ HLL_CASES = [
     ("EulerHLLC", [40, 24, 20], "RungeKutta3TVD", "MIRROR"),
     ("EulerHLL", [70, 20, 18], "RungeKutta2", "FREEFLOW"),
     ("EulerHLLC", [36, 28, 30], "RungeKutta4", "PERIODIC"),
+    # 4 interior rows (the smallest grid); 14 interior planes = two full
+    # z-blocks of the 3D kernel
+    ("EulerHLL", [30, 8, 18], "ForwardEuler", "FREEFLOW"),
+    # three strips (60 + 60 + 6 columns), z-blocks of 7, 7 and 5 planes
+    ("EulerHLLC", [130, 33, 23], "RungeKutta4", "MIRROR"),
 ]
 
 
@@ -125,3 +130,28 @@ def test_blast_3d_full_size_properties(gpu_lib):
     rho = qa[:, 0].reshape(256, 256, 256)[2:-2, 2:-2, 2:-2]
     noise_free = np.abs(rho - rho[:, :, ::-1]).max()
     assert noise_free < 1e-3  # only the 1e-4 seeded density noise breaks the symmetry
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_3d_first_order_decomposition_invariant(gpu_lib, precision, monkeypatch):
+    # the y-chunk height changes which waves compute the chunk-edge faces,
+    # not their arithmetic: the results are bit-identical
+    from hydrodynamicsgpu_amd import capi
+
+    d = config.make_desc([70, 45, 25], solverName="EulerHLLC", integratorName="RungeKutta3TVD",
+                         boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4, precision=precision)
+    q = _rand_state(d, 5)
+    out = []
+    for rows in (None, "8", "41"):
+        if rows is None:
+            monkeypatch.delenv("HYDRO_ROWS_PER_WAVE", raising=False)
+        else:
+            monkeypatch.setenv("HYDRO_ROWS_PER_WAVE", rows)
+        g = capi.Sim(d)
+        g.set_state(q)
+        g.run(3)
+        out.append(g.get_state())
+        g.close()
+    monkeypatch.delenv("HYDRO_ROWS_PER_WAVE", raising=False)
+    np.testing.assert_array_equal(out[1], out[0])
+    np.testing.assert_array_equal(out[2], out[0])
