@@ -7,6 +7,7 @@
 #include "device_common.hpp"
 #include "launch.hpp"
 #include "phys_euler.hpp"
+#include "phys_srhd.hpp"
 #include "policies.hpp"
 #include "stage_common.hpp"
 
@@ -179,13 +180,20 @@ HYDRO_DEV void finalize_dt(const FinalizeArgs& f, double m, bool anynan) {
 // One wave's minimum over columns strip*60+2 .. strip*60+61 of rows
 // 2 + chunk*rows .. (1D: the single row); NaN when a wave speed of one of
 // its cells was not finite.  strip and chunk are wave-uniform.
+// SOLVER 3 is SRHD Roe: `u0` is then the FP64 primitive planes and the
+// speeds are the Marti-Mueller eigenvalues of the mean primitives
+// (res/SRHDRoe.cl:76-118), with the Roe form of the cell dt.
 template <typename S, int DIM, int SOLVER>
 HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int chunk) {
   using Roe = EulerRoe<DIM, S>;
-  using HLL = HLLPolicy<DIM, S, SOLVER == 0 ? 1 : SOLVER>;
-  using Frozen = typename std::conditional<SOLVER == 0, typename Roe::Cell, typename HLL::Frozen>::type;
+  using HLL = HLLPolicy<DIM, S, SOLVER == 2 ? 2 : 1>;
+  using SR = SRHDRoePolicy<DIM, S, 0>;
+  using Frozen = typename std::conditional<
+      SOLVER == 0, typename Roe::Cell,
+      typename std::conditional<SOLVER == 3, typename SR::Frozen, typename HLL::Frozen>::type>::type;
+  constexpr bool kRoeForm = SOLVER == 0 || SOLVER == 3;
   constexpr int N = DIM + 2;
-  constexpr int NF = SOLVER == 0 ? DIM + 2 : DIM + 5;
+  constexpr int NF = SOLVER == 0 ? RoePolicy<DIM, S, 0>::NF : (SOLVER == 3 ? SR::NF : HLL::NF);
 
   const int lane = threadIdx.x & 63;
   const int sx = a.sx, sy = a.sy;
@@ -202,14 +210,17 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int
   };
   auto frozen = [&](const S q[N]) {
     if constexpr (SOLVER == 0) return Roe::template cell<S>(q, gamma);
+    else if constexpr (SOLVER == 3) return SR::template frozen<S>(q, gamma);
     else return HLL::template frozen<S>(q, gamma);
   };
   auto pack = [&](const Frozen& f, S o[NF]) {
     if constexpr (SOLVER == 0) RoePolicy<DIM, S, 0>::pack(f, o);
+    else if constexpr (SOLVER == 3) SR::pack(f, o);
     else HLL::pack(f, o);
   };
   auto unpack = [&](const S o[NF]) {
     if constexpr (SOLVER == 0) return RoePolicy<DIM, S, 0>::unpack(o);
+    else if constexpr (SOLVER == 3) return SR::unpack(o);
     else return HLL::unpack(o);
   };
   // (min, max) wave speed of the face between L and R, normal `side`
@@ -218,6 +229,10 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int
       const auto f = Roe::face(L, R, side, gamma);
       lmin = f.v[0] - f.c;
       lmax = f.v[0] + f.c;
+    } else if constexpr (SOLVER == 3) {
+      const auto m = SR::mean(L, R, side, gamma);
+      lmin = m.lmin;
+      lmax = m.lmax;
     } else {
       HLL::speeds(L, R, side, gamma, lmin, lmax);
     }
@@ -227,7 +242,7 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int
   // right face's min speed
   auto side_dt = [&](S lmaxL, S lminR, bool right_ok, double kdx) {
     bad |= !(lmaxL == lmaxL) || (right_ok && !(lminR == lminR));
-    if constexpr (SOLVER == 0) {
+    if constexpr (kRoeForm) {
       const double lmax = fmax(0.0, double(lmaxL));
       const double lmin = fmin(0.0, double(lminR));
       return kdx / (fabs(lmax - lmin) + 1e-9);

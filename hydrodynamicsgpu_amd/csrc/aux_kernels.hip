@@ -220,6 +220,12 @@ static void dt_march(const DtMarchArgs& a, const S* u0, hipStream_t st) {
   const int blocks = (a.nwaves + 3) / 4;
   dim3 g(blocks), b(256);
 #define L(D, SV) hipLaunchKernelGGL((dt_march_kernel<S, D, SV>), g, b, 0, st, a, u0)
+  if constexpr (sizeof(S) == 8) {
+    if (a.solver == 3) {  // SRHD: u0 is the FP64 primitive planes
+      if (a.dim == 1) L(1, 3); else L(2, 3);
+      return;
+    }
+  }
   if (a.dim == 1) {
     if (a.solver == 0) L(1, 0); else if (a.solver == 1) L(1, 1); else L(1, 2);
   } else {

@@ -62,6 +62,7 @@ struct hydro_sim {
   int tot_blocks = 256;
   // stage decomposition
   int rows = 32, wpb = 4;
+  int zwaves = 8;  // first-order 3D kernel: waves per workgroup
   // measurement: event pairs around fused stage launches
   bool profiling = false;
   // two captured updates per starting parity (ping-pong state), replayed by run()
@@ -173,7 +174,9 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
   const int march_strips = (s->sz[0] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 2) / 60 + 1;
   const long long march_waves =
       (long long)march_strips * (s->dim > 1 ? (s->sz[1] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 1 + kDtRows - 1) / kDtRows : 1);
-  if (!s->d.use_fixed_dt && s->dim <= 2 && s->d.equation == HYDRO_EQN_EULER && march_waves <= kDtMaxWaves) {
+  const bool srhd = s->d.equation == HYDRO_EQN_SRHD;
+  if (!s->d.use_fixed_dt && s->dim <= 2 && (s->d.equation == HYDRO_EQN_EULER || srhd) &&
+      march_waves <= kDtMaxWaves) {
     // strip march, one minimum per wave, then the one-block tail
     DtMarchArgs a{};
     a.dim = s->dim;
@@ -183,7 +186,7 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
     a.kdx[0] = s->kdx[0];
     a.kdx[1] = s->kdx[1];
     a.gamma = s->d.gamma;
-    a.solver = s->d.solver;
+    a.solver = srhd ? 3 : s->d.solver;
     const int hi = s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
     a.last_x = s->sz[0] - hi;
     a.last_y = s->dim > 1 ? s->sz[1] - hi : 0;
@@ -200,7 +203,8 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
     }
     a.nwaves = a.nstrips * a.nchunks;
     a.wavemin = s->blockmin;
-    if (s->mixed) launch_dt_march_f32(a, (const float*)s->U[s->cur], s->stream);
+    if (srhd) launch_dt_march_f64(a, s->prim, s->stream);
+    else if (s->mixed) launch_dt_march_f32(a, (const float*)s->U[s->cur], s->stream);
     else launch_dt_march_f64(a, (const double*)s->U[s->cur], s->stream);
     f.nblocks = a.nwaves;
     launch_finalize_dt(f, s->stream);
@@ -281,7 +285,8 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
     // 3D first order: rows = y rows per chunk, workgroups of W-1 z-planes
     a.rows = s->rows;
     a.nchunks = (s->sz[1] - 4 + a.rows - 1) / a.rows;
-    const int nzb = (s->sz[2] - 4 + kStage3dR1Waves - 2) / (kStage3dR1Waves - 1);
+    a.zwaves = s->zwaves;
+    const int nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
     a.nwaves = a.nstrips * a.nchunks * nzb;
   } else {
     // 3D Roe family: rows = z-planes per workgroup, nchunks = y-chunks, nwaves = workgroups
@@ -505,8 +510,10 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     s->rows = pick_rows_2d((s->sz[0] - 4 + 59) / 60, s->sz[1] - 4, 4 * ncu, min_waves);
   } else if (s->dim == 3 && (kernel_kind(*d) == 1 || kernel_kind(*d) == 2)) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
-    const long long nzb = (s->sz[2] - 4 + kStage3dR1Waves - 2) / (kStage3dR1Waves - 1);
-    s->rows = pick_rows_3d_r1(strips * nzb, s->sz[1] - 4, ncu);
+    s->zwaves = stage3d_r1_waves(!s->mixed);
+    if (const char* e = getenv("HYDRO_3D_ZWAVES")) s->zwaves = atoi(e) == 4 ? 4 : 8;
+    const long long nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
+    s->rows = pick_rows_3d_r1(strips * nzb, s->sz[1] - 4, ncu * (8 / s->zwaves));
   } else if (s->dim == 3) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
     const int ry = stage3d_rows_per_block(kernel_kind(*d));

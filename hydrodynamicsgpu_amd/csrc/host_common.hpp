@@ -99,23 +99,37 @@ struct StagePlan {
   int mode = CM_GENERIC;  // CombineMode the stage kernel runs
 };
 
-// Replay plan: valid when every row before the last is d_{k+1} = beta[k][k]
-// dt L_k + (alpha[k][0] u0 part), i.e. the rows do not mix earlier stages
-// (classic RK4, RK2 midpoint, Heun, Ralston).  Then dt L_k = d_{k+1} /
-// beta[k][k] exactly up to one rounding, the last row is a combination of the
-// stored increments, and no partial-sum buffer is read or written.
+// Replay plan: valid when every row before the last has the form
+// d_{k+1} = alpha[k][k] d_k + beta[k][k] dt L_k (plus its u0 part), i.e. the
+// rows mix no earlier stage than their own input (classic RK4, RK2 midpoint,
+// Heun, Ralston, TVD-RK2, the reference's TVD-RK3, RK4 non-TVD).  Then
+// dt L_k = (d_{k+1} - alpha[k][k] d_k) / beta[k][k] up to rounding, the last
+// row is a combination of the stored increments, and no partial-sum buffer
+// is read or written: stage k reads d_k (buffer k-1) and writes d_{k+1}
+// (buffer k).
 inline bool plan_replay(const Tableau& t, std::vector<StagePlan>& out) {
   const int S = t.order;
   if (S < 2 || S - 1 > kMaxPartials) return false;
-  for (int k = 0; k < S; ++k) {
-    for (int j = 1; j < S; ++j)
+  for (int k = 0; k < S - 1; ++k) {
+    for (int j = 1; j < k; ++j)
       if (t.a[k][j] != 0.0) return false;
-    if (k < S - 1) {
-      if (t.b[k][k] == 0.0) return false;
-      for (int j = 0; j < k; ++j)
-        if (t.b[k][j] != 0.0) return false;
-    }
+    for (int j = 0; j < k; ++j)
+      if (t.b[k][j] != 0.0) return false;
+    if (t.b[k][k] == 0.0) return false;
   }
+  // last row: d_S = sum_j c[j] d_j + beta[S-1][S-1] dt L_{S-1}
+  double c[4] = {0, 0, 0, 0};
+  for (int j = 1; j < S; ++j) c[j] = t.a[S - 1][j];
+  for (int j = 0; j < S - 1; ++j) {
+    const double bj = t.b[S - 1][j];
+    if (bj == 0.0) continue;
+    c[j + 1] += bj / t.b[j][j];
+    if (j > 0) c[j] -= bj * t.a[j][j] / t.b[j][j];
+  }
+  int nrep = 0;
+  for (int j = 1; j < S - 1; ++j)
+    if (c[j] != 0.0) ++nrep;
+  if (nrep > kMaxPartials) return false;
   out.assign(S, StagePlan());
   for (int k = 0; k < S; ++k) {
     StagePlan& p = out[k];
@@ -123,13 +137,14 @@ inline bool plan_replay(const Tableau& t, std::vector<StagePlan>& out) {
     p.dout = k < S - 1 ? k : -1;
     p.b_self = t.b[k][k];
     p.final_stage = k == S - 1;
-    if (p.final_stage) {
-      // d_S = sum_{j<S-1} beta[S-1][j] / beta[j][j] d_{j+1} + beta[S-1][S-1] dt L_{S-1}
-      p.a_self = t.b[S - 1][S - 2] / t.b[S - 2][S - 2];  // coefficient of the stencil input d_{S-1}
-      for (int j = 0; j < S - 2; ++j) {
-        if (t.b[S - 1][j] == 0.0) continue;
-        p.rep[p.nrep] = j;  // buffer holding d_{j+1}
-        p.rc[p.nrep] = t.b[S - 1][j] / t.b[j][j];
+    if (!p.final_stage) {
+      p.a_self = k > 0 ? t.a[k][k] : 0.0;  // coefficient of the stencil input d_k
+    } else {
+      p.a_self = c[S - 1];
+      for (int j = 1; j < S - 1; ++j) {
+        if (c[j] == 0.0) continue;
+        p.rep[p.nrep] = j - 1;  // buffer holding d_j
+        p.rc[p.nrep] = c[j];
         p.nrep++;
       }
     }

@@ -378,8 +378,10 @@ __global__ void __launch_bounds__(64 * W) stage3d_r1_kernel(StageArgs<T, S> a) {
 template <class Pol, typename T, typename S, bool KAHAN, int MODE>
 static void launch3_r1_mode(const StageArgs<T, S>& a, hipStream_t st) {
   const int blocks = (a.nwaves + 7) & ~7;  // multiple of 8 for the XCD remap
-  hipLaunchKernelGGL((stage3d_r1_kernel<Pol, T, S, KAHAN, MODE, kStage3dR1Waves>), dim3(blocks),
-                     dim3(64 * kStage3dR1Waves), 0, st, a);
+  if (a.zwaves == 4)
+    hipLaunchKernelGGL((stage3d_r1_kernel<Pol, T, S, KAHAN, MODE, 4>), dim3(blocks), dim3(64 * 4), 0, st, a);
+  else
+    hipLaunchKernelGGL((stage3d_r1_kernel<Pol, T, S, KAHAN, MODE, 8>), dim3(blocks), dim3(64 * 8), 0, st, a);
 }
 
 template <class Pol, typename T, typename S, bool KAHAN>

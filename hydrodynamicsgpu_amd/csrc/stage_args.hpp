@@ -55,6 +55,7 @@ struct StageArgs {
   int lim;                   // runtime limiter id (generic instantiation)
   int nstrips, rows, nchunks;  // wave -> (strip, chunk) decomposition
   int nwaves;
+  int zwaves;                // first-order 3D kernel: waves (z-planes) per workgroup
 };
 
 }  // namespace hydro

@@ -118,7 +118,8 @@ def test_fp64_parity(gpu_lib, size, integ, lim, bm):
     assert _relerr(g.get_state(), o.get_state()) < 1e-10
 
 
-@pytest.mark.parametrize("integ", ["RungeKutta4", "RungeKutta2", "RungeKutta2Heun", "RungeKutta2Ralston"])
+@pytest.mark.parametrize("integ", ["RungeKutta4", "RungeKutta2", "RungeKutta2Heun", "RungeKutta2Ralston",
+                                   "RungeKutta2TVD", "RungeKutta3TVD", "RungeKutta4NonTVD"])
 def test_rk_plans_agree(gpu_lib, integ, monkeypatch):
     # the increment-replay plan (no partial sums) and the generic partial-sum
     # plan evaluate the same tableau; both match the oracle
