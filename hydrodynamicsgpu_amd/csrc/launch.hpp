@@ -21,7 +21,7 @@ constexpr int kStage3dWaves = 8;
 // first-order 3D kernel: waves (z-planes) per workgroup (4 or 8), one of
 // them supplies only the top z-face.  Four measured faster in both
 // precisions on the 256^3 blast (more workgroups per CU hide the barrier).
-inline int stage3d_r1_waves(bool fp64) { return fp64 ? 4 : 4; }
+inline int stage3d_r1_waves(bool) { return 4; }
 int stage3d_rows_per_block(int solver);
 void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, double>& a, hipStream_t st);
 void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,

@@ -237,8 +237,8 @@ static void launch_pol_mode(const StageArgs<T, S>& a, int wpb, hipStream_t st) {
   hipLaunchKernelGGL((stage2d_kernel<Pol, DIM, T, S, KAHAN, MODE>), dim3(blocks), dim3(64 * wpb), 0, st, a);
 }
 
-// Fixed-function combination modes are instantiated for the 2D Euler
-// policies with compile-time limiters; everything else runs CM_GENERIC.
+// Fixed-function combination modes are instantiated for the 2D policies
+// with compile-time limiters; everything else runs CM_GENERIC.
 template <class Pol, int DIM, typename T, typename S, bool KAHAN, bool MODES>
 static void launch_pol(int mode, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
   if constexpr (MODES) {
@@ -263,16 +263,16 @@ static void launch_solver(int kind, int lim, int mode, const StageArgs<T, S>& a,
     launch_pol<HLLPolicy<DIM, T, 2>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
   } else if (kind == 3) {
     switch (lim) {
-    case LIM_SUPERBEE: launch_pol<SRHDRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, false>(mode, a, wpb, st); break;
+    case LIM_SUPERBEE: launch_pol<SRHDRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, M>(mode, a, wpb, st); break;
     case LIM_MONOTIZEDCENTRAL:
-      launch_pol<SRHDRoePolicy<DIM, T, LIM_MONOTIZEDCENTRAL>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+      launch_pol<SRHDRoePolicy<DIM, T, LIM_MONOTIZEDCENTRAL>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
       break;
     default: launch_pol<SRHDRoePolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN, false>(mode, a, wpb, st); break;
     }
   } else if (kind == 4) {
     switch (lim) {
     case LIM_SUPERBEE:
-      launch_pol<MaxwellRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+      launch_pol<MaxwellRoePolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
       break;
     default: launch_pol<MaxwellRoePolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN, false>(mode, a, wpb, st); break;
     }
