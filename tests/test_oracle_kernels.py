@@ -170,3 +170,95 @@ def test_srhd_round_trip_and_basis():
         lam, L, R = oracle.srhd_eigenbasis(2, side, gamma, primL, primR)
         np.testing.assert_allclose(L @ R, np.eye(4), atol=1e-11)
         assert lam[0] < lam[1] < lam[3]
+
+
+def _toro_hllc(qL, qR, sl, sr, dim, side, gamma):
+    """Toro, Riemann Solvers and Numerical Methods for Fluid Dynamics (3rd
+    ed.), eqs. 10.37-10.39: the star speed, the star states U*_K and
+    F*_K = F_K + S_K (U*_K - U_K), picked by the signs of S_L, S*, S_R."""
+    def prim(q):
+        rho = q[0]
+        v = q[1:1 + dim] / rho
+        p = (gamma - 1) * (q[dim + 1] - 0.5 * rho * v @ v)
+        return rho, v, p
+    rL, vL, pL = prim(qL)
+    rR, vR, pR = prim(qR)
+    uL, uR = vL[side], vR[side]
+    ss = (pR - pL + rL * uL * (sl - uL) - rR * uR * (sr - uR)) / (rL * (sl - uL) - rR * (sr - uR))
+    if 0 <= sl:
+        return euler_flux(qL, dim, side, gamma)
+    if sr <= 0:
+        return euler_flux(qR, dim, side, gamma)
+    rho, v, p, q, s = (rL, vL, pL, qL, sl) if 0 <= ss else (rR, vR, pR, qR, sr)
+    u = v[side]
+    f = rho * (s - u) / (s - ss)
+    ustar = np.empty(dim + 2)
+    ustar[0] = f
+    for a in range(dim):
+        ustar[1 + a] = f * (ss if a == side else v[a])
+    ustar[dim + 1] = f * (q[dim + 1] / rho + (ss - u) * (ss + p / (rho * (s - u))))
+    return euler_flux(q, dim, side, gamma) + s * (ustar - q)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_hllc_star_flux_matches_toro(dim):
+    # the reference's HLLC (res/EulerHLLC.cl, HLLC_METHOD 2) writes the star
+    # flux with the mean pressure P_LR; with its own star speed P_LR equals
+    # each side's p*, so it is Toro's F*_K up to rounding
+    gamma = 1.4
+    rng = np.random.default_rng(5 + dim)
+    checked = 0
+    for _ in range(40):
+        vL = rng.uniform(-.6, .6, dim)
+        vR = rng.uniform(-.6, .6, dim)
+        qL = cons(rng.uniform(.2, 2), vL, rng.uniform(.2, 2), gamma)
+        qR = cons(rng.uniform(.2, 2), vR, rng.uniform(.2, 2), gamma)
+        for side in range(dim):
+            ev = oracle.hll_speeds(2, dim, side, gamma, qL, qR)
+            sl, sr = ev[0], ev[-1]
+            got = oracle.hll_flux(2, dim, side, gamma, qL, qR, ev)
+            want = _toro_hllc(qL, qR, sl, sr, dim, side, gamma)
+            np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12)
+            checked += sl < 0 < sr
+    assert checked > 20  # most draws exercise the star region
+
+
+def _srhd_cons_flux(P, gamma, dim, side):
+    """SR conserved state and flux along `side` from primitives (rho, v, eInt)
+    (Marti & Mueller, Living Reviews in Relativity 2003, eqs. 11-13)."""
+    rho, v, eint = P[0], np.asarray(P[1:1 + dim]), P[1 + dim]
+    W = 1 / np.sqrt(1 - v @ v)
+    p = (gamma - 1) * rho * eint
+    h = 1 + eint + p / rho
+    D = rho * W
+    S = rho * h * W * W * v
+    tau = rho * h * W * W - p - D
+    U = np.concatenate([[D], S, [tau]])
+    vn = v[side]
+    F = np.concatenate([[D * vn], S * vn + p * (np.arange(dim) == side), [S[side] - D * vn]])
+    return U, F
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_srhd_basis_diagonalises_the_flux_jacobian(dim):
+    # with equal sides the mean state is the cell's own: R diag(lambda) L must
+    # be dF/dU there; dF/dU = (dF/dP)(dU/dP)^-1 by central differences in the
+    # primitives
+    gamma = 5 / 3
+    for rho, v, eint in ((1.0, [.3, -.2], 1.5), (.4, [-.6, .1], .2)):
+        P = np.array([rho] + v[:dim] + [eint])
+        for side in range(dim):
+            lam, L, R = oracle.srhd_eigenbasis(dim, side, gamma, P, P)
+            n = dim + 2
+            dU = np.zeros((n, n))
+            dF = np.zeros((n, n))
+            for j in range(n):
+                e = np.zeros(n)
+                e[j] = 1e-6 * max(1.0, abs(P[j]))
+                Up, Fp = _srhd_cons_flux(P + e, gamma, dim, side)
+                Um, Fm = _srhd_cons_flux(P - e, gamma, dim, side)
+                dU[:, j] = (Up - Um) / (2 * e[j])
+                dF[:, j] = (Fp - Fm) / (2 * e[j])
+            A = dF @ np.linalg.inv(dU)
+            np.testing.assert_allclose(R @ np.diag(lam) @ L, A, rtol=1e-6, atol=1e-7)
+            np.testing.assert_allclose(np.sort(lam), np.sort(np.linalg.eigvals(A).real), rtol=1e-6, atol=1e-8)
