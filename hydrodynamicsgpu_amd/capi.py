@@ -39,6 +39,9 @@ def lib():
         L.hydro_sim_volume.restype = sz
         for n in ("hydro_sim_set_state", "hydro_sim_get_state", "hydro_sim_get_primitives"):
             getattr(L, n).argtypes = [vp, dp, sz]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.hydro_sim_restore.argtypes = [vp, dp, fp, dp, sz, ctypes.c_double, ctypes.c_longlong]
+        L.hydro_sim_get_storage.argtypes = [vp, dp, fp, sz]
         L.hydro_sim_boundary.argtypes = [vp]
         L.hydro_sim_calc_timestep.argtypes = [vp, dp]
         L.hydro_sim_step.argtypes = [vp, ctypes.c_double]
@@ -127,6 +130,29 @@ class Sim:
         check(lib().hydro_sim_get_primitives(self._h, _dp(out), out.size))
         return out
 
+    def get_storage(self):
+        """(high words as float64, low words as float32) of compensated FP32
+        storage; (state, zeros) otherwise."""
+        hi = np.empty(self.N * self.volume)
+        lo = np.empty(self.N * self.volume, dtype=np.float32)
+        check(lib().hydro_sim_get_storage(self._h, _dp(hi), lo.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                          hi.size))
+        return hi, lo
+
+    def restore(self, aos, t, steps, prims=None, low=None):
+        """Install a state read back by get_state / get_storage (with its low
+        words, and for SRHD its primitives) unchanged, with the clock at
+        (t, steps)."""
+        a = np.ascontiguousarray(aos, dtype=np.float64)
+        p = None if prims is None else np.ascontiguousarray(prims, dtype=np.float64)
+        lo = None if low is None else np.ascontiguousarray(low, dtype=np.float32)
+        for x in (p, lo):
+            if x is not None and x.size != a.size:
+                raise HydroError("primitives and low words must have the state's size")
+        fp = None if lo is None else lo.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        check(lib().hydro_sim_restore(self._h, _dp(a), fp, None if p is None else _dp(p), a.size, float(t),
+                                      int(steps)))
+
     def boundary(self):
         check(lib().hydro_sim_boundary(self._h))
 
@@ -184,6 +210,7 @@ EXPORTED_SYMBOLS = [
     "hydro_sim_desc_default", "hydro_last_error", "hydro_version", "hydro_ctx_create",
     "hydro_ctx_destroy", "hydro_sim_create", "hydro_sim_destroy", "hydro_sim_num_states",
     "hydro_sim_volume", "hydro_sim_set_state", "hydro_sim_get_state", "hydro_sim_get_primitives",
+    "hydro_sim_restore", "hydro_sim_get_storage",
     "hydro_sim_boundary", "hydro_sim_calc_timestep", "hydro_sim_step", "hydro_sim_update",
     "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
     "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_batch_create", "hydro_batch_destroy", "hydro_batch_set_state",

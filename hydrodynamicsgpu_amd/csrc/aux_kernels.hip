@@ -322,7 +322,8 @@ __global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, flo
     double p[N];
 #pragma unroll
     for (int s = 0; s < N; ++s) p[s] = prim[s * a.plane + cell];
-    if (srhd_con2prim<DIM>(defs, q, p)) {
+    // FP32 storage: warm-started Newton (same root, see srhd_con2prim)
+    if (srhd_con2prim<DIM, sizeof(S) == 4>(defs, q, p)) {
 #pragma unroll
       for (int s = 0; s < N; ++s) prim[s * a.plane + cell] = p[s];
     } else {

@@ -575,6 +575,33 @@ int hydro_sim_num_states(const hydro_sim* s) { return s ? s->N : 0; }
 size_t hydro_sim_volume(const hydro_sim* s) { return s ? (size_t)s->vol : 0; }
 void* hydro_sim_stream(hydro_sim* s) { return s ? (void*)s->stream : nullptr; }
 
+// AoS (reference layout) -> the device planes of the current state buffer;
+// FP32 storage keeps the residual as the low word when compensated, or
+// takes the low words given (`low`, compensated storage only)
+static int upload_state(hydro_sim* s, const double* aos, const float* low = nullptr) {
+  const int N = s->N;
+  const long long V = s->vol;
+  if (s->mixed) {
+    std::vector<float> hi((size_t)s->plane * N, 0.f), lo((size_t)s->plane * N, 0.f);
+    for (long long c = 0; c < V; ++c)
+      for (int k = 0; k < N; ++k) {
+        const double v = aos[k + (size_t)N * c];
+        const float h = (float)v;
+        hi[(size_t)k * s->plane + c] = h;
+        lo[(size_t)k * s->plane + c] = low ? low[k + (size_t)N * c] : (float)(v - (double)h);
+      }
+    HIP_TRY(hipMemcpy(s->U[s->cur], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (s->kahan) HIP_TRY(hipMemcpy(s->Ulo[s->cur], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> buf((size_t)s->plane * N, 0.0);
+    for (long long c = 0; c < V; ++c)
+      for (int k = 0; k < N; ++k) buf[(size_t)k * s->plane + c] = aos[k + (size_t)N * c];
+    HIP_TRY(hipMemcpy(s->U[s->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  s->ghosts_pending = false;
+  return HYDRO_OK;
+}
+
 int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
   if (!s || !aos) return fail(HYDRO_E_INVALID, "null argument");
   if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "state size must be N * volume");
@@ -611,26 +638,34 @@ int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
     HIP_TRY(hipMemcpy(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
     aos = conv.data();
   }
-  if (s->mixed) {
-    std::vector<float> hi((size_t)s->plane * N, 0.f), lo((size_t)s->plane * N, 0.f);
-    for (long long c = 0; c < V; ++c)
-      for (int k = 0; k < N; ++k) {
-        const double v = aos[k + (size_t)N * c];
-        const float h = (float)v;
-        hi[(size_t)k * s->plane + c] = h;
-        lo[(size_t)k * s->plane + c] = (float)(v - (double)h);
-      }
-    HIP_TRY(hipMemcpy(s->U[s->cur], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
-    if (s->kahan) HIP_TRY(hipMemcpy(s->Ulo[s->cur], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
-  } else {
-    std::vector<double> buf((size_t)s->plane * N, 0.0);
-    for (long long c = 0; c < V; ++c)
-      for (int k = 0; k < N; ++k) buf[(size_t)k * s->plane + c] = aos[k + (size_t)N * c];
-    HIP_TRY(hipMemcpy(s->U[s->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
-  }
-  s->ghosts_pending = false;
+  const int rc = upload_state(s, aos);
+  if (rc) return rc;
   // a new initial state restarts the clock
   DevScalars z{};
+  HIP_TRY(hipMemcpy(s->sc, &z, sizeof z, hipMemcpyHostToDevice));
+  return HYDRO_OK;
+}
+
+int hydro_sim_restore(hydro_sim* s, const double* aos, const float* lo, const double* prims, size_t n, double t,
+                      long long steps) {
+  if (!s || !aos) return fail(HYDRO_E_INVALID, "null argument");
+  if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "state size must be N * volume");
+  if (lo && !s->kahan) return fail(HYDRO_E_UNSUPPORTED, "low words exist for compensated FP32 storage only");
+  if (s->prim && !prims) return fail(HYDRO_E_INVALID, "an SRHD state needs its primitives");
+  if (!s->prim && prims) return fail(HYDRO_E_UNSUPPORTED, "primitives exist for SRHD only");
+  if (!(t >= 0.0) || steps < 0) return fail(HYDRO_E_INVALID, "time and step count must be >= 0");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (prims) {
+    std::vector<double> pr((size_t)s->plane * s->N, 0.0);
+    for (long long c = 0; c < s->vol; ++c)
+      for (int k = 0; k < s->N; ++k) pr[(size_t)k * s->plane + c] = prims[k + (size_t)s->N * c];
+    HIP_TRY(hipMemcpy(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  const int rc = upload_state(s, aos, lo);
+  if (rc) return rc;
+  DevScalars z{};
+  z.t = t;
+  z.steps = steps;
   HIP_TRY(hipMemcpy(s->sc, &z, sizeof z, hipMemcpyHostToDevice));
   return HYDRO_OK;
 }
@@ -664,6 +699,32 @@ int hydro_sim_get_state(hydro_sim* s, double* aos, size_t n) {
     for (long long c = 0; c < V; ++c)
       for (int k = 0; k < N; ++k) aos[k + (size_t)N * c] = buf[(size_t)k * s->plane + c];
   }
+  return HYDRO_OK;
+}
+
+int hydro_sim_get_storage(hydro_sim* s, double* hi, float* lo, size_t n) {
+  if (!s || !hi) return fail(HYDRO_E_INVALID, "null argument");
+  if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "state size must be N * volume");
+  if (!s->kahan) {
+    const int rc = hydro_sim_get_state(s, hi, n);
+    if (rc) return rc;
+    if (lo) std::fill(lo, lo + n, 0.f);
+    return HYDRO_OK;
+  }
+  {
+    const int rc = materialize_ghosts(s);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int N = s->N;
+  std::vector<float> h((size_t)s->plane * N), l((size_t)s->plane * N);
+  HIP_TRY(hipMemcpy(h.data(), s->U[s->cur], h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(l.data(), s->Ulo[s->cur], l.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (long long c = 0; c < s->vol; ++c)
+    for (int k = 0; k < N; ++k) {
+      hi[k + (size_t)N * c] = h[(size_t)k * s->plane + c];
+      if (lo) lo[k + (size_t)N * c] = l[(size_t)k * s->plane + c];
+    }
   return HYDRO_OK;
 }
 

@@ -264,7 +264,15 @@ struct SRHDDefs {
   double D_min, D_max, tau_min, tau_max;
 };
 
-template <int DIM>
+//
+// WARM: start the Newton iteration from the cell's previous pressure
+// (gamma-1) rho eInt, clamped to [PMin, PMax], instead of the reference's
+// bracket midpoint.  Both starts converge to the same root; the stopping
+// test (relative change < stop_eps, quadratic convergence) leaves them
+// within about stop_eps^2 of it, far below FP32 resolution, and the warm
+// start needs a few iterations instead of tens.  The FP32-mixed path uses
+// it; the FP64 path keeps the reference's start.
+template <int DIM, bool WARM = false>
 HYDRO_DEV bool srhd_con2prim(const SRHDDefs& d, const double cons[DIM + 2], double prim[DIM + 2]) {
   constexpr int E = DIM + 1;
   const double gamma = d.gamma;
@@ -277,6 +285,10 @@ HYDRO_DEV bool srhd_con2prim(const SRHDDefs& d, const double cons[DIM + 2], doub
   double PMax = (gamma - 1.) * tau;
   PMax = fmax(PMax, PMin);
   double P = .5 * (PMin + PMax);
+  if constexpr (WARM) {
+    const double Pold = (gamma - 1.) * prim[0] * prim[E];
+    if (Pold == Pold) P = fmin(fmax(Pold, PMin), PMax);
+  }
   for (int iter = 0; iter < d.max_iter; ++iter) {
     const double vLen = SLen / (tau + D + P);
     const double vSq = vLen * vLen;

@@ -74,6 +74,19 @@ struct Solver : public ISolver {
   std::vector<real> getState();
   void setState(const std::vector<real>& state);
 
+  // src/Solver/Solver.cpp:547-589: one FITS image per state channel named
+  // `<state name><index>.fits` at the first free index (getSaveIndex), the
+  // full grid with ghosts, as 32-bit floats
+  virtual std::vector<std::string> getSaveChannelNames();
+  int getSaveIndex();
+  virtual void save();
+
+  // no counterpart in the reference: a checkpoint of the state (with the
+  // SRHD primitives) and the clock, and the exact resume from one (the
+  // descriptor must describe the same grid and equation)
+  void saveCheckpoint(const std::string& path);
+  void loadCheckpoint(const std::string& path);
+
   // no counterpart in the reference: `nsteps` updates with dt kept on the
   // device (no per-step host synchronisation); simulated time and step count;
   // FP64 interior totals; health flags; the C handle

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <iostream>
 #include <map>
 
 #include "HydroGPU/Equation/Euler.h"
@@ -14,6 +15,7 @@
 #include "HydroGPU/HydroGPUApp.h"
 #include "HydroGPU/Integrator/ForwardEuler.h"
 #include "HydroGPU/Integrator/RungeKutta.h"
+#include "HydroGPU/Shared/Files.h"
 #include "HydroGPU/Solver/EulerHLL.h"
 #include "HydroGPU/Solver/EulerHLLC.h"
 #include "HydroGPU/Solver/EulerRoe.h"
@@ -222,6 +224,69 @@ unsigned Solver::health() {
   unsigned h = 0;
   checkStatus(hydro_sim_health(sim, &h), "health");
   return h;
+}
+
+std::vector<std::string> Solver::getSaveChannelNames() { return equation->states; }
+
+int Solver::getSaveIndex() {
+  const std::string first = getSaveChannelNames()[0];
+  for (int i = 0; i < 1000000; ++i)
+    if (!fileExists(first + std::to_string(i) + ".fits")) return i;
+  throw Exception() << "failed to find available save filename";
+}
+
+void Solver::save() {
+  const int saveIndex = getSaveIndex();
+  const std::vector<real> state = getState();
+  const std::vector<std::string> names = getSaveChannelNames();
+  const int n = numStates();
+  const int sx = app->size[0], sy = app->size[1], sz = app->size[2];
+  std::vector<float> image((size_t)sx * sy * sz);
+  for (int channel = 0; channel < (int)names.size(); ++channel) {
+    for (size_t cell = 0; cell < image.size(); ++cell) image[cell] = (float)state[channel + (size_t)n * cell];
+    const std::string filename = names[channel] + std::to_string(saveIndex) + ".fits";
+    std::cout << "saving file " << filename << std::endl;
+    writeFits(filename, image, sx, sy, sz);
+  }
+}
+
+void Solver::saveCheckpoint(const std::string& path) {
+  Checkpoint c;
+  c.desc = desc_;
+  c.numStates = numStates();
+  c.volume = getVolume();
+  checkStatus(hydro_sim_time(sim, &c.t, &c.steps, nullptr), "saveCheckpoint");
+  c.state.resize((size_t)c.numStates * (size_t)c.volume);
+  if (desc_.precision == HYDRO_PRECISION_FP32_MIXED && (desc_.flags & HYDRO_FLAG_KAHAN)) {
+    c.low.resize(c.state.size());
+    checkStatus(hydro_sim_get_storage(sim, c.state.data(), c.low.data(), c.state.size()), "saveCheckpoint");
+  } else {
+    c.state = getState();
+  }
+  if (desc_.equation == HYDRO_EQN_SRHD) {
+    c.prims.resize(c.state.size());
+    checkStatus(hydro_sim_get_primitives(sim, c.prims.data(), c.prims.size()), "saveCheckpoint");
+  }
+  writeCheckpoint(path, c);
+}
+
+void Solver::loadCheckpoint(const std::string& path) {
+  const Checkpoint c = readCheckpoint(path);
+  if (c.numStates != numStates() || c.volume != getVolume() || c.desc.equation != desc_.equation ||
+      c.desc.dim != desc_.dim)
+    throw Exception() << path << " holds a different grid or equation";
+  std::vector<double> state = c.state;
+  const float* low = nullptr;
+  if (!c.low.empty()) {
+    if (desc_.precision == HYDRO_PRECISION_FP32_MIXED && (desc_.flags & HYDRO_FLAG_KAHAN))
+      low = c.low.data();
+    else
+      for (size_t i = 0; i < state.size(); ++i) state[i] += (double)c.low[i];
+  }
+  checkStatus(hydro_sim_restore(sim, state.data(), low, c.prims.empty() ? nullptr : c.prims.data(), state.size(),
+                                c.t, c.steps),
+              "loadCheckpoint");
+  frame = (int)c.steps;
 }
 
 void EulerRoe::createEquation() { equation = std::make_shared<Equation::Euler>(app); }

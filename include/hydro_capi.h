@@ -188,6 +188,26 @@ int hydro_sim_get_state(hydro_sim* sim, double* aos, size_t n);
 /* SRHD primitives (rho, v, eInt) in the reference layout (src/Solver/SRHDRoe.cpp:9-13). */
 int hydro_sim_get_primitives(hydro_sim* sim, double* aos, size_t n);
 
+/* The stored form of the state: for compensated FP32 storage
+ * (HYDRO_FLAG_KAHAN) the high words, widened to double, and the low words
+ * (`lo`, may be NULL), whose sum hydro_sim_get_state returns; for the other
+ * forms the state itself, and zeros in `lo`.  No counterpart in the
+ * reference; checkpoints keep this form so a resumed run repeats the
+ * original bit for bit. */
+int hydro_sim_get_storage(hydro_sim* sim, double* hi, float* lo, size_t n);
+
+/* Installs a state read back with hydro_sim_get_state or
+ * hydro_sim_get_storage as it is (no Newtonian-to-SR conversion, unlike
+ * hydro_sim_set_state): `lo` are the low words of compensated FP32 storage
+ * (NULL: split as hydro_sim_set_state does; must be NULL for the other
+ * forms), `prims` the SRHD primitives from hydro_sim_get_primitives (must
+ * be NULL for the other equations).  Sets the simulated time and step count.
+ * No counterpart in the reference, which has no checkpoints; it resumes a
+ * run saved by the checkpoint writers (hydrodynamicsgpu_amd/io.py,
+ * HydroGPU::Solver::Solver::saveCheckpoint). */
+int hydro_sim_restore(hydro_sim* sim, const double* aos, const float* lo, const double* prims, size_t n, double t,
+                      long long steps);
+
 /* Replaces Solver::boundary (src/Solver/Solver.cpp:446-464; SRHD adds the
  * primitive buffer, src/Solver/SRHDRoe.cpp:79-96). */
 int hydro_sim_boundary(hydro_sim* sim);

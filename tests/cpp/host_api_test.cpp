@@ -5,9 +5,16 @@
 //   host_api_test cpu   configuration errors that throw before any device work
 //   host_api_test gpu   Sod (Euler Roe, TVD-RK2, Superbee, free-flow, CFL 0.5,
 //                       to t = 0.2) and short runs of every solver class,
-//                       each compared with the oracle on the same descriptor
+//                       each compared with the oracle on the same descriptor;
+//                       checkpoint resume and Solver::save (in the cwd)
+//   host_api_test fits <path> <sx> <sy> <sz>
+//                       writes the image data[i] = .5 i - 3 (for the
+//                       byte-for-byte comparison with io.py)
+//   host_api_test ckpt-write <path> / ckpt-read <path>
+//                       writes a synthetic checkpoint / prints one's summary
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -15,6 +22,7 @@
 
 #include "HydroGPU/Equation/Equation.h"
 #include "HydroGPU/HydroGPUApp.h"
+#include "HydroGPU/Shared/Files.h"
 #include "HydroGPU/Solver/Solver.h"
 #include "../../oracle/hydro_oracle.h"
 
@@ -163,6 +171,23 @@ static int run_gpu() {
     CHECK(relerr(solver->getState(), ref) < 1e-10, "Sod run(5)");
     CHECK(solver->getSteps() == steps + 5, "step count %lld", solver->getSteps());
     oracle_destroy(o);
+    // checkpoint: save, run on, reload, run the same steps again -> the same
+    // bits and clock
+    solver->saveCheckpoint("sod.ckpt");
+    solver->run(7);
+    const std::vector<real> a = solver->getState();
+    const real ta = solver->getTime();
+    solver->loadCheckpoint("sod.ckpt");
+    CHECK(solver->getSteps() == steps + 5, "restored step count %lld", solver->getSteps());
+    solver->run(7);
+    CHECK(solver->getState() == a, "checkpoint resume differs");
+    CHECK(solver->getTime() == ta, "checkpoint resume time");
+    // Solver::save: one FITS image per state channel at the first free index
+    solver->save();
+    solver->save();
+    for (const char* f : {"DENSITY0.fits", "MOMENTUM_X0.fits", "ENERGY_TOTAL0.fits", "DENSITY1.fits"})
+      CHECK(HydroGPU::fileExists(f), "missing %s", f);
+    CHECK(!HydroGPU::fileExists("MOMENTUM_Y0.fits"), "1D grid saved a y channel");
   }
   auto kh = [&](real x, real y, real) {
     const bool in = std::fabs(y) < .25;
@@ -214,10 +239,58 @@ static int run_gpu() {
   return failures ? 1 : 0;
 }
 
+static int run_files(int argc, char** argv) {
+  const std::string mode = argv[1];
+  if (mode == "fits" && argc == 6) {
+    const int sx = std::atoi(argv[3]), sy = std::atoi(argv[4]), sz = std::atoi(argv[5]);
+    std::vector<float> img((size_t)sx * sy * sz);
+    for (size_t i = 0; i < img.size(); ++i) img[i] = .5f * (float)i - 3.f;
+    HydroGPU::writeFits(argv[2], img, sx, sy, sz);
+    return 0;
+  }
+  if (mode == "ckpt-write" && argc == 3) {
+    HydroGPU::Checkpoint c;
+    hydro_sim_desc_default(&c.desc);
+    c.desc.dim = 2;
+    c.desc.size[0] = 12;
+    c.desc.size[1] = 10;
+    c.desc.equation = HYDRO_EQN_SRHD;
+    c.numStates = 4;
+    c.volume = 120;
+    c.t = 0.125;
+    c.steps = 7;
+    c.state.resize(480);
+    c.prims.resize(480);
+    c.low.resize(480);
+    for (size_t i = 0; i < 480; ++i) {
+      c.state[i] = 1.0 + 1e-3 * (double)i;
+      c.prims[i] = -2.0 + 1e-2 * (double)i;
+      c.low[i] = 1e-9f * (float)i;
+    }
+    HydroGPU::writeCheckpoint(argv[2], c);
+    return 0;
+  }
+  if (mode == "ckpt-read" && argc == 3) {
+    const HydroGPU::Checkpoint c = HydroGPU::readCheckpoint(argv[2]);
+    double s = 0, p = 0, l = 0;
+    for (double v : c.state) s += v;
+    for (double v : c.prims) p += v;
+    for (float v : c.low) l += v;
+    std::printf("dim %d size %d %d %d equation %d states %d volume %lld t %.17g steps %lld sum %.17g prims %zu %.17g "
+                "low %zu %.17g\n",
+                c.desc.dim, c.desc.size[0], c.desc.size[1], c.desc.size[2], c.desc.equation, c.numStates, c.volume,
+                c.t, c.steps, s, c.prims.size(), p, c.low.size(), l);
+    return 0;
+  }
+  std::printf("usage: host_api_test cpu|gpu|fits <path> <sx> <sy> <sz>|ckpt-write <path>|ckpt-read <path>\n");
+  return 2;
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   try {
     if (mode == "gpu") return run_gpu();
+    if (mode != "cpu") return run_files(argc, argv);
     return run_cpu();
   } catch (const std::exception& e) {
     std::printf("FAIL: exception %s\n", e.what());

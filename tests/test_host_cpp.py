@@ -33,8 +33,9 @@ def test_host_layer_config_errors_cpu():
 
 
 @pytest.mark.gpu
-def test_host_layer_solvers_match_oracle(gpu_lib):
-    r = subprocess.run([_binary(), "gpu"], capture_output=True, text=True, timeout=300)
+def test_host_layer_solvers_match_oracle(gpu_lib, tmp_path):
+    # cwd: Solver::save and the checkpoint check write their files there
+    r = subprocess.run([_binary(), "gpu"], capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "passed gpu checks" in r.stdout
