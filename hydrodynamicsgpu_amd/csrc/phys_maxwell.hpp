@@ -100,11 +100,8 @@ struct MaxwellRoePolicy : StageIsState<6, T> {
       const bool up = l >= T(0);
       const T num = up ? dqlo[k] : dqhi[k];
       const T theta = up ? T(1) : T(-1);
-      const T r = tdiv(num, a.dq[k]);
-      const T phi = limiter<LIM, T>(lim, r);
       // l qavg - 1/2 l dq (theta + phi (eps - theta)), eps = l dt/dx
-      const T x = theta + phi * (l * dt_dx - theta);
-      ft[k] = l * (a.lq[k] - T(.5) * (a.dq[k] * x));
+      ft[k] = l * (a.lq[k] - T(.5) * limited_wave<LIM, T>(lim, num, a.dq[k], theta, l * dt_dx));
     }
     right(ft, side, a.se, a.su, F);
   }

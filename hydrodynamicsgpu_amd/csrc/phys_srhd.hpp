@@ -2,6 +2,8 @@
 //
 // State order (src/Equation/SRHD.cpp:36-40): D, S[dim], tau.  Primitives
 // (res/SRHDCommon.cl): rho, v[dim], eInt, kept in FP64.
+// FP32 arithmetic divides by reciprocal multiplies (tdiv / trcp, ~1 ulp);
+// FP64 keeps the correctly rounded division.
 // The interface basis follows res/SRHDRoe.cl:21-301: arithmetic mean of the
 // primitives (not a Roe average), Marti-Mueller eigenvalues and analytic left
 // and right eigenvectors built for the normal rotated into x, then rotated
@@ -91,11 +93,11 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
     m.vSq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     const T oneOverW2 = T(1) - m.vSq;
     m.oneOverW = tsqrt(oneOverW2);
-    m.W = T(1) / m.oneOverW;
-    m.W2 = T(1) / oneOverW2;
+    m.W = trcp(m.oneOverW);
+    m.W2 = trcp(oneOverW2);
     const T P = (gamma - T(1)) * rho * eInt;
-    m.h = T(1) + eInt + P / rho;
-    const T P_over_rho_h = P / (rho * m.h);
+    m.h = T(1) + eInt + tdiv(P, rho);
+    const T P_over_rho_h = tdiv(P, rho * m.h);
     if (side == 1) { m.v[0] = v[1]; m.v[1] = -v[0]; m.v[2] = v[2]; }
     else if (side == 2) { m.v[0] = v[2]; m.v[1] = v[1]; m.v[2] = -v[0]; }
     else { m.v[0] = v[0]; m.v[1] = v[1]; m.v[2] = v[2]; }
@@ -104,7 +106,7 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
     m.csSq = gamma * P_over_rho_h;
     m.cs = tsqrt(m.csSq);
     const T discr = tsqrt((T(1) - m.vSq) * ((T(1) - m.vSq * m.csSq) - vxSq * (T(1) - m.csSq)));
-    const T den = T(1) / (T(1) - m.vSq * m.csSq);
+    const T den = trcp(T(1) - m.vSq * m.csSq);
     m.lmin = (vx * (T(1) - m.csSq) - m.cs * discr) * den;
     m.lmax = (vx * (T(1) - m.csSq) + m.cs * discr) * den;
     return m;
@@ -125,8 +127,8 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
 #pragma unroll
       for (int j = 0; j < N; ++j) Lm[i][j] = Rm[i][j] = T(0);
     const T Kappa = h;
-    const T AMinus = (T(1) - vxSq) / (T(1) - vx * lmin);
-    const T APlus = (T(1) - vxSq) / (T(1) - vx * lmax);
+    const T AMinus = tdiv(T(1) - vxSq, T(1) - vx * lmin);
+    const T APlus = tdiv(T(1) - vxSq, T(1) - vx * lmax);
     // right eigenvectors: column j
     Rm[0][0] = T(1);
     Rm[1][0] = hW * AMinus * lmin;
@@ -162,20 +164,21 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
     const T Delta = hSq * hW * (Kappa - T(1)) * (T(1) - vxSq) * (APlus * lmax - AMinus * lmin);
     const T k2 = T(2) * Kappa - T(1);
     const T tq = W2 * (vSq - vxSq) * k2;
-    T scale = hSq / Delta;
+    const T hSq_Delta = tdiv(hSq, Delta);
+    T scale = hSq_Delta;
     Lm[0][0] = scale * (hW * APlus * (vx - lmax) - vx - tq * (vx - APlus * lmax) + Kappa * APlus * lmax);
     Lm[0][1] = scale * (T(1) + tq * (T(1) - APlus) - Kappa * APlus);
     if (DIM > 1) Lm[0][2] = scale * (W2 * vy * k2 * APlus * (vx - lmax));
     if (DIM > 2) Lm[0][3] = scale * (W2 * vz * k2 * APlus * (vx - lmax));
     Lm[0][E] = scale * (-vx - tq * (vx - APlus * lmax) + Kappa * APlus * lmax);
-    scale = W / (Kappa - T(1));
+    scale = tdiv(W, Kappa - T(1));
     Lm[1][0] = scale * (h - W);
     Lm[1][1] = scale * (W * vx);
     if (DIM > 1) Lm[1][2] = scale * (W * vy);
     if (DIM > 2) Lm[1][3] = scale * (W * vz);
     Lm[1][E] = scale * (-W);
     if (DIM > 1) {
-      scale = T(1) / (h * (T(1) - vxSq));
+      scale = trcp(h * (T(1) - vxSq));
       Lm[2][0] = scale * (-vy);
       Lm[2][1] = scale * (vx * vy);
       Lm[2][2] = scale * (T(1) - vxSq);
@@ -187,7 +190,7 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
         Lm[3][E] = scale * (-vz);
       }
     }
-    scale = -hSq / Delta;
+    scale = -hSq_Delta;
     Lm[E][0] = scale * (hW * AMinus * (vx - lmin) - vx - tq * (vx - AMinus * lmin) + Kappa * AMinus * lmin);
     Lm[E][1] = scale * (T(1) + tq * (T(1) - AMinus) - Kappa * AMinus);
     if (DIM > 1) Lm[E][2] = scale * (W2 * vy * k2 * AMinus * (vx - lmin));
@@ -235,11 +238,8 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
       const bool up = l >= T(0);
       const T num = up ? dqlo[k] : dqhi[k];
       const T theta = up ? T(1) : T(-1);
-      const T r = tdiv(num, a.dq[k]);
-      const T phi = limiter<LIM, T>(lim, r);
       // l qavg - 1/2 l dq (theta + phi (eps - theta)), eps = l dt/dx
-      const T x = theta + phi * (l * dt_dx - theta);
-      ft[k] = l * (a.lq[k] - T(.5) * (a.dq[k] * x));
+      ft[k] = l * (a.lq[k] - T(.5) * limited_wave<LIM, T>(lim, num, a.dq[k], theta, l * dt_dx));
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) {

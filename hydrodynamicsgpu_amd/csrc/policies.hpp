@@ -36,6 +36,35 @@ struct StageIsState {
   static HYDRO_DEV Stage unpack_stage(const T o[N]) { return stage(o, T(0)); }
 };
 
+// The limited characteristic term of the flux, dq (theta + phi(r) (eps -
+// theta)) with r = num / dq (res/Roe.cl:272-298).  The limiters with
+// compile-time piecewise-linear forms skip the division: with s = sign(dq),
+// b = |dq|, c = s num, dq phi(num / dq) = s phi~(c, b), phi written on the
+// pair (MC: max(0, min(2b, (b+c)/2, 2c)); Superbee: max(0, min(b, 2c),
+// min(2b, c)); MinMod: max(0, min(c, b))).  dq = 0 gives 0, which is what
+// the product gives for the reference's r = +-inf or 0/0 (phi stays finite
+// under fmin/fmax).
+template <int LIM, typename T>
+HYDRO_DEV T limited_wave(int lim, T num, T d, T theta, T eps) {
+  if constexpr (LIM == LIM_MONOTIZEDCENTRAL || LIM == LIM_SUPERBEE || LIM == LIM_MINMOD) {
+    (void)lim;
+    const T b = tabs(d);
+    const T c = d < T(0) ? -num : num;
+    T m;
+    if constexpr (LIM == LIM_MONOTIZEDCENTRAL)
+      m = tmax(T(0), tmin(T(2) * b, tmin(T(.5) * (b + c), T(2) * c)));
+    else if constexpr (LIM == LIM_SUPERBEE)
+      m = tmax(T(0), tmax(tmin(b, T(2) * c), tmin(T(2) * b, c)));
+    else
+      m = tmax(T(0), tmin(c, b));
+    const T dphi = d < T(0) ? -m : m;
+    return d * theta + dphi * (eps - theta);
+  } else {
+    const T phi = limiter<LIM, T>(lim, tdiv(num, d));
+    return d * (theta + phi * (eps - theta));
+  }
+}
+
 // ------------------------------------------------------------------ Roe
 template <int DIM, typename T, int LIM>
 struct RoePolicy : StageIsState<DIM + 2, T> {
@@ -106,11 +135,8 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
       const bool up = l >= T(0);
       const T num = up ? dqlo[k] : dqhi[k];
       const T theta = up ? T(1) : T(-1);
-      const T r = tdiv(num, a.dq[k]);
-      const T phi = limiter<LIM, T>(lim, r);
       // l qavg - 1/2 l dq (theta + phi (eps - theta)), eps = l dt/dx
-      const T x = theta + phi * (l * dt_dx - theta);
-      ft[k] = l * (a.lq[k] - T(.5) * (a.dq[k] * x));
+      ft[k] = l * (a.lq[k] - T(.5) * limited_wave<LIM, T>(lim, num, a.dq[k], theta, l * dt_dx));
     }
     P::right(a.f, ft, F);
     P::rotate(F, side);
