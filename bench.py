@@ -48,6 +48,9 @@ def parse():
                     help="skip the HBM traffic measurement (rocprofv3 FETCH_SIZE / WRITE_SIZE child runs)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (Sod 1D, 3D HLLC blast, SRHD quadrants)")
+    ap.add_argument("--only-config", choices=["C1", "C3", "C4"], default=None,
+                    help="run only that configuration leg (Sod 1D, 3D HLLC blast, SRHD quadrants) and print "
+                         "its JSON (for profiling one configuration)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the HBM "
                          "byte counters of a profiler run against a known transfer), then exit")
@@ -285,7 +288,7 @@ def traffic_leg(kahan):
             "launches": len(fs)}
 
 
-def config_legs(steps, warmup, with_cpu, cpu_seconds):
+def config_legs(steps, warmup, with_cpu, cpu_seconds, only=None):
     """The other configurations: Sod 1D (FP64, to t = 0.2, L1 vs the exact
     solution), the 3D HLLC blast (256^3, TVD-RK3, mixed with Kahan vs FP64)
     and the SRHD quadrants (1024^2, RK2, FP32 fluxes + FP64 con2prim vs FP64).
@@ -294,6 +297,8 @@ def config_legs(steps, warmup, with_cpu, cpu_seconds):
     from hydrodynamicsgpu_amd.riemann import sample
 
     res = {}
+    if only not in (None, "C1"):
+        return _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only)
     # C1: Sod, 1024 cells incl. ghosts on [-1, 1], TVD-RK2, Superbee, CFL 0.5, t = 0.2
     n = 1024
     kw = dict(xmin=[-1], xmax=[1], integratorName="RungeKutta2TVD", slopeLimiter="Superbee", cfl=0.5, t_end=0.2)
@@ -331,8 +336,18 @@ def config_legs(steps, warmup, with_cpu, cpu_seconds):
         c1["cpu_value"] = (n - 4) * ocount / c1["cpu_seconds_to_t"]
         c1["cpu_cores"] = 1
     res["C1"] = c1
+    if only == "C1":
+        return res
+    return _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, None)
+
+
+def _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only):
+    """C3 and C4, each timed in FP64 and FP32-mixed (`only`: one of them)."""
+    from hydrodynamicsgpu_amd import config, ics
 
     def pair(name, workload, make, ic, N, S, b_extra=0, mixed_flags=0):
+        if only not in (None, name):
+            return
         d64 = make("fp64", 0)
         q0 = ic(d64)
         cells = int(np.prod([sz - 4 for sz in list(d64.size)[:d64.dim]]))
@@ -413,6 +428,9 @@ def main():
     a = parse()
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.only_config:
+        print(json.dumps(config_legs(a.steps, a.warmup, not a.no_cpu, min(a.cpu_seconds, 10.0), a.only_config)))
         return
     if a.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
