@@ -165,6 +165,76 @@ HYDRO_DEV void finalize_dt(const FinalizeArgs& f, double m, bool anynan) {
   *f.steps = *f.steps + 1;
 }
 
+// The per-cell CFL pieces of one solver: frozen data of a cell, its packed
+// form for lane shifts, the (min, max) wave speed of a face, and one axis'
+// rate of a cell -- the reciprocal of its dt -- from the left face's maximal
+// and the right face's minimal speed.  Roe (res/Roe.cl:4-65) and SRHD Roe
+// (SOLVER 3, res/SRHDRoe.cl:76-118): (|max(0, lmax) - min(0, lmin)| + 1e-9)
+// / dx; HLL/HLLC: (max(0, sR) - min(0, sL)) / dx with the never-written
+// right face of the last cell read as 0 (res/EulerHLL.cl:142-175).  The
+// reference's min over cells of dx / (...) is 1 / max of these rates; the
+// march takes one reciprocal per wave instead of one division per cell.
+template <typename S, int DIM, int SOLVER>
+struct DtSolver {
+  using Roe = EulerRoe<DIM, S>;
+  using HLL = HLLPolicy<DIM, S, SOLVER == 2 ? 2 : 1>;
+  using SR = SRHDRoePolicy<DIM, S, 0>;
+  using Frozen = typename std::conditional<
+      SOLVER == 0, typename Roe::Cell,
+      typename std::conditional<SOLVER == 3, typename SR::Frozen, typename HLL::Frozen>::type>::type;
+  static constexpr bool kRoeForm = SOLVER == 0 || SOLVER == 3;
+  static constexpr int N = DIM + 2;
+  static constexpr int NF = SOLVER == 0 ? RoePolicy<DIM, S, 0>::NF : (SOLVER == 3 ? SR::NF : HLL::NF);
+
+  static HYDRO_DEV Frozen frozen(const S q[N], S gamma) {
+    if constexpr (SOLVER == 0) return Roe::template cell<S>(q, gamma);
+    else if constexpr (SOLVER == 3) return SR::template frozen<S>(q, gamma);
+    else return HLL::template frozen<S>(q, gamma);
+  }
+  static HYDRO_DEV void pack(const Frozen& f, S o[NF]) {
+    if constexpr (SOLVER == 0) RoePolicy<DIM, S, 0>::pack(f, o);
+    else if constexpr (SOLVER == 3) SR::pack(f, o);
+    else HLL::pack(f, o);
+  }
+  static HYDRO_DEV Frozen unpack(const S o[NF]) {
+    if constexpr (SOLVER == 0) return RoePolicy<DIM, S, 0>::unpack(o);
+    else if constexpr (SOLVER == 3) return SR::unpack(o);
+    else return HLL::unpack(o);
+  }
+  static HYDRO_DEV void speeds(const Frozen& L, const Frozen& R, int side, S gamma, S& lmin, S& lmax) {
+    if constexpr (SOLVER == 0) {
+      const auto f = Roe::face(L, R, side, gamma);
+      lmin = f.v[0] - f.c;
+      lmax = f.v[0] + f.c;
+    } else if constexpr (SOLVER == 3) {
+      const auto m = SR::mean(L, R, side, gamma);
+      lmin = m.lmin;
+      lmax = m.lmax;
+    } else {
+      HLL::speeds(L, R, side, gamma, lmin, lmax);
+    }
+  }
+  static HYDRO_DEV double side_rate(S lmaxL, S lminR, bool right_ok, double inv_dx, bool& bad) {
+    bad |= !(lmaxL == lmaxL) || (right_ok && !(lminR == lminR));
+    if constexpr (kRoeForm) {
+      const double lmax = fmax(0.0, double(lmaxL));
+      const double lmin = fmin(0.0, double(lminR));
+      return (fabs(lmax - lmin) + 1e-9) * inv_dx;
+    } else {
+      const double maxL = fmax(0.0, double(lmaxL));
+      const double minL = fmin(0.0, right_ok ? double(lminR) : 0.0);
+      return (maxL - minL) * inv_dx;
+    }
+  }
+};
+
+// A wave's largest rate -> its smallest dt (NaN when a speed was not finite)
+HYDRO_DEV double wave_dt(double rate, bool bad) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) rate = fmax(rate, __shfl_xor(rate, o, 64));
+  return __any(bad) ? NAN : 1.0 / rate;
+}
+
 // ---------------------------------------------------------------- CFL, 1D/2D
 // The CFL of the Euler Roe / HLL / HLLC solvers for 1D and 2D grids as a
 // strip march (the decomposition of the fused stage kernels): a wave owns 64
@@ -185,16 +255,9 @@ HYDRO_DEV void finalize_dt(const FinalizeArgs& f, double m, bool anynan) {
 // (res/SRHDRoe.cl:76-118), with the Roe form of the cell dt.
 template <typename S, int DIM, int SOLVER>
 HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int chunk) {
-  using Roe = EulerRoe<DIM, S>;
-  using HLL = HLLPolicy<DIM, S, SOLVER == 2 ? 2 : 1>;
-  using SR = SRHDRoePolicy<DIM, S, 0>;
-  using Frozen = typename std::conditional<
-      SOLVER == 0, typename Roe::Cell,
-      typename std::conditional<SOLVER == 3, typename SR::Frozen, typename HLL::Frozen>::type>::type;
-  constexpr bool kRoeForm = SOLVER == 0 || SOLVER == 3;
-  constexpr int N = DIM + 2;
-  constexpr int NF = SOLVER == 0 ? RoePolicy<DIM, S, 0>::NF : (SOLVER == 3 ? SR::NF : HLL::NF);
-
+  using D = DtSolver<S, DIM, SOLVER>;
+  using Frozen = typename D::Frozen;
+  constexpr int N = DIM + 2, NF = D::NF;
   const int lane = threadIdx.x & 63;
   const int sx = a.sx, sy = a.sy;
   const int c = strip * 60 + lane;
@@ -202,74 +265,33 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int
   const bool out_col = lane >= 2 && lane < 62 && c <= a.last_x;
   const bool right_ok_x = c + 1 <= sx - 2;
   const S gamma = S(a.gamma);
+  const double idx0 = 1.0 / a.kdx[0];
+  bool bad = false;
 
   auto fetch = [&](int row, S q[N]) {
     const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;
 #pragma unroll
     for (int s = 0; s < N; ++s) q[s] = ldp(u0, a.plane, s, e, cl);
   };
-  auto frozen = [&](const S q[N]) {
-    if constexpr (SOLVER == 0) return Roe::template cell<S>(q, gamma);
-    else if constexpr (SOLVER == 3) return SR::template frozen<S>(q, gamma);
-    else return HLL::template frozen<S>(q, gamma);
-  };
-  auto pack = [&](const Frozen& f, S o[NF]) {
-    if constexpr (SOLVER == 0) RoePolicy<DIM, S, 0>::pack(f, o);
-    else if constexpr (SOLVER == 3) SR::pack(f, o);
-    else HLL::pack(f, o);
-  };
-  auto unpack = [&](const S o[NF]) {
-    if constexpr (SOLVER == 0) return RoePolicy<DIM, S, 0>::unpack(o);
-    else if constexpr (SOLVER == 3) return SR::unpack(o);
-    else return HLL::unpack(o);
-  };
-  // (min, max) wave speed of the face between L and R, normal `side`
-  auto speeds = [&](const Frozen& L, const Frozen& R, int side, S& lmin, S& lmax) {
-    if constexpr (SOLVER == 0) {
-      const auto f = Roe::face(L, R, side, gamma);
-      lmin = f.v[0] - f.c;
-      lmax = f.v[0] + f.c;
-    } else if constexpr (SOLVER == 3) {
-      const auto m = SR::mean(L, R, side, gamma);
-      lmin = m.lmin;
-      lmax = m.lmax;
-    } else {
-      HLL::speeds(L, R, side, gamma, lmin, lmax);
-    }
-  };
-  bool bad = false;
-  // dt of one cell along one side from its left face's max speed and its
-  // right face's min speed
-  auto side_dt = [&](S lmaxL, S lminR, bool right_ok, double kdx) {
-    bad |= !(lmaxL == lmaxL) || (right_ok && !(lminR == lminR));
-    if constexpr (kRoeForm) {
-      const double lmax = fmax(0.0, double(lmaxL));
-      const double lmin = fmin(0.0, double(lminR));
-      return kdx / (fabs(lmax - lmin) + 1e-9);
-    } else {
-      const double maxL = fmax(0.0, double(lmaxL));
-      const double minL = fmin(0.0, right_ok ? double(lminR) : 0.0);
-      return kdx / (maxL - minL);
-    }
-  };
-  auto x_dt = [&](const Frozen& f) {
+  auto x_rate = [&](const Frozen& f) {
     S pk[NF], pkL[NF];
-    pack(f, pk);
+    D::pack(f, pk);
 #pragma unroll
     for (int i = 0; i < NF; ++i) pkL[i] = lane_from_left(pk[i]);
     S lmin, lmax;
-    speeds(unpack(pkL), f, 0, lmin, lmax);  // face c: cells c-1, c
-    const S lminR = lane_from_right(lmin);  // face c+1
-    return side_dt(lmax, lminR, right_ok_x, a.kdx[0]);
+    D::speeds(D::unpack(pkL), f, 0, gamma, lmin, lmax);  // face c: cells c-1, c
+    const S lminR = lane_from_right(lmin);                // face c+1
+    return D::side_rate(lmax, lminR, right_ok_x, idx0, bad);
   };
 
-  double m = INFINITY;
+  double m = 0.0;  // largest rate of the wave's cells
   if constexpr (DIM == 1) {
     S q[N];
     fetch(0, q);
-    const double d = x_dt(frozen(q));
+    const double d = x_rate(D::frozen(q, gamma));
     if (out_col) m = d;
   } else {
+    const double idx1 = 1.0 / a.kdx[1];
     // a chunk is at most kDtRows rows: all its rows (plus one below and one
     // above) are loaded up front, so the wave waits for memory once
     const int y0 = 2 + chunk * a.rows;
@@ -278,27 +300,111 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int
     S q[kDtRows + 2][N];
 #pragma unroll
     for (int i = 0; i < kDtRows + 2; ++i) fetch(y0 - 1 + i <= y1 ? y0 - 1 + i : y1, q[i]);
-    Frozen cur = frozen(q[1]);
+    Frozen cur = D::frozen(q[1], gamma);
     S lminY, lmaxY_prev;
-    speeds(frozen(q[0]), cur, 1, lminY, lmaxY_prev);  // face y0: bottom face of row y0
-    double xd = x_dt(cur);
+    D::speeds(D::frozen(q[0], gamma), cur, 1, gamma, lminY, lmaxY_prev);  // face y0: bottom face of row y0
+    double xd = x_rate(cur);
 #pragma unroll
     for (int i = 2; i < kDtRows + 2; ++i) {
       const int r = y0 - 1 + i;  // face r: top face of row r-1
       if (r > y1) break;
       const Frozen prev = cur;
-      cur = frozen(q[i]);
+      cur = D::frozen(q[i], gamma);
       S lmaxY;
-      speeds(prev, cur, 1, lminY, lmaxY);
-      const double yd = side_dt(lmaxY_prev, lminY, r <= sy - 2, a.kdx[1]);
-      if (out_col) m = fmin(m, fmin(xd, yd));
-      if (r < y1) xd = x_dt(cur);
+      D::speeds(prev, cur, 1, gamma, lminY, lmaxY);
+      const double yd = D::side_rate(lmaxY_prev, lminY, r <= sy - 2, idx1, bad);
+      if (out_col) m = fmax(m, fmax(xd, yd));
+      if (r < y1) xd = x_rate(cur);
       lmaxY_prev = lmaxY;
     }
   }
+  return wave_dt(m, bad && out_col);
+}
+
+// ---------------------------------------------------------------- CFL, 3D
+// The CFL of the Euler solvers on 3D grids as a strip march: a wave owns 64
+// columns of one z-plane and a chunk of rows, reads for every row the row
+// itself and the rows of the planes below and above (the waves of those
+// planes read them too: L2 hits), takes the x-faces through DPP shifts, the
+// y-face against the previous row held in registers and both z-faces, and
+// keeps its minimum.  Cells 2 .. last_x / last_y / last_z of every axis.
+template <typename S, int SOLVER>
+HYDRO_DEV double dt_march3_wave(const DtMarchArgs& a, const S* u0, int strip, int chunk, int z) {
+  using D = DtSolver<S, 3, SOLVER>;
+  using Frozen = typename D::Frozen;
+  constexpr int N = 5, NF = D::NF;
+  const int lane = threadIdx.x & 63;
+  const int sx = a.sx, sy = a.sy, sz = a.sz;
+  const int c = strip * 60 + lane;
+  const uint32_t cl = c < sx ? c : sx - 1;
+  const bool out_col = lane >= 2 && lane < 62 && c <= a.last_x;
+  const bool right_ok_x = c + 1 <= sx - 2;
+  const bool right_ok_z = z + 1 <= sz - 2;
+  const S gamma = S(a.gamma);
+  const long long psz = (long long)sx * sy;
+  bool bad = false;
+
+  auto fetch = [&](int plane, int row, S q[N]) {
+    const long long e = (long long)__builtin_amdgcn_readfirstlane(plane) * psz +
+                        (long long)__builtin_amdgcn_readfirstlane(row) * sx;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
-  return __any(bad && out_col) ? NAN : m;
+    for (int s = 0; s < N; ++s) q[s] = ldp(u0, a.plane, s, e, cl);
+  };
+  struct Rows {
+    S m[N], c[N], p[N];  // planes z-1, z, z+1
+  };
+  auto fetch3 = [&](int row, Rows& r) {
+    fetch(z - 1, row, r.m);
+    fetch(z, row, r.c);
+    fetch(z + 1, row, r.p);
+  };
+  const double idx[3] = {1.0 / a.kdx[0], 1.0 / a.kdx[1], 1.0 / a.kdx[2]};
+  auto x_rate = [&](const Frozen& f) {
+    S pk[NF], pkL[NF];
+    D::pack(f, pk);
+#pragma unroll
+    for (int i = 0; i < NF; ++i) pkL[i] = lane_from_left(pk[i]);
+    S lmin, lmax;
+    D::speeds(D::unpack(pkL), f, 0, gamma, lmin, lmax);  // face c: cells c-1, c
+    const S lminR = lane_from_right(lmin);                // face c+1
+    return D::side_rate(lmax, lminR, right_ok_x, idx[0], bad);
+  };
+  auto z_rate = [&](const Rows& r, const Frozen& fc) {
+    S lmin0, lmax0, lmin1, lmax1;
+    D::speeds(D::frozen(r.m, gamma), fc, 2, gamma, lmin0, lmax0);  // lower face
+    D::speeds(fc, D::frozen(r.p, gamma), 2, gamma, lmin1, lmax1);  // upper face
+    return D::side_rate(lmax0, lmin1, right_ok_z, idx[2], bad);
+  };
+
+  const int y0 = 2 + chunk * a.rows;
+  int y1 = y0 + a.rows;
+  if (y1 > a.last_y + 1) y1 = a.last_y + 1;
+  double m = 0.0;  // largest rate of the wave's cells
+  // step r: face r (rows r-1 | r) closes row r-1, whose x and z terms the
+  // previous step left; the loads of row r+1 are issued before row r's work
+  S q[N];
+  fetch(z, y0 - 1, q);
+  Rows nx;
+  fetch3(y0, nx);
+  Frozen prev = D::frozen(q, gamma);
+  S lmaxY_prev = S(0);
+  double xz_prev = 0.0;
+#pragma unroll 1
+  for (int r = y0; r <= y1; ++r) {
+    const Rows cur = nx;
+    if (r < y1) fetch3(r + 1, nx);  // r + 1 <= y1 <= sy - 1
+    const Frozen fc = D::frozen(cur.c, gamma);
+    S lminY, lmaxY;
+    D::speeds(prev, fc, 1, gamma, lminY, lmaxY);  // face r
+    if (r > y0) {
+      const double yd = D::side_rate(lmaxY_prev, lminY, r <= sy - 2, idx[1], bad);
+      if (out_col) m = fmax(m, fmax(xz_prev, yd));
+    }
+    if (r < y1) xz_prev = fmax(x_rate(fc), z_rate(cur, fc));
+    lmaxY_prev = lmaxY;
+    prev = fc;
+  }
+  return wave_dt(m, bad && out_col);
 }
 
 }  // namespace hydro

@@ -236,6 +236,28 @@ static void dt_march(const DtMarchArgs& a, const S* u0, hipStream_t st) {
 void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st) { dt_march<double>(a, u0, st); }
 void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st) { dt_march<float>(a, u0, st); }
 
+// wave -> (strip, row chunk, plane), planes slowest
+template <typename S, int SOLVER>
+__global__ void __launch_bounds__(256) dt_march3_kernel(DtMarchArgs a, const S* u0) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (wave >= a.nwaves) return;
+  const int per_plane = a.nstrips * a.nchunks;
+  const int z = 2 + wave / per_plane;
+  const int rest = wave % per_plane;
+  const double m = dt_march3_wave<S, SOLVER>(a, u0, rest % a.nstrips, rest / a.nstrips, z);
+  if ((threadIdx.x & 63) == 0) a.wavemin[wave] = m;
+}
+
+template <typename S>
+static void dt_march3(const DtMarchArgs& a, const S* u0, hipStream_t st) {
+  dim3 g((a.nwaves + 3) / 4), b(256);
+  if (a.solver == 0) hipLaunchKernelGGL((dt_march3_kernel<S, 0>), g, b, 0, st, a, u0);
+  else if (a.solver == 1) hipLaunchKernelGGL((dt_march3_kernel<S, 1>), g, b, 0, st, a, u0);
+  else hipLaunchKernelGGL((dt_march3_kernel<S, 2>), g, b, 0, st, a, u0);
+}
+void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st) { dt_march3<double>(a, u0, st); }
+void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st) { dt_march3<float>(a, u0, st); }
+
 // ---------------------------------------------------------------- totals
 template <typename S>
 __global__ void __launch_bounds__(256) totals_kernel(int dim, int sx, int sy, int sz, long long plane, int n,

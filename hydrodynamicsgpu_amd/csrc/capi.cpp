@@ -211,6 +211,37 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
     HIP_TRY(hipGetLastError());
     return HYDRO_OK;
   }
+  const long long strips3 = (s->sz[0] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 2) / 60 + 1;
+  const long long planes3 = s->sz[2] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 1;
+  if (!s->d.use_fixed_dt && s->dim == 3 && s->d.equation == HYDRO_EQN_EULER && strips3 * planes3 <= kDtMaxWaves) {
+    // 3D strip march: waves = strips x row chunks x planes
+    DtMarchArgs a{};
+    a.dim = 3;
+    a.sx = s->sz[0];
+    a.sy = s->sz[1];
+    a.sz = s->sz[2];
+    a.plane = s->plane;
+    for (int i = 0; i < 3; ++i) a.kdx[i] = s->kdx[i];
+    a.gamma = s->d.gamma;
+    a.solver = s->d.solver;
+    const int hi = s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
+    a.last_x = s->sz[0] - hi;
+    a.last_y = s->sz[1] - hi;
+    a.last_z = s->sz[2] - hi;
+    a.nstrips = (a.last_x - 2) / 60 + 1;
+    const long long ny = a.last_y - 1, nz = a.last_z - 1;
+    a.rows = 8;
+    while ((long long)a.nstrips * ((ny + a.rows - 1) / a.rows) * nz > kDtMaxWaves && a.rows < ny) a.rows *= 2;
+    a.nchunks = (int)((ny + a.rows - 1) / a.rows);
+    a.nwaves = (int)((long long)a.nstrips * a.nchunks * nz);
+    a.wavemin = s->blockmin;
+    if (s->mixed) launch_dt_march3_f32(a, (const float*)s->U[s->cur], s->stream);
+    else launch_dt_march3_f64(a, (const double*)s->U[s->cur], s->stream);
+    f.nblocks = a.nwaves;
+    launch_finalize_dt(f, s->stream);
+    HIP_TRY(hipGetLastError());
+    return HYDRO_OK;
+  }
   if (!s->d.use_fixed_dt) {
     DtArgs a{};
     a.dim = s->dim;

@@ -99,17 +99,20 @@ void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st);
 // launch_finalize_dt over nblocks = nwaves
 constexpr int kDtRows = 8;  // rows per wave of the strip-march CFL kernel (at most)
 struct DtMarchArgs {
-  int dim, sx, sy;
+  int dim, sx, sy, sz;
   long long plane;
-  double kdx[2];
+  double kdx[3];
   double gamma;
   int solver;                   // 0 Roe, 1 HLL, 2 HLLC, 3 SRHD Roe (reads the FP64 primitives)
-  int last_x, last_y;           // last cell with a dt: S-3 (Roe), S-2 (HLL)
+  int last_x, last_y, last_z;   // last cell with a dt: S-3 (Roe), S-2 (HLL)
   int nstrips, rows, nchunks, nwaves;
   double* wavemin;
 };
 void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
+// 3D (Euler): waves = strips x row chunks x planes (nchunks counts row chunks)
+void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
+void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
 
 // FP64 interior sums, per block: partial[b * n + s]
 void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u,

@@ -218,7 +218,8 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
 // workgroup only supplies that face (W-1 output planes per workgroup).  One
 // barrier per row.
 template <class Pol, typename T, typename S, bool KAHAN, int MODE, int W>
-__global__ void __launch_bounds__(64 * W) stage3d_r1_kernel(StageArgs<T, S> a) {
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MODE == CM_GENERIC || sizeof(T) == 8 ? 1 : 4)))
+stage3d_r1_kernel(StageArgs<T, S> a) {
   constexpr int N = Pol::N;
   using Frozen = typename Pol::Frozen;
   using Comb = StageCombine<T, S, KAHAN, N, MODE>;
@@ -323,8 +324,12 @@ __global__ void __launch_bounds__(64 * W) stage3d_r1_kernel(StageArgs<T, S> a) {
     }
     const long long eo = (long long)__builtin_amdgcn_readfirstlane(pown) * psz +
                          (long long)__builtin_amdgcn_readfirstlane(r - 1) * sx;
+    // the final and generic combinations load more operands (replayed
+    // increments, partial sums): they are fetched just before use, which
+    // keeps them out of the registers live across the flux work
+    constexpr bool kLateFin = MODE == CM_FINAL || MODE == CM_GENERIC;
     typename Comb::Fin fin;
-    if (ow && r > y0) comb.fetch_fin_rest(eo, cl, w1.q0, w1.dk, fin);
+    if (!kLateFin && ow && r > y0) comb.fetch_fin_rest(eo, cl, w1.q0, w1.dk, fin);
     Row C;
     if (zw) build(cur, C);
     T G[N], Fz[N], xd[N];
@@ -349,6 +354,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_r1_kernel(StageArgs<T, S> a) {
         T deriv[N];
 #pragma unroll
         for (int s = 0; s < N; ++s) deriv[s] = acc[s] + (Gp[s] - G[s]) * idy;
+        if (kLateFin) comb.fetch_fin_rest(eo, cl, w1.q0, w1.dk, fin);
         comb.finish(eo, c, fin, deriv);
       }
 #pragma unroll
