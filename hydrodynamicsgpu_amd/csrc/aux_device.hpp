@@ -38,49 +38,57 @@ HYDRO_DEV int ghost_source(int method, int k, int n) {
 }
 
 // ---------------------------------------------------------------- ghosts
-// Enumerates exactly the ghost cells (some coordinate in {0,1,S-2,S-1}).
-HYDRO_DEV bool ghost_cell(int g, int dim, int sx, int sy, int sz, long long& cell) {
-  auto pick = [](int k, int n) { return k < 2 ? k : n - 4 + k; };  // k in 0..3
+// Enumerates exactly the ghost cells (some coordinate in {0,1,S-2,S-1}) as
+// coordinates; 32-bit index arithmetic (a ghost count fits in 31 bits for
+// any grid that fits in memory), so no 64-bit division is emitted.
+HYDRO_DEV bool ghost_cell(unsigned g, int dim, int sx, int sy, int sz, int& x, int& y, int& z) {
+  auto pick = [](unsigned k, int n) { return k < 2 ? (int)k : n - 4 + (int)k; };  // k in 0..3
+  const unsigned ux = (unsigned)sx, uy = (unsigned)sy, uz = (unsigned)sz;
+  y = 0;
+  z = 0;
   if (dim == 1) {
-    if (g >= 4) return false;
-    cell = pick(g, sx);
+    if (g >= 4u) return false;
+    x = pick(g, sx);
     return true;
   }
   if (dim == 2) {
-    const int rows = 4 * sx;
+    const unsigned rows = 4u * ux;
     if (g < rows) {
-      const int y = pick(g / sx, sy), x = g % sx;
-      cell = (long long)y * sx + x;
+      y = pick(g / ux, sy);
+      x = (int)(g % ux);
       return true;
     }
     g -= rows;
-    if (g >= 4 * (sy - 4)) return false;
-    const int y = 2 + g / 4, x = pick(g % 4, sx);
-    cell = (long long)y * sx + x;
+    if (g >= 4u * (uy - 4u)) return false;
+    y = 2 + (int)(g >> 2);
+    x = pick(g & 3u, sx);
     return true;
   }
-  const long long planes = 4LL * sx * sy;
-  if (g < planes) {
-    const int z = pick((int)(g / ((long long)sx * sy)), sz);
-    const long long r = g % ((long long)sx * sy);
-    cell = (long long)z * sx * sy + r;
+  const unsigned plane = ux * uy;
+  if (g < 4u * plane) {
+    const unsigned k = g / plane, r = g - k * plane;
+    z = pick(k, sz);
+    y = (int)(r / ux);
+    x = (int)(r - (unsigned)y * ux);
     return true;
   }
-  long long h = g - planes;
-  const long long rows = 4LL * sx * (sz - 4);
-  if (h < rows) {
-    const int z = 2 + (int)(h / (4LL * sx));
-    const long long r = h % (4LL * sx);
-    const int y = pick((int)(r / sx), sy), x = (int)(r % sx);
-    cell = ((long long)z * sy + y) * sx + x;
+  unsigned h = g - 4u * plane;
+  const unsigned rows = 4u * ux;
+  if (h < rows * (uz - 4u)) {
+    const unsigned zz = h / rows, r = h - zz * rows;
+    z = 2 + (int)zz;
+    const unsigned yy = r / ux;
+    y = pick(yy, sy);
+    x = (int)(r - yy * ux);
     return true;
   }
-  h -= rows;
-  if (h >= 4LL * (sy - 4) * (sz - 4)) return false;
-  const int z = 2 + (int)(h / (4LL * (sy - 4)));
-  const long long r = h % (4LL * (sy - 4));
-  const int y = 2 + (int)(r / 4), x = pick((int)(r % 4), sx);
-  cell = ((long long)z * sy + y) * sx + x;
+  h -= rows * (uz - 4u);
+  const unsigned cols = 4u * (uy - 4u);
+  if (h >= cols * (uz - 4u)) return false;
+  const unsigned zz = h / cols, r = h - zz * cols;
+  z = 2 + (int)zz;
+  y = 2 + (int)(r >> 2);
+  x = pick(r & 3u, sx);
   return true;
 }
 
@@ -93,12 +101,9 @@ HYDRO_DEV long long ghost_count(int dim, int sx, int sy, int sz) {
 // ghost cell number g of a grid (g < ghost_count)
 template <typename S>
 HYDRO_DEV void boundary_cell(const BoundaryArgs& b, S* planes, float* lo, long long g) {
-  long long cell;
-  if (!ghost_cell((int)g, b.dim, b.sx, b.sy, b.sz, cell)) return;
-  const int x = (int)(cell % b.sx);
-  const long long yz = cell / b.sx;
-  const int y = (int)(yz % b.sy);
-  const int z = (int)(yz / b.sy);
+  int x, y, z;
+  if (!ghost_cell((unsigned)g, b.dim, b.sx, b.sy, b.sz, x, y, z)) return;
+  const long long cell = x + (long long)b.sx * (y + (long long)b.sy * z);
   int sc[3] = {x, y, z};
   const int n[3] = {b.sx, b.sy, b.sz};
   // one source cell for every state (mirror and reflect share the index

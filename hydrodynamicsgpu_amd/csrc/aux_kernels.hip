@@ -22,9 +22,10 @@ template <typename S>
 __global__ void copy_ghosts_kernel(int dim, int sx, int sy, int sz, long long plane, int n, const S* src,
                                    S* dst) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long cell;
   if (g >= ghost_count(dim, sx, sy, sz)) return;
-  if (!ghost_cell((int)g, dim, sx, sy, sz, cell)) return;
+  int x, y, z;
+  if (!ghost_cell((unsigned)g, dim, sx, sy, sz, x, y, z)) return;
+  const long long cell = x + (long long)sx * (y + (long long)sy * z);
   for (int s = 0; s < n; ++s) dst[s * plane + cell] = src[s * plane + cell];
 }
 

@@ -218,8 +218,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
 // workgroup only supplies that face (W-1 output planes per workgroup).  One
 // barrier per row.
 template <class Pol, typename T, typename S, bool KAHAN, int MODE, int W>
-__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MODE == CM_GENERIC || sizeof(T) == 8 ? 1 : 4)))
-stage3d_r1_kernel(StageArgs<T, S> a) {
+__global__ void __launch_bounds__(64 * W) stage3d_r1_kernel(StageArgs<T, S> a) {
   constexpr int N = Pol::N;
   using Frozen = typename Pol::Frozen;
   using Comb = StageCombine<T, S, KAHAN, N, MODE>;

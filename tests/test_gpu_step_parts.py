@@ -32,8 +32,10 @@ def _rand_state(desc, seed=0, vscale=1.0):
 
 
 # sizes straddling the 60-column strips (64 = 60 interior columns exactly,
-# 65/124/125 leave one column over) and uneven row chunks
-SIZES = [[64], [65], [124], [300], [64, 37], [125, 64], [124, 130], [61, 300]]
+# 65/124/125 leave one column over), uneven row chunks, and 3D grids (the 3D
+# march: planes x row chunks)
+SIZES = [[64], [65], [124], [300], [64, 37], [125, 64], [124, 130], [61, 300], [24, 20, 18], [70, 33, 21],
+         [130, 9, 40]]
 
 
 @pytest.mark.parametrize("solver", ["Roe", "HLL", "HLLC"])
