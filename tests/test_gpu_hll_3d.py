@@ -134,24 +134,28 @@ def test_blast_3d_full_size_properties(gpu_lib):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
 def test_3d_first_order_decomposition_invariant(gpu_lib, precision, monkeypatch):
-    # the y-chunk height changes which waves compute the chunk-edge faces,
-    # not their arithmetic: the results are bit-identical
+    # the y-chunk height and the workgroup shape change which waves compute
+    # the chunk- and block-edge faces, not their arithmetic: the results are
+    # bit-identical
     from hydrodynamicsgpu_amd import capi
 
     d = config.make_desc([70, 45, 25], solverName="EulerHLLC", integratorName="RungeKutta3TVD",
                          boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4, precision=precision)
     q = _rand_state(d, 5)
     out = []
-    for rows in (None, "8", "41"):
-        if rows is None:
-            monkeypatch.delenv("HYDRO_ROWS_PER_WAVE", raising=False)
-        else:
-            monkeypatch.setenv("HYDRO_ROWS_PER_WAVE", rows)
+    # (rows per chunk, waves per workgroup)
+    for rows, zw in ((None, None), ("8", None), ("41", None), (None, "8"), ("13", "8")):
+        for k, v in (("HYDRO_ROWS_PER_WAVE", rows), ("HYDRO_3D_ZWAVES", zw)):
+            if v is None:
+                monkeypatch.delenv(k, raising=False)
+            else:
+                monkeypatch.setenv(k, v)
         g = capi.Sim(d)
         g.set_state(q)
         g.run(3)
         out.append(g.get_state())
         g.close()
-    monkeypatch.delenv("HYDRO_ROWS_PER_WAVE", raising=False)
-    np.testing.assert_array_equal(out[1], out[0])
-    np.testing.assert_array_equal(out[2], out[0])
+    for k in ("HYDRO_ROWS_PER_WAVE", "HYDRO_3D_ZWAVES"):
+        monkeypatch.delenv(k, raising=False)
+    for o in out[1:]:
+        np.testing.assert_array_equal(o, out[0])
