@@ -258,8 +258,10 @@ HYDRO_DEV double wave_dt(double rate, bool bad) {
 // SOLVER 3 is SRHD Roe: `u0` is then the FP64 primitive planes and the
 // speeds are the Marti-Mueller eigenvalues of the mean primitives
 // (res/SRHDRoe.cl:76-118), with the Roe form of the cell dt.
-template <typename S, int DIM, int SOLVER>
-HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int chunk) {
+// L: the element type in memory (the FP32-mixed SRHD march reads the FP64
+// primitives and computes its speeds in FP32).
+template <typename S, int DIM, int SOLVER, typename L = S>
+HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const L* u0, int strip, int chunk) {
   using D = DtSolver<S, DIM, SOLVER>;
   using Frozen = typename D::Frozen;
   constexpr int N = DIM + 2, NF = D::NF;
@@ -276,7 +278,7 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const S* u0, int strip, int
   auto fetch = [&](int row, S q[N]) {
     const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;
 #pragma unroll
-    for (int s = 0; s < N; ++s) q[s] = ldp(u0, a.plane, s, e, cl);
+    for (int s = 0; s < N; ++s) q[s] = S(ldp(u0, a.plane, s, e, cl));
   };
   auto x_rate = [&](const Frozen& f) {
     S pk[NF], pkL[NF];

@@ -208,11 +208,11 @@ void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st) {
   hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, st, f);
 }
 
-template <typename S, int DIM, int SOLVER>
-__global__ void __launch_bounds__(256) dt_march_kernel(DtMarchArgs a, const S* u0) {
+template <typename S, int DIM, int SOLVER, typename L = S>
+__global__ void __launch_bounds__(256) dt_march_kernel(DtMarchArgs a, const L* u0) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (wave >= a.nwaves) return;
-  const double m = dt_march_wave<S, DIM, SOLVER>(a, u0, wave % a.nstrips, wave / a.nstrips);
+  const double m = dt_march_wave<S, DIM, SOLVER, L>(a, u0, wave % a.nstrips, wave / a.nstrips);
   if ((threadIdx.x & 63) == 0) a.wavemin[wave] = m;
 }
 
@@ -220,22 +220,28 @@ template <typename S>
 static void dt_march(const DtMarchArgs& a, const S* u0, hipStream_t st) {
   const int blocks = (a.nwaves + 3) / 4;
   dim3 g(blocks), b(256);
-#define L(D, SV) hipLaunchKernelGGL((dt_march_kernel<S, D, SV>), g, b, 0, st, a, u0)
-  if constexpr (sizeof(S) == 8) {
-    if (a.solver == 3) {  // SRHD: u0 is the FP64 primitive planes
-      if (a.dim == 1) L(1, 3); else L(2, 3);
-      return;
-    }
-  }
+#define LAUNCH(D, SV) hipLaunchKernelGGL((dt_march_kernel<S, D, SV>), g, b, 0, st, a, u0)
   if (a.dim == 1) {
-    if (a.solver == 0) L(1, 0); else if (a.solver == 1) L(1, 1); else L(1, 2);
+    if (a.solver == 0) LAUNCH(1, 0); else if (a.solver == 1) LAUNCH(1, 1); else LAUNCH(1, 2);
   } else {
-    if (a.solver == 0) L(2, 0); else if (a.solver == 1) L(2, 1); else L(2, 2);
+    if (a.solver == 0) LAUNCH(2, 0); else if (a.solver == 1) LAUNCH(2, 1); else LAUNCH(2, 2);
   }
-#undef L
+#undef LAUNCH
 }
 void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st) { dt_march<double>(a, u0, st); }
 void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st) { dt_march<float>(a, u0, st); }
+
+// SRHD: the FP64 primitive planes, speeds in FP64 or (FP32-mixed) FP32
+void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, hipStream_t st) {
+  dim3 g((a.nwaves + 3) / 4), b(256);
+  if (fp32) {
+    if (a.dim == 1) hipLaunchKernelGGL((dt_march_kernel<float, 1, 3, double>), g, b, 0, st, a, prim);
+    else hipLaunchKernelGGL((dt_march_kernel<float, 2, 3, double>), g, b, 0, st, a, prim);
+  } else {
+    if (a.dim == 1) hipLaunchKernelGGL((dt_march_kernel<double, 1, 3, double>), g, b, 0, st, a, prim);
+    else hipLaunchKernelGGL((dt_march_kernel<double, 2, 3, double>), g, b, 0, st, a, prim);
+  }
+}
 
 // wave -> (strip, row chunk, plane), planes slowest
 template <typename S, int SOLVER>
@@ -308,57 +314,54 @@ namespace hydro {
 // SRHDRoe::step tail (src/Solver/SRHDRoe.cpp:62-73): constrainState on every
 // cell (res/SRHDCommon.cl:94-109), then updatePrimitives on cells 2..S-2 of
 // every axis (res/SRHDCommon.cl:113-196), in FP64.
+// One thread per cell: x from the block's 256-column slice, y and z from the
+// grid (no index division).
 template <typename S, int DIM>
 __global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, float* lo, double* prim) {
   constexpr int N = DIM + 2, E = DIM + 1;
-  const long long vol = (long long)a.sx * a.sy * a.sz;
+  const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (x >= a.sx) return;
+  const int y = (int)blockIdx.y, z = (int)blockIdx.z;
+  const long long cell = x + (long long)a.sx * (y + (long long)a.sy * z);
   const SRHDDefs defs{a.gamma, a.max_iter, a.stop_eps, a.vel_eps, a.pmin_eps, a.rho_min, a.rho_max,
                       a.eint_max, a.D_min, a.D_max, a.tau_min, a.tau_max};
-  bool fail = false;
-  for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < vol;
-       cell += (long long)gridDim.x * blockDim.x) {
-    double q[N];
+  double q[N];
 #pragma unroll
-    for (int s = 0; s < N; ++s) {
-      q[s] = double(u[s * a.plane + cell]);
-      if (lo) q[s] += double(lo[s * a.plane + cell]);
-    }
-    double D = fmin(fmax(q[0], a.D_min), a.D_max);
-    double tau = fmin(fmax(q[E], a.tau_min), a.tau_max);
-    if (D != q[0]) {
-      q[0] = D;
-      u[cell] = S(D);
-      if (lo) lo[cell] = float(D - double(S(D)));
-    }
-    if (tau != q[E]) {
-      q[E] = tau;
-      u[E * a.plane + cell] = S(tau);
-      if (lo) lo[E * a.plane + cell] = float(tau - double(S(tau)));
-    }
-    const int x = (int)(cell % a.sx);
-    const int y = (int)((cell / a.sx) % a.sy);
-    const int z = (int)(cell / ((long long)a.sx * a.sy));
-    bool act = x >= 2 && x <= a.sx - 2;
-    if (DIM > 1) act = act && y >= 2 && y <= a.sy - 2;
-    if (DIM > 2) act = act && z >= 2 && z <= a.sz - 2;
-    if (!act) continue;
-    double p[N];
-#pragma unroll
-    for (int s = 0; s < N; ++s) p[s] = prim[s * a.plane + cell];
-    // FP32 storage: warm-started Newton (same root, see srhd_con2prim)
-    if (srhd_con2prim<DIM, sizeof(S) == 4>(defs, q, p)) {
-#pragma unroll
-      for (int s = 0; s < N; ++s) prim[s * a.plane + cell] = p[s];
-    } else {
-      fail = true;
-    }
+  for (int s = 0; s < N; ++s) {
+    q[s] = double(u[s * a.plane + cell]);
+    if (lo) q[s] += double(lo[s * a.plane + cell]);
   }
-  if (fail) atomicOr(a.health, 4u);
+  const double D = fmin(fmax(q[0], a.D_min), a.D_max);
+  const double tau = fmin(fmax(q[E], a.tau_min), a.tau_max);
+  if (D != q[0]) {
+    q[0] = D;
+    u[cell] = S(D);
+    if (lo) lo[cell] = float(D - double(S(D)));
+  }
+  if (tau != q[E]) {
+    q[E] = tau;
+    u[E * a.plane + cell] = S(tau);
+    if (lo) lo[E * a.plane + cell] = float(tau - double(S(tau)));
+  }
+  bool act = x >= 2 && x <= a.sx - 2;
+  if (DIM > 1) act = act && y >= 2 && y <= a.sy - 2;
+  if (DIM > 2) act = act && z >= 2 && z <= a.sz - 2;
+  if (!act) return;
+  double p[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) p[s] = prim[s * a.plane + cell];
+  // FP32 storage: warm-started Newton (same root, see srhd_con2prim)
+  if (srhd_con2prim<DIM, sizeof(S) == 4>(defs, q, p)) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) prim[s * a.plane + cell] = p[s];
+  } else {
+    atomicOr(a.health, 4u);
+  }
 }
 
 template <typename S>
 static void con2prim(const Con2PrimArgs& a, S* u, float* lo, double* prim, hipStream_t st) {
-  dim3 g(2048), b(256);
+  dim3 g((a.sx + 255) / 256, a.dim > 1 ? a.sy : 1, a.dim > 2 ? a.sz : 1), b(256);
   if (a.dim == 1) hipLaunchKernelGGL((con2prim_kernel<S, 1>), g, b, 0, st, a, u, lo, prim);
   else if (a.dim == 2) hipLaunchKernelGGL((con2prim_kernel<S, 2>), g, b, 0, st, a, u, lo, prim);
   else hipLaunchKernelGGL((con2prim_kernel<S, 3>), g, b, 0, st, a, u, lo, prim);

@@ -203,7 +203,7 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
     }
     a.nwaves = a.nstrips * a.nchunks;
     a.wavemin = s->blockmin;
-    if (srhd) launch_dt_march_f64(a, s->prim, s->stream);
+    if (srhd) launch_dt_march_srhd(a, s->prim, s->mixed, s->stream);
     else if (s->mixed) launch_dt_march_f32(a, (const float*)s->U[s->cur], s->stream);
     else launch_dt_march_f64(a, (const double*)s->U[s->cur], s->stream);
     f.nblocks = a.nwaves;

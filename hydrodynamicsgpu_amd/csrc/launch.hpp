@@ -110,6 +110,8 @@ struct DtMarchArgs {
 };
 void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
+// SRHD (solver 3): reads the FP64 primitives; fp32: speeds in FP32 (FP32-mixed)
+void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, hipStream_t st);
 // 3D (Euler): waves = strips x row chunks x planes (nchunks counts row chunks)
 void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
