@@ -321,7 +321,8 @@ def config_legs(steps, warmup, with_cpu, cpu_seconds, only=None):
     sim.close()
     c1 = {"workload": "euler1d_roe_sod_1024_rk2tvd_superbee_t0.2", "precision": "fp64", "steps": nsteps,
           "gpu_value": (n - 4) * nsteps / el, "unit": "cell-updates/s", "gpu_seconds_to_t": el, "l1_rho_vs_exact": l1,
-          "note": "latency-bound: 1020 cells per step, the host waits for t each 64 steps"}
+          "note": "latency-bound: 1020 cells per step, every update in one single-workgroup launch per 64 steps, the host "
+                  "reads t between launches"}
     if with_cpu:
         import oracle
 

@@ -158,16 +158,36 @@ HYDRO_DEV double block_min(double v, double* sh) {
 
 // Solver::findMinTimestep tail (src/Solver/Solver.cpp:510-515): dt = min * cfl,
 // or the fixed dt; clipping to t_end, time/step bookkeeping, health bit 0.
-HYDRO_DEV void finalize_dt(const FinalizeArgs& f, double m, bool anynan) {
-  double dt = f.use_fixed ? f.fixed_dt : m * f.cfl;
-  const double t = *f.t;
+// With clipping, an update that starts at t >= t_end is a no-op: dt = 0 and
+// the clock, the step count and the health bits stay as they are.  Returns
+// false for such a no-op.  The clock is passed in registers here and kept in
+// device memory by finalize_dt.
+HYDRO_DEV bool finalize_step(const FinalizeArgs& f, double m, bool anynan, double& t, long long& steps,
+                             unsigned& health, double& dt) {
+  if (f.clip && !(t < f.t_end)) {
+    dt = 0.0;
+    return false;
+  }
+  dt = f.use_fixed ? f.fixed_dt : m * f.cfl;
   if (f.clip && t + dt > f.t_end) dt = f.t_end - t;
-  unsigned h = *f.health;
-  if (anynan || !(dt > 0.0) || dt == INFINITY) h |= 1u;
-  *f.health = h;
+  if (anynan || !(dt > 0.0) || dt == INFINITY) health |= 1u;
+  t = t + dt;
+  steps = steps + 1;
+  return true;
+}
+
+HYDRO_DEV bool finalize_dt(const FinalizeArgs& f, double m, bool anynan) {
+  double t = *f.t, dt;
+  long long steps = *f.steps;
+  unsigned health = *f.health;
+  const bool adv = finalize_step(f, m, anynan, t, steps, health, dt);
   *f.dt = dt;
-  *f.t = t + dt;
-  *f.steps = *f.steps + 1;
+  if (adv) {
+    *f.health = health;
+    *f.t = t;
+    *f.steps = steps;
+  }
+  return adv;
 }
 
 // The per-cell CFL pieces of one solver: frozen data of a cell, its packed

@@ -50,10 +50,10 @@ __global__ void __launch_bounds__(256) batch_stage_kernel(BatchStageArgs<T, S> b
     a.inv_dx[1] = m.inv_dx[1];
     a.dt = b.dts + (long long)task.member * kBatchDtStride;
     if (task.dim == 2) {
-      stage2d_wave<Pol2, 2, T, S, KAHAN, MODE>(a, task.a, task.b);
+      stage2d_wave<Pol2, 2, T, S, KAHAN, MODE>(a, task.a, task.b, *a.dt);
     } else {
 #pragma unroll 1
-      for (int st = task.a; st < task.a + task.b; ++st) stage2d_wave<Pol1, 1, T, S, KAHAN, MODE>(a, st, 0);
+      for (int st = task.a; st < task.a + task.b; ++st) stage2d_wave<Pol1, 1, T, S, KAHAN, MODE>(a, st, 0, *a.dt);
     }
   }
 }

@@ -69,6 +69,7 @@ struct hydro_sim {
   hipGraphExec_t graph2[2] = {nullptr, nullptr};
   bool graph_pending[2] = {false, false};  // ghosts_pending after the captured pair
   bool use_graphs = true;
+  bool use_run1d = true;  // small 1D Euler grids: whole runs in one workgroup
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
@@ -157,7 +158,39 @@ int enqueue_boundary(hydro_sim* s) {
   return HYDRO_OK;
 }
 
-int enqueue_dt(hydro_sim* s, bool advance_time) {
+// The CFL strip march of a 1D/2D Euler or SRHD grid: waves = strips x row chunks
+DtMarchArgs march_args(const hydro_sim* s) {
+  DtMarchArgs a{};
+  a.dim = s->dim;
+  a.sx = s->sz[0];
+  a.sy = s->sz[1];
+  a.plane = s->plane;
+  a.kdx[0] = s->kdx[0];
+  a.kdx[1] = s->kdx[1];
+  a.gamma = s->d.gamma;
+  a.solver = s->d.equation == HYDRO_EQN_SRHD ? 3 : s->d.solver;
+  const int hi = s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
+  a.last_x = s->sz[0] - hi;
+  a.last_y = s->dim > 1 ? s->sz[1] - hi : 0;
+  a.nstrips = (a.last_x - 2) / 60 + 1;
+  if (s->dim > 1) {
+    const long long ny = a.last_y - 1;
+    // up to kDtRows rows per wave, fewer when that leaves SIMDs idle
+    a.rows = kDtRows;
+    while (a.rows > 2 && (long long)a.nstrips * ((ny + a.rows - 1) / a.rows) < 4096) --a.rows;
+    a.nchunks = (int)((ny + a.rows - 1) / a.rows);
+  } else {
+    a.rows = 1;
+    a.nchunks = 1;
+  }
+  a.nwaves = a.nstrips * a.nchunks;
+  a.wavemin = s->blockmin;
+  return a;
+}
+
+// The dt tail; advance_time false leaves the clock and the step count in a
+// scratch copy (calc_timestep)
+FinalizeArgs finalize_args(hydro_sim* s, bool advance_time) {
   FinalizeArgs f{};
   f.blockmin = s->blockmin;
   f.nblocks = s->d.use_fixed_dt ? 0 : s->dt_blocks;
@@ -167,10 +200,14 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
   f.clip = (s->d.flags & HYDRO_FLAG_CLIP_TO_TEND) ? 1 : 0;
   f.t_end = s->d.t_end;
   f.dt = &s->sc->dt;
-  // without advance the time/steps land in a scratch copy
   f.t = advance_time ? &s->sc->t : &s->sc[1].t;
   f.steps = advance_time ? &s->sc->steps : &s->sc[1].steps;
   f.health = &s->sc->health;
+  return f;
+}
+
+int enqueue_dt(hydro_sim* s, bool advance_time) {
+  FinalizeArgs f = finalize_args(s, advance_time);
   const int march_strips = (s->sz[0] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 2) / 60 + 1;
   const long long march_waves =
       (long long)march_strips * (s->dim > 1 ? (s->sz[1] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 1 + kDtRows - 1) / kDtRows : 1);
@@ -178,31 +215,7 @@ int enqueue_dt(hydro_sim* s, bool advance_time) {
   if (!s->d.use_fixed_dt && s->dim <= 2 && (s->d.equation == HYDRO_EQN_EULER || srhd) &&
       march_waves <= kDtMaxWaves) {
     // strip march, one minimum per wave, then the one-block tail
-    DtMarchArgs a{};
-    a.dim = s->dim;
-    a.sx = s->sz[0];
-    a.sy = s->sz[1];
-    a.plane = s->plane;
-    a.kdx[0] = s->kdx[0];
-    a.kdx[1] = s->kdx[1];
-    a.gamma = s->d.gamma;
-    a.solver = srhd ? 3 : s->d.solver;
-    const int hi = s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
-    a.last_x = s->sz[0] - hi;
-    a.last_y = s->dim > 1 ? s->sz[1] - hi : 0;
-    a.nstrips = (a.last_x - 2) / 60 + 1;
-    if (s->dim > 1) {
-      const long long ny = a.last_y - 1;
-      // up to kDtRows rows per wave, fewer when that leaves SIMDs idle
-      a.rows = kDtRows;
-      while (a.rows > 2 && (long long)a.nstrips * ((ny + a.rows - 1) / a.rows) < 4096) --a.rows;
-      a.nchunks = (int)((ny + a.rows - 1) / a.rows);
-    } else {
-      a.rows = 1;
-      a.nchunks = 1;
-    }
-    a.nwaves = a.nstrips * a.nchunks;
-    a.wavemin = s->blockmin;
+    const DtMarchArgs a = march_args(s);
     if (srhd) launch_dt_march_srhd(a, s->prim, s->mixed, s->stream);
     else if (s->mixed) launch_dt_march_f32(a, (const float*)s->U[s->cur], s->stream);
     else launch_dt_march_f64(a, (const double*)s->U[s->cur], s->stream);
@@ -406,6 +419,51 @@ int enqueue_stages(hydro_sim* s) {
   return HYDRO_OK;
 }
 
+// A run of a small 1D Euler grid: every update of up to kRun1dChunk steps in
+// one single-workgroup launch (run1d.hip) instead of S + 3 launches per
+// update; the same device code on the same buffers, so the same results.
+constexpr int kRun1dChunk = 256;
+
+bool run1d_eligible(const hydro_sim* s) {
+  if (!s->use_run1d || s->profiling || s->dim != 1 || s->d.equation != HYDRO_EQN_EULER) return false;
+  if (!boundary_covers_ghosts(s) || s->tab.order > kRun1dMaxStages) return false;
+  const int strips = (s->sz[0] - 4 + 59) / 60;  // 60 output columns per strip
+  return strips <= (s->mixed ? kRun1dMaxStripsF32 : kRun1dMaxStripsF64);
+}
+
+template <typename T>
+int run1d(hydro_sim* s, int nsteps) {
+  Run1dArgs<T, T> r{};
+  static_assert(sizeof(r) <= 4096, "kernel arguments fit the kernarg segment");
+  r.nstages = s->tab.order;
+  r.b = boundary_args(s);
+  r.march = s->d.use_fixed_dt ? 0 : 1;
+  r.dt = march_args(s);
+  r.f = finalize_args(s, true);
+  // ForwardEuler's zero-gravity pass refreshes the ghosts after the update
+  // (src/Solver/SelfGravitation.cpp:122 + src/Integrator/ForwardEuler.cpp:29-31)
+  r.post_boundary =
+      (s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER ? 1 : 0;
+  for (int i = 0; i < nsteps;) {
+    const int c = std::min(nsteps - i, kRun1dChunk);
+    for (int k = 0; k < r.nstages; ++k) fill_stage_args<T>(s, k, r.st[k]);
+    r.U[0] = (T*)s->U[s->cur];
+    r.U[1] = (T*)s->U[s->cur ^ 1];
+    r.lo[0] = s->kahan ? s->Ulo[s->cur] : nullptr;
+    r.lo[1] = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
+    r.nsteps = c;
+    if constexpr (sizeof(T) == 8) launch_run1d_f64(kernel_kind(s->d), s->d.limiter, r, s->stream);
+    else launch_run1d_f32(kernel_kind(s->d), s->d.limiter, s->kahan, r, s->stream);
+    HIP_TRY(hipGetLastError());
+    s->cur ^= c & 1;
+    i += c;
+  }
+  // as after enqueue_update: the new state's ghost cells are u0's until
+  // copied, unless the post-update boundary rewrote them
+  s->ghosts_pending = !r.post_boundary;
+  return HYDRO_OK;
+}
+
 int enqueue_update(hydro_sim* s) {
   int rc = enqueue_boundary(s);
   if (rc) return rc;
@@ -555,6 +613,7 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   }
   if (const char* e = getenv("HYDRO_ROWS_PER_WAVE")) s->rows = std::max(1, atoi(e));
   if (const char* e = getenv("HYDRO_GRAPHS")) s->use_graphs = atoi(e) != 0;
+  if (const char* e = getenv("HYDRO_RUN1D")) s->use_run1d = atoi(e) != 0;
   if (const char* e = getenv("HYDRO_WAVES_PER_BLOCK")) s->wpb = std::min(4, std::max(1, atoi(e)));
 
   auto cleanup = [&](int code, const std::string& m) {
@@ -780,6 +839,8 @@ int hydro_sim_calc_timestep(hydro_sim* s, double* dt_out) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
   int rc = materialize_ghosts(s);  // the CFL reads the ghost neighbours of edge cells
   if (rc) return rc;
+  // the scratch clock starts at the real one (clipping to t_end reads it)
+  HIP_TRY(hipMemcpyAsync(&s->sc[1], &s->sc[0], sizeof(DevScalars), hipMemcpyDeviceToDevice, s->stream));
   rc = enqueue_dt(s, false);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(s->host_sc, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
@@ -817,6 +878,7 @@ int hydro_sim_update(hydro_sim* s, double* dt_out) {
 
 int hydro_sim_run(hydro_sim* s, int nsteps) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  if (nsteps > 0 && run1d_eligible(s)) return s->mixed ? run1d<float>(s, nsteps) : run1d<double>(s, nsteps);
   // Pairs of updates replay a captured hipGraph (one submission instead of
   // a launch per kernel); an odd step, or profiling, runs the launches
   // directly.  The pair returns the state to the same buffer parity and the

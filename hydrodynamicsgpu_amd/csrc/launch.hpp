@@ -116,6 +116,34 @@ void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, h
 void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
 
+// Whole updates of a small 1D Euler grid in one workgroup (run1d.hip): per
+// step the boundary, the CFL march, the dt tail and every RK stage, with
+// workgroup barriers between them.  st[k] are the stage arguments with
+// u0 = U[0], unew = U[1]; odd steps swap the two.
+constexpr int kRun1dMaxStages = 4;
+// One CU's arithmetic bounds the single-workgroup run: beyond these strip
+// counts (measured crossover) the launch-per-kernel path, which spreads a
+// stage over many CUs, is faster.
+constexpr int kRun1dMaxStripsF64 = 24;
+constexpr int kRun1dMaxStripsF32 = 48;
+constexpr int kRun1dThreads = 1024;
+template <typename T, typename S>
+struct Run1dArgs {
+  StageArgs<T, S> st[kRun1dMaxStages];
+  int nstages;
+  int nsteps;
+  S* U[2];
+  float* lo[2];         // compensated storage low words, or null
+  BoundaryArgs b;
+  int march;            // 0: fixed dt (no CFL pass)
+  DtMarchArgs dt;       // the 1D strip march (wavemin unused)
+  FinalizeArgs f;       // the dt tail (blockmin unused)
+  int post_boundary;    // boundary on the new state after each update
+};
+// kind: 0 Roe, 1 HLL, 2 HLLC (Euler)
+void launch_run1d_f64(int kind, int lim, const Run1dArgs<double, double>& a, hipStream_t st);
+void launch_run1d_f32(int kind, int lim, bool kahan, const Run1dArgs<float, float>& a, hipStream_t st);
+
 // FP64 interior sums, per block: partial[b * n + s]
 void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u,
                        double* partial, int nblocks, hipStream_t st);

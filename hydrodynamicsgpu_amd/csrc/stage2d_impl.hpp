@@ -35,9 +35,9 @@ namespace hydro {
 
 // One wavefront's share of a stage: columns strip*60-2 .. strip*60+61 of
 // rows 2 + chunk*rows .. (1D: the single row).  strip and chunk are
-// wave-uniform.
+// wave-uniform; dtd is the step's dt (*a.dt unless the caller holds it).
 template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>
-HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
+HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, double dtd) {
   constexpr int N = Pol::N;
   using Frozen = typename Pol::Frozen;
   using Stage = typename Pol::Stage;
@@ -53,7 +53,6 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
 
   const PhysParams<T> pp = make_params<T>(a);
   const T gamma = pp.gamma;
-  const double dtd = *a.dt;
   const T dt_dx0 = T(dtd * a.inv_dx[0]);
   const T idx0 = T(a.inv_dx[0]);
   const StageCombine<T, S, KAHAN, N, MODE> comb(a, dtd);
@@ -117,11 +116,16 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk) {
   };
 
   if constexpr (DIM == 1) {
+    // the combination's operands are loaded with the row, ahead of the flux
+    Raw w;
+    fetch(0, w);
+    typename Comb::Fin fin;
+    comb.fetch_fin_rest(0, cl, w.q0, w.dk, fin);
     Row R;
-    load_row(0, R);
+    build(w, R);
     T div[N];
     xdiv(R, div);
-    if (out_lane) comb.finish(0, c, div);
+    if (out_lane) comb.finish(0, c, fin, div);
   } else {
     const T dt_dy = T(dtd * a.inv_dx[1]);
     const T idy = T(a.inv_dx[1]);
@@ -227,7 +231,7 @@ __global__ void __launch_bounds__(256) stage2d_kernel(StageArgs<T, S> a) {
   const int lb = (b & 7) * per + (b >> 3);
   const int wave = __builtin_amdgcn_readfirstlane(lb * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
   if (wave >= a.nwaves) return;
-  stage2d_wave<Pol, DIM, T, S, KAHAN, MODE>(a, wave % a.nstrips, wave / a.nstrips);
+  stage2d_wave<Pol, DIM, T, S, KAHAN, MODE>(a, wave % a.nstrips, wave / a.nstrips, *a.dt);
 }
 
 template <class Pol, int DIM, typename T, typename S, bool KAHAN, int MODE>

@@ -1246,6 +1246,9 @@ void oracle_step(oracle_sim* s, double dt) {
 /* Solver::update (src/Solver/Solver.cpp:521-545) */
 double oracle_update(oracle_sim* s) {
   oracle_boundary(s);
+  /* HYDRO_FLAG_CLIP_TO_TEND (this build's own flag): once t_end is reached an
+     update only refreshes the ghost cells; the clock and the step count stay */
+  if ((s->d.flags & HYDRO_FLAG_CLIP_TO_TEND) && !(s->t < s->d.t_end)) return 0.0;
   oracle_init_step(s); /* Solver::initStep */
   double dt;
   if (s->d.use_fixed_dt) {
