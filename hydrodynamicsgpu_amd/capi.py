@@ -55,6 +55,18 @@ def lib():
         L.hydro_sim_stage_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.hydro_sim_stream.argtypes = [vp]
         L.hydro_sim_stream.restype = vp
+        L.hydro_sim_element_size.argtypes = [vp]
+        L.hydro_sim_element_size.restype = sz
+        L.hydro_integrator_tableau.argtypes = [ip, ctypes.POINTER(ip), dp, dp]
+        L.hydro_sim_buffer_create.argtypes = [vp, ctypes.POINTER(vp)]
+        L.hydro_sim_buffer_destroy.argtypes = [vp, vp]
+        L.hydro_sim_state_buffer.argtypes = [vp, ctypes.POINTER(vp)]
+        L.hydro_sim_init_step.argtypes = [vp]
+        L.hydro_sim_calc_deriv.argtypes = [vp, ctypes.c_double, vp]
+        L.hydro_sim_buffer_copy.argtypes = [vp, vp, vp]
+        L.hydro_sim_buffer_zero.argtypes = [vp, vp]
+        L.hydro_sim_mult_add.argtypes = [vp, vp, vp, vp, ctypes.c_double]
+        L.hydro_sim_finish_step.argtypes = [vp, ctypes.c_double]
         L.hydro_batch_create.argtypes = [vp, vp, ip, ctypes.POINTER(vp)]
         L.hydro_batch_destroy.argtypes = [vp]
         L.hydro_batch_set_state.argtypes = [vp, ip, dp, sz]
@@ -192,6 +204,41 @@ class Sim:
         check(lib().hydro_sim_health(self._h, ctypes.byref(f)))
         return f.value
 
+    # generic stage evaluation (device buffers are opaque handles here)
+    def element_size(self):
+        return lib().hydro_sim_element_size(self._h)
+
+    def buffer_create(self):
+        b = ctypes.c_void_p()
+        check(lib().hydro_sim_buffer_create(self._h, ctypes.byref(b)))
+        return b
+
+    def buffer_destroy(self, buf):
+        check(lib().hydro_sim_buffer_destroy(self._h, buf))
+
+    def state_buffer(self):
+        b = ctypes.c_void_p()
+        check(lib().hydro_sim_state_buffer(self._h, ctypes.byref(b)))
+        return b
+
+    def init_step(self):
+        check(lib().hydro_sim_init_step(self._h))
+
+    def calc_deriv(self, dt, deriv):
+        check(lib().hydro_sim_calc_deriv(self._h, float(dt), deriv))
+
+    def buffer_copy(self, dst, src):
+        check(lib().hydro_sim_buffer_copy(self._h, dst, src))
+
+    def buffer_zero(self, buf):
+        check(lib().hydro_sim_buffer_zero(self._h, buf))
+
+    def mult_add(self, a, b, c, k):
+        check(lib().hydro_sim_mult_add(self._h, a, b, c, float(k)))
+
+    def finish_step(self, dt):
+        check(lib().hydro_sim_finish_step(self._h, float(dt)))
+
     def set_profiling(self, enable):
         check(lib().hydro_sim_set_profiling(self._h, 1 if enable else 0))
 
@@ -213,13 +260,83 @@ EXPORTED_SYMBOLS = [
     "hydro_sim_restore", "hydro_sim_get_storage",
     "hydro_sim_boundary", "hydro_sim_calc_timestep", "hydro_sim_step", "hydro_sim_update",
     "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
-    "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_batch_create", "hydro_batch_destroy", "hydro_batch_set_state",
+    "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing",
+    "hydro_sim_element_size", "hydro_integrator_tableau", "hydro_sim_buffer_create", "hydro_sim_buffer_destroy",
+    "hydro_sim_state_buffer", "hydro_sim_init_step", "hydro_sim_calc_deriv", "hydro_sim_buffer_copy",
+    "hydro_sim_buffer_zero", "hydro_sim_mult_add", "hydro_sim_finish_step",
+    "hydro_batch_create", "hydro_batch_destroy", "hydro_batch_set_state",
     "hydro_batch_get_state", "hydro_batch_run", "hydro_batch_sync", "hydro_batch_member_dt",
     "hydro_batch_interior_cells", "hydro_batch_member_time", "hydro_batch_member_health",
     "hydro_batch_set_decomposition", "hydro_batch_autotune",
 ]
 
-__all__ = ["Sim", "Batch", "HydroError", "lib", "check", "context", "config", "EXPORTED_SYMBOLS", "LIB_PATH"]
+def tableau(integrator):
+    """(alphas, betas) of an integrator id, each (order, order)
+    (include/HydroGPU/Integrator/RungeKutta.h:129-294)."""
+    order = ctypes.c_int()
+    A = np.zeros(16)
+    B = np.zeros(16)
+    check(lib().hydro_integrator_tableau(int(integrator), ctypes.byref(order), _dp(A), _dp(B)))
+    n = order.value
+    return A.reshape(4, 4)[:n, :n].copy(), B.reshape(4, 4)[:n, :n].copy()
+
+
+def integrate(sim, dt, callback=None):
+    """One step of the simulation's integrator with the combination on the
+    host side, as the reference's Integrator::integrate(dt, callback) does
+    (include/HydroGPU/Integrator/RungeKutta.h:59-125,
+    src/Integrator/ForwardEuler.cpp:8-32): u(0) = state; for each stage the
+    callback fills a zeroed derivative buffer for the state buffer's current
+    contents; the state is rebuilt as sum_k alphas(i-1,k) u(k) +
+    dt betas(i-1,k) L(u(k)) in that order, skipping zero coefficients; stage
+    states and derivatives are kept only when a later row uses them.  The
+    default callback is the solver's derivative (Roe::step's lambda,
+    src/Solver/Roe.cpp:94-99).  Frames the step with init_step /
+    finish_step."""
+    if callback is None:
+        def callback(deriv):
+            sim.calc_deriv(dt, deriv)
+    A, B = tableau(sim.desc.integrator)
+    n = A.shape[0]
+    state = sim.state_buffer()
+    sim.init_step()
+    bufs = []
+
+    def new():
+        b = sim.buffer_create()
+        bufs.append(b)
+        return b
+
+    try:
+        u = [new() if any(A[m, i] != 0 for m in range(i, n)) else None for i in range(n)]
+        d = [new() if any(B[m, i] != 0 for m in range(i, n)) else None for i in range(n)]
+        if u[0] is not None:
+            sim.buffer_copy(u[0], state)
+        if d[0] is not None:
+            sim.buffer_zero(d[0])
+            callback(d[0])
+        for i in range(1, n + 1):
+            sim.buffer_zero(state)
+            for k in range(i):
+                if A[i - 1, k]:
+                    sim.mult_add(state, state, u[k], A[i - 1, k])
+                if B[i - 1, k]:
+                    sim.mult_add(state, state, d[k], B[i - 1, k] * dt)
+            if i < n:
+                if u[i] is not None:
+                    sim.buffer_copy(u[i], state)
+                if d[i] is not None:
+                    sim.buffer_zero(d[i])
+                    callback(d[i])
+        sim.finish_step(dt)
+    finally:
+        sim.sync()
+        for b in bufs:
+            sim.buffer_destroy(b)
+
+
+__all__ = ["Sim", "Batch", "HydroError", "lib", "check", "context", "config", "EXPORTED_SYMBOLS", "LIB_PATH",
+           "tableau", "integrate"]
 
 
 class Batch:

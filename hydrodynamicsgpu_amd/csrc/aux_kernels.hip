@@ -359,6 +359,32 @@ __global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, flo
   }
 }
 
+// ---------------------------------------------------------------- buffers
+// The host-side integrator's buffer operations (include/HydroGPU/Integrator/
+// RungeKutta.h:59-125): multAdd a = b + c * k (res/Common.cl:201-210) and a
+// zero fill (Solver::CL::zero, src/Solver/Solver.cpp:22-32), as kernels so a
+// captured graph holds no memset node.  Grid-stride loops over n elements.
+template <typename V>
+__global__ void __launch_bounds__(256) mult_add_kernel(V* a, const V* b, const V* c, V k, long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) a[i] = b[i] + c[i] * k;
+}
+__global__ void __launch_bounds__(256) zero_words_kernel(uint32_t* p, long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = 0u;
+}
+static unsigned stride_blocks(long long n) {
+  const long long b = (n + 255) / 256;
+  return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+void launch_mult_add_f64(double* a, const double* b, const double* c, double k, long long n, hipStream_t st) {
+  hipLaunchKernelGGL((mult_add_kernel<double>), dim3(stride_blocks(n)), dim3(256), 0, st, a, b, c, k, n);
+}
+void launch_mult_add_f32(float* a, const float* b, const float* c, float k, long long n, hipStream_t st) {
+  hipLaunchKernelGGL((mult_add_kernel<float>), dim3(stride_blocks(n)), dim3(256), 0, st, a, b, c, k, n);
+}
+void launch_zero_words(void* p, long long nwords, hipStream_t st) {
+  hipLaunchKernelGGL(zero_words_kernel, dim3(stride_blocks(nwords)), dim3(256), 0, st, (uint32_t*)p, nwords);
+}
+
 template <typename S>
 static void con2prim(const Con2PrimArgs& a, S* u, float* lo, double* prim, hipStream_t st) {
   dim3 g((a.sx + 255) / 256, a.dim > 1 ? a.sy : 1, a.dim > 2 ? a.sz : 1), b(256);

@@ -70,6 +70,10 @@ struct hydro_sim {
   bool graph_pending[2] = {false, false};  // ghosts_pending after the captured pair
   bool use_graphs = true;
   bool use_run1d = true;  // small 1D Euler grids: whole runs in one workgroup
+  // generic stage evaluation: buffers handed out to a host-side integrator,
+  // and whether U[cur ^ 1] holds the frozen start-of-step state
+  std::vector<void*> user_bufs;
+  bool frozen = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
@@ -342,11 +346,53 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   }
 }
 
+// one fused stage launch in the given combination mode
+void launch_stage(hydro_sim* s, int mode, const StageArgs<float, float>& a) {
+  const int kind = kernel_kind(s->d);
+  if (s->dim == 3) launch_stage3d_mixed(kind, s->d.limiter, mode, s->kahan, a, s->stream);
+  else if (s->kahan) launch_stage2d_f32k(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
+  else launch_stage2d_f32(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
+}
+void launch_stage(hydro_sim* s, int mode, const StageArgs<double, double>& a) {
+  const int kind = kernel_kind(s->d);
+  if (s->dim == 3) launch_stage3d_f64(kind, s->d.limiter, mode, a, s->stream);
+  else launch_stage2d_f64(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
+}
+
+// SRHDRoe::step tail: constrainState + updatePrimitives on the current state
+// (src/Solver/SRHDRoe.cpp:62-73)
+int enqueue_srhd_tail(hydro_sim* s) {
+  Con2PrimArgs c{};
+  c.dim = s->dim;
+  c.sx = s->sz[0];
+  c.sy = s->sz[1];
+  c.sz = s->sz[2];
+  c.plane = s->plane;
+  c.gamma = s->d.gamma;
+  c.max_iter = s->d.srhd_solve_prim_max_iter;
+  c.stop_eps = s->d.srhd_solve_prim_stop_epsilon;
+  c.vel_eps = s->d.srhd_solve_prim_vel_epsilon;
+  c.pmin_eps = s->d.srhd_solve_prim_pmin_epsilon;
+  c.rho_min = s->d.srhd_rho_min;
+  c.rho_max = s->d.srhd_rho_max;
+  c.eint_max = s->d.srhd_eint_max;
+  c.D_min = s->d.srhd_D_min;
+  c.D_max = s->d.srhd_D_max;
+  c.tau_min = s->d.srhd_tau_min;
+  c.tau_max = s->d.srhd_tau_max;
+  c.health = &s->sc->health;
+  if (s->mixed) launch_con2prim_f32(c, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->stream);
+  else launch_con2prim_f64(c, (double*)s->U[s->cur], s->prim, s->stream);
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
 int enqueue_stages(hydro_sim* s) {
   {
     const int rc = materialize_ghosts(s);  // the stencil reads u0's ghost cells
     if (rc) return rc;
   }
+  s->frozen = false;  // U[cur ^ 1] becomes the new state
   const int S = s->tab.order;
   for (int k = 0; k < S; ++k) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -356,18 +402,14 @@ int enqueue_stages(hydro_sim* s) {
       HIP_TRY(hipEventRecord(e0, s->stream));
     }
     const int mode = s->plan[k].mode;
-    const int kind = kernel_kind(s->d);
     if (s->mixed) {
       StageArgs<float, float> a{};
       fill_stage_args<float>(s, k, a);
-      if (s->dim == 3) launch_stage3d_mixed(kind, s->d.limiter, mode, s->kahan, a, s->stream);
-      else if (s->kahan) launch_stage2d_f32k(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
-      else launch_stage2d_f32(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
+      launch_stage(s, mode, a);
     } else {
       StageArgs<double, double> a{};
       fill_stage_args<double>(s, k, a);
-      if (s->dim == 3) launch_stage3d_f64(kind, s->d.limiter, mode, a, s->stream);
-      else launch_stage2d_f64(s->dim, kind, s->d.limiter, mode, a, s->wpb, s->stream);
+      launch_stage(s, mode, a);
     }
     HIP_TRY(hipGetLastError());
     if (s->profiling) {
@@ -392,30 +434,7 @@ int enqueue_stages(hydro_sim* s) {
     if (rc) return rc;
   }
   // SRHDRoe::step tail: constrainState + updatePrimitives (src/Solver/SRHDRoe.cpp:62-73)
-  if (s->d.equation == HYDRO_EQN_SRHD) {
-    Con2PrimArgs c{};
-    c.dim = s->dim;
-    c.sx = s->sz[0];
-    c.sy = s->sz[1];
-    c.sz = s->sz[2];
-    c.plane = s->plane;
-    c.gamma = s->d.gamma;
-    c.max_iter = s->d.srhd_solve_prim_max_iter;
-    c.stop_eps = s->d.srhd_solve_prim_stop_epsilon;
-    c.vel_eps = s->d.srhd_solve_prim_vel_epsilon;
-    c.pmin_eps = s->d.srhd_solve_prim_pmin_epsilon;
-    c.rho_min = s->d.srhd_rho_min;
-    c.rho_max = s->d.srhd_rho_max;
-    c.eint_max = s->d.srhd_eint_max;
-    c.D_min = s->d.srhd_D_min;
-    c.D_max = s->d.srhd_D_max;
-    c.tau_min = s->d.srhd_tau_min;
-    c.tau_max = s->d.srhd_tau_max;
-    c.health = &s->sc->health;
-    if (s->mixed) launch_con2prim_f32(c, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->stream);
-    else launch_con2prim_f64(c, (double*)s->U[s->cur], s->prim, s->stream);
-    HIP_TRY(hipGetLastError());
-  }
+  if (s->d.equation == HYDRO_EQN_SRHD) return enqueue_srhd_tail(s);
   return HYDRO_OK;
 }
 
@@ -433,6 +452,7 @@ bool run1d_eligible(const hydro_sim* s) {
 
 template <typename T>
 int run1d(hydro_sim* s, int nsteps) {
+  s->frozen = false;
   Run1dArgs<T, T> r{};
   static_assert(sizeof(r) <= 4096, "kernel arguments fit the kernarg segment");
   r.nstages = s->tab.order;
@@ -461,6 +481,54 @@ int run1d(hydro_sim* s, int nsteps) {
   // as after enqueue_update: the new state's ghost cells are u0's until
   // copied, unless the post-update boundary rewrote them
   s->ghosts_pending = !r.post_boundary;
+  return HYDRO_OK;
+}
+
+// ---- generic stage evaluation helpers
+bool known_buffer(const hydro_sim* s, const void* p) {
+  if (p && p == s->U[s->cur]) return true;
+  return p && std::find(s->user_bufs.begin(), s->user_bufs.end(), p) != s->user_bufs.end();
+}
+
+int generic_supported(const hydro_sim* s) {
+  if (s->kahan)
+    return fail(HYDRO_E_UNSUPPORTED, "generic stage evaluation needs plain storage (no HYDRO_FLAG_KAHAN)");
+  return HYDRO_OK;
+}
+
+// deriv += L(U[cur]) with the frozen start-of-step data of U[cur ^ 1] and dt
+template <typename T>
+int enqueue_deriv(hydro_sim* s, double dt, void* deriv) {
+  // dt into the scratch scalars (the limiter's full-step dt), clock untouched
+  FinalizeArgs f{};
+  f.use_fixed = 1;
+  f.fixed_dt = dt;
+  f.dt = &s->sc[1].dt;
+  f.t = &s->sc[1].t;
+  f.steps = &s->sc[1].steps;
+  f.health = &s->sc[1].health;
+  launch_finalize_dt(f, s->stream);
+  StageArgs<T, T> a{};
+  fill_stage_args<T>(s, 0, a);
+  a.u0 = (const T*)s->U[s->cur ^ 1];
+  a.u0lo = nullptr;
+  a.dk = (const T*)s->U[s->cur];
+  a.dnext = (T*)deriv;
+  a.unew = nullptr;
+  a.unewlo = nullptr;
+  a.pkp = nullptr;
+  a.nf = 0;
+  a.nrep = 0;
+  for (int j = 0; j < kMaxPartials; ++j) {
+    a.fp[j] = nullptr;
+    a.rep[j] = nullptr;
+  }
+  a.final_stage = 0;
+  a.a_self = 0.0;
+  a.b_self = 1.0;
+  a.dt = &s->sc[1].dt;
+  launch_stage(s, CM_DERIV, a);
+  HIP_TRY(hipGetLastError());
   return HYDRO_OK;
 }
 
@@ -543,6 +611,7 @@ void hydro_sim_destroy(hydro_sim* s) {
     if (s->D[i]) hipFree(s->D[i]);
   for (int j = 0; j < kMaxPartials; ++j)
     if (s->P[j]) hipFree(s->P[j]);
+  for (void* p : s->user_bufs) hipFree(p);
   if (s->prim) hipFree(s->prim);
   if (s->blockmin) hipFree(s->blockmin);
   if (s->partial) hipFree(s->partial);
@@ -669,6 +738,7 @@ void* hydro_sim_stream(hydro_sim* s) { return s ? (void*)s->stream : nullptr; }
 // FP32 storage keeps the residual as the low word when compensated, or
 // takes the low words given (`low`, compensated storage only)
 static int upload_state(hydro_sim* s, const double* aos, const float* low = nullptr) {
+  s->frozen = false;
   const int N = s->N;
   const long long V = s->vol;
   if (s->mixed) {
@@ -987,6 +1057,125 @@ int hydro_sim_totals(hydro_sim* s, double* totals_out) {
     for (int b = 0; b < s->tot_blocks; ++b) acc += part[(size_t)b * s->N + k];
     totals_out[k] = acc * cellvol;
   }
+  return HYDRO_OK;
+}
+
+// ---- generic stage evaluation (host-side integrators)
+
+size_t hydro_sim_element_size(const hydro_sim* s) { return s ? esize(s) : 0; }
+
+int hydro_integrator_tableau(int integrator, int* order, double* alpha, double* beta) {
+  Tableau t;
+  if (!order || !alpha || !beta) return fail(HYDRO_E_INVALID, "null argument");
+  if (!get_tableau(integrator, t)) return fail(HYDRO_E_INVALID, "unknown integrator");
+  *order = t.order;
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 4; ++k) {
+      alpha[4 * i + k] = t.a[i][k];
+      beta[4 * i + k] = t.b[i][k];
+    }
+  return HYDRO_OK;
+}
+
+int hydro_sim_buffer_create(hydro_sim* s, void** buf) {
+  if (!s || !buf) return fail(HYDRO_E_INVALID, "null argument");
+  const size_t bytes = esize(s) * (size_t)s->plane * s->N;
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, bytes));
+  const hipError_t e = hipMemset(p, 0, bytes);
+  if (e != hipSuccess) {
+    hipFree(p);
+    return fail(HYDRO_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  }
+  s->user_bufs.push_back(p);
+  *buf = p;
+  return HYDRO_OK;
+}
+
+int hydro_sim_buffer_destroy(hydro_sim* s, void* buf) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  auto it = std::find(s->user_bufs.begin(), s->user_bufs.end(), buf);
+  if (it == s->user_bufs.end()) return fail(HYDRO_E_INVALID, "not a buffer of this simulation");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipFree(buf));
+  s->user_bufs.erase(it);
+  return HYDRO_OK;
+}
+
+int hydro_sim_state_buffer(hydro_sim* s, void** buf) {
+  if (!s || !buf) return fail(HYDRO_E_INVALID, "null argument");
+  const int rc = materialize_ghosts(s);
+  if (rc) return rc;
+  *buf = s->U[s->cur];
+  return HYDRO_OK;
+}
+
+int hydro_sim_init_step(hydro_sim* s) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  int rc = generic_supported(s);
+  if (rc) return rc;
+  rc = materialize_ghosts(s);
+  if (rc) return rc;
+  // the frozen start-of-step state: the fused path's scratch buffer holds it
+  HIP_TRY(hipMemcpyAsync(s->U[s->cur ^ 1], s->U[s->cur], esize(s) * (size_t)s->plane * s->N,
+                         hipMemcpyDeviceToDevice, s->stream));
+  s->frozen = true;
+  return HYDRO_OK;
+}
+
+int hydro_sim_calc_deriv(hydro_sim* s, double dt, void* deriv) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  int rc = generic_supported(s);
+  if (rc) return rc;
+  if (!s->frozen) return fail(HYDRO_E_INVALID, "hydro_sim_init_step must precede hydro_sim_calc_deriv");
+  if (!known_buffer(s, deriv) || deriv == s->U[s->cur])
+    return fail(HYDRO_E_INVALID, "the derivative must be a buffer from hydro_sim_buffer_create");
+  return s->mixed ? enqueue_deriv<float>(s, dt, deriv) : enqueue_deriv<double>(s, dt, deriv);
+}
+
+int hydro_sim_buffer_copy(hydro_sim* s, void* dst, const void* src) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  if (!known_buffer(s, dst) || !known_buffer(s, src)) return fail(HYDRO_E_INVALID, "unknown buffer");
+  if (dst == src) return HYDRO_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, esize(s) * (size_t)s->plane * s->N, hipMemcpyDeviceToDevice, s->stream));
+  return HYDRO_OK;
+}
+
+int hydro_sim_buffer_zero(hydro_sim* s, void* buf) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  if (!known_buffer(s, buf)) return fail(HYDRO_E_INVALID, "unknown buffer");
+  launch_zero_words(buf, (long long)(esize(s) / 4) * s->plane * s->N, s->stream);
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
+int hydro_sim_mult_add(hydro_sim* s, void* a, const void* b, const void* c, double k) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  if (!known_buffer(s, a) || !known_buffer(s, b) || !known_buffer(s, c)) return fail(HYDRO_E_INVALID, "unknown buffer");
+  const long long n = s->plane * s->N;
+  if (s->mixed) launch_mult_add_f32((float*)a, (const float*)b, (const float*)c, (float)k, n, s->stream);
+  else launch_mult_add_f64((double*)a, (const double*)b, (const double*)c, k, n, s->stream);
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
+int hydro_sim_finish_step(hydro_sim* s, double dt) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  int rc = generic_supported(s);
+  if (rc) return rc;
+  // the clock advanced by dt (the tail of findMinTimestep with a given dt)
+  FinalizeArgs f{};
+  f.use_fixed = 1;
+  f.fixed_dt = dt;
+  f.dt = &s->sc->dt;
+  f.t = &s->sc->t;
+  f.steps = &s->sc->steps;
+  f.health = &s->sc->health;
+  launch_finalize_dt(f, s->stream);
+  HIP_TRY(hipGetLastError());
+  s->frozen = false;
+  s->ghosts_pending = false;  // the integrator combined every cell, ghosts included
+  if (s->d.equation == HYDRO_EQN_SRHD) return enqueue_srhd_tail(s);
   return HYDRO_OK;
 }
 

@@ -144,6 +144,12 @@ struct Run1dArgs {
 void launch_run1d_f64(int kind, int lim, const Run1dArgs<double, double>& a, hipStream_t st);
 void launch_run1d_f32(int kind, int lim, bool kahan, const Run1dArgs<float, float>& a, hipStream_t st);
 
+// buffer operations of host-side integrators: a = b + c * k over n
+// elements (multAdd), and a zero fill of n 32-bit words
+void launch_mult_add_f64(double* a, const double* b, const double* c, double k, long long n, hipStream_t st);
+void launch_mult_add_f32(float* a, const float* b, const float* c, float k, long long n, hipStream_t st);
+void launch_zero_words(void* p, long long nwords, hipStream_t st);
+
 // FP64 interior sums, per block: partial[b * n + s]
 void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u,
                        double* partial, int nblocks, hipStream_t st);

@@ -245,6 +245,12 @@ static void launch_pol_mode(const StageArgs<T, S>& a, int wpb, hipStream_t st) {
 // with compile-time limiters; everything else runs CM_GENERIC.
 template <class Pol, int DIM, typename T, typename S, bool KAHAN, bool MODES>
 static void launch_pol(int mode, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
+  if constexpr (!KAHAN) {  // generic stage evaluation (plain storage only)
+    if (mode == CM_DERIV) {
+      launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_DERIV>(a, wpb, st);
+      return;
+    }
+  }
   if constexpr (MODES) {
     switch (mode) {
     case CM_FIRST: launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_FIRST>(a, wpb, st); return;

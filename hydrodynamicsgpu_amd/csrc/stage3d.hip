@@ -17,7 +17,7 @@
 
 namespace hydro {
 
-template <class Pol, typename T, typename S, bool KAHAN, int W>
+template <class Pol, typename T, typename S, bool KAHAN, int W, int MODE>
 __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   constexpr int N = Pol::N;
   constexpr int NF = Pol::NF;
@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   const double dtd = *a.dt;
   const T dt_dx = T(dtd * a.inv_dx[0]), dt_dy = T(dtd * a.inv_dx[1]), dt_dz = T(dtd * a.inv_dx[2]);
   const T idx = T(a.inv_dx[0]), idy = T(a.inv_dx[1]), idz = T(a.inv_dx[2]);
-  const StageCombine<T, S, KAHAN, N, CM_GENERIC> comb(a, dtd);
+  const StageCombine<T, S, KAHAN, N, MODE> comb(a, dtd);
   const long long psz = (long long)sx * sy;
 
   auto load = [&](int z, Stage& st, Frozen& fz) {
@@ -396,14 +396,26 @@ static void launch3_r1(int mode, const StageArgs<T, S>& a, hipStream_t st) {
   case CM_MID: launch3_r1_mode<Pol, T, S, KAHAN, CM_MID>(a, st); return;
   case CM_FINAL: launch3_r1_mode<Pol, T, S, KAHAN, CM_FINAL>(a, st); return;
   case CM_SINGLE: launch3_r1_mode<Pol, T, S, KAHAN, CM_SINGLE>(a, st); return;
+  case CM_DERIV:
+    if constexpr (!KAHAN) {
+      launch3_r1_mode<Pol, T, S, KAHAN, CM_DERIV>(a, st);
+      return;
+    }
+    [[fallthrough]];
   default: launch3_r1_mode<Pol, T, S, KAHAN, CM_GENERIC>(a, st); return;
   }
 }
 
 template <class Pol, typename T, typename S, bool KAHAN, int W>
-static void launch3(const StageArgs<T, S>& a, hipStream_t st) {
+static void launch3(int mode, const StageArgs<T, S>& a, hipStream_t st) {
   int blocks = (a.nwaves + 7) & ~7;  // nwaves = workgroups; multiple of 8 for the XCD remap
-  hipLaunchKernelGGL((stage3d_kernel<Pol, T, S, KAHAN, W>), dim3(blocks), dim3(64 * W), 0, st, a);
+  if constexpr (!KAHAN) {
+    if (mode == CM_DERIV) {
+      hipLaunchKernelGGL((stage3d_kernel<Pol, T, S, KAHAN, W, CM_DERIV>), dim3(blocks), dim3(64 * W), 0, st, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((stage3d_kernel<Pol, T, S, KAHAN, W, CM_GENERIC>), dim3(blocks), dim3(64 * W), 0, st, a);
 }
 
 template <typename T, typename S, bool KAHAN, int W>
@@ -413,15 +425,15 @@ static void launch3_solver(int kind, int lim, int mode, const StageArgs<T, S>& a
   } else if (kind == 2) {
     launch3_r1<HLLPolicy<3, T, 2>, T, S, KAHAN>(mode, a, st);
   } else if (kind == 3) {
-    launch3<SRHDRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(a, st);
+    launch3<SRHDRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
   } else if (kind == 4) {
-    launch3<MaxwellRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(a, st);
+    launch3<MaxwellRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
   } else {
     switch (lim) {
-    case LIM_SUPERBEE: launch3<RoePolicy<3, T, LIM_SUPERBEE>, T, S, KAHAN, W>(a, st); break;
-    case LIM_MONOTIZEDCENTRAL: launch3<RoePolicy<3, T, LIM_MONOTIZEDCENTRAL>, T, S, KAHAN, W>(a, st); break;
-    case LIM_MINMOD: launch3<RoePolicy<3, T, LIM_MINMOD>, T, S, KAHAN, W>(a, st); break;
-    default: launch3<RoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(a, st); break;
+    case LIM_SUPERBEE: launch3<RoePolicy<3, T, LIM_SUPERBEE>, T, S, KAHAN, W>(mode, a, st); break;
+    case LIM_MONOTIZEDCENTRAL: launch3<RoePolicy<3, T, LIM_MONOTIZEDCENTRAL>, T, S, KAHAN, W>(mode, a, st); break;
+    case LIM_MINMOD: launch3<RoePolicy<3, T, LIM_MINMOD>, T, S, KAHAN, W>(mode, a, st); break;
+    default: launch3<RoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st); break;
     }
   }
 }
@@ -430,8 +442,8 @@ static void launch3_solver(int kind, int lim, int mode, const StageArgs<T, S>& a
 // (limiter neighbours)
 int stage3d_rows_per_block(int kind) { return kStage3dWaves - 2 * 2; }
 
-// the Roe-family kernel runs the generic combination (mode ignored); the
-// first-order kernel takes every mode
+// the Roe-family kernel runs the generic combination (or the derivative
+// alone); the first-order kernel takes every mode
 void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, double>& a, hipStream_t st) {
   launch3_solver<double, double, false, kStage3dWaves>(solver, lim, mode, a, st);
 }

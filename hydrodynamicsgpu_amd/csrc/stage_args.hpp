@@ -17,6 +17,7 @@ enum CombineMode {
   CM_MID = 2,      // 0 < k < S-1:       d_{k+1} = a d_k + b dt L_k
   CM_FINAL = 3,    // k = S-1 > 0:       u_new = u0 + a d_k + sum_j rc_j d_{j+1} + b dt L_k
   CM_SINGLE = 4,   // single stage:      u_new = u0 + b dt L_0
+  CM_DERIV = 5,    // derivative only:   dnext += L(dk), dk a full state, frozen data of u0
 };
 
 // One fused RK stage: L(u^(k)) with the frozen start-of-step data, then the

@@ -59,7 +59,7 @@ HYDRO_DEV PhysParams<T> make_params(const StageArgs<T, S>& a) {
 // once per wave; a cell is (wave-uniform element offset e, column col).
 template <typename T, typename S, bool KAHAN, int N, int MODE>
 struct StageCombine {
-  static constexpr bool kReadsDk = MODE == CM_MID || MODE == CM_FINAL || MODE == CM_GENERIC;
+  static constexpr bool kReadsDk = MODE == CM_MID || MODE == CM_FINAL || MODE == CM_GENERIC || MODE == CM_DERIV;
   const StageArgs<T, S>& a;
   const long long pl;
   T bdt, aself;
@@ -83,7 +83,7 @@ struct StageCombine {
     for (int s = 0; s < N; ++s) q0[s] = ldp(a.u0, pl, s, e, col);
   }
   HYDRO_DEV void load_dk(long long e, uint32_t col, T dk[N]) const {
-    if constexpr (MODE == CM_MID || MODE == CM_FINAL) {
+    if constexpr (MODE == CM_MID || MODE == CM_FINAL || MODE == CM_DERIV) {
 #pragma unroll
       for (int s = 0; s < N; ++s) dk[s] = ldp(a.dk, pl, s, e, col);
     } else if constexpr (MODE == CM_GENERIC) {
@@ -96,11 +96,13 @@ struct StageCombine {
       }
     }
   }
-  // stage state u = u0 + d_k in arithmetic precision
+  // stage state u = u0 + d_k in arithmetic precision (CM_DERIV: d_k is the
+  // stage state itself)
   HYDRO_DEV void stage_state(const S q0[N], const T dk[N], T u[N]) const {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-      if constexpr (kReadsDk) u[s] = T(q0[s]) + dk[s];
+      if constexpr (MODE == CM_DERIV) u[s] = dk[s];
+      else if constexpr (kReadsDk) u[s] = T(q0[s]) + dk[s];
       else u[s] = T(q0[s]);
     }
   }
@@ -199,7 +201,12 @@ struct StageCombine {
 
   // combine the derivative of a cell into the increment / state
   HYDRO_DEV void finish(long long e, uint32_t col, const Fin& f, const T deriv[N]) const {
-    if constexpr (MODE == CM_FIRST) {
+    if constexpr (MODE == CM_DERIV) {
+      // calcFluxDeriv accumulates into the caller's derivative buffer
+      // (res/CalcFluxDeriv.cl:7-44: deriv -= flux differences)
+#pragma unroll
+      for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, ldp((const T*)a.dnext, pl, s, e, col) + deriv[s]);
+    } else if constexpr (MODE == CM_FIRST) {
 #pragma unroll
       for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, bdt * deriv[s]);
     } else if constexpr (MODE == CM_MID) {

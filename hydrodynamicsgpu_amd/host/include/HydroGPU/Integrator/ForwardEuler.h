@@ -13,6 +13,10 @@ struct ForwardEuler : public Integrator {
   int integratorId() const override { return HYDRO_INTEGRATOR_FORWARDEULER; }
   std::string name() const override { return "ForwardEuler"; }
   int numStages() const override { return 1; }
+  using Super::integrate;
+  // src/Integrator/ForwardEuler.cpp:20-32: zero the derivative, callback,
+  // state += dt * deriv
+  void integrate(real dt, std::function<void(void* derivBuffer)> callback) override;
 };
 
 }  // namespace Integrator

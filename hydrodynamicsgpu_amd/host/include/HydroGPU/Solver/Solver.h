@@ -62,6 +62,13 @@ struct Solver : public ISolver {
   friend struct HydroGPU::Integrator::Integrator;
 
  public:
+  // Roe::calcDeriv / HLL::calcDeriv (src/Solver/Roe.cpp:101-112): adds the
+  // flux derivative of the state buffer's current contents to derivBuffer,
+  // with the frozen start-of-step data and the full-step dt; the callback of
+  // the host-side integrate(dt, callback) form
+  virtual void calcDeriv(void* derivBuffer, real dt);
+
+ public:
   // src/Solver/Solver.cpp:521-545: boundary -> initStep -> calcTimestep (or
   // fixedDT) -> step
   virtual void update();

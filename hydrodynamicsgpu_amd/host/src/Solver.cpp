@@ -59,6 +59,71 @@ namespace Integrator {
 
 void Integrator::integrate(real dt) { solver->step(dt); }
 
+void Integrator::integrate(real dt, std::function<void(void*)> callback) {
+  hydro_sim* sim = solver->handle();
+  int order = 0;
+  double A[16], B[16];
+  checkStatus(hydro_integrator_tableau(integratorId(), &order, A, B), "hydro_integrator_tableau");
+  auto alpha = [&](int i, int k) { return A[4 * i + k]; };
+  auto beta = [&](int i, int k) { return B[4 * i + k]; };
+  // RungeKutta::RungeKutta (RungeKutta.h:28-51): a stage buffer exists only
+  // if some later row reads it
+  if (stateBuffers.empty()) {
+    stateBuffers.assign(order, nullptr);
+    derivBuffers.assign(order, nullptr);
+    for (int i = 0; i < order; ++i) {
+      bool su = false, sd = false;
+      for (int m = i; m < order; ++m) {
+        su |= alpha(m, i) != 0;
+        sd |= beta(m, i) != 0;
+      }
+      if (su) checkStatus(hydro_sim_buffer_create(sim, &stateBuffers[i]), "hydro_sim_buffer_create");
+      if (sd) checkStatus(hydro_sim_buffer_create(sim, &derivBuffers[i]), "hydro_sim_buffer_create");
+    }
+  }
+  void* state = nullptr;
+  checkStatus(hydro_sim_state_buffer(sim, &state), "hydro_sim_state_buffer");
+  checkStatus(hydro_sim_init_step(sim), "hydro_sim_init_step");
+  // RungeKutta::integrate (RungeKutta.h:59-125)
+  if (stateBuffers[0]) checkStatus(hydro_sim_buffer_copy(sim, stateBuffers[0], state), "hydro_sim_buffer_copy");
+  if (derivBuffers[0]) {
+    checkStatus(hydro_sim_buffer_zero(sim, derivBuffers[0]), "hydro_sim_buffer_zero");
+    callback(derivBuffers[0]);
+  }
+  for (int i = 1; i <= order; ++i) {
+    checkStatus(hydro_sim_buffer_zero(sim, state), "hydro_sim_buffer_zero");
+    for (int k = 0; k < i; ++k) {
+      if (alpha(i - 1, k) != 0)
+        checkStatus(hydro_sim_mult_add(sim, state, state, stateBuffers[k], alpha(i - 1, k)), "hydro_sim_mult_add");
+      if (beta(i - 1, k) != 0)
+        checkStatus(hydro_sim_mult_add(sim, state, state, derivBuffers[k], beta(i - 1, k) * dt), "hydro_sim_mult_add");
+    }
+    if (i < order) {
+      if (stateBuffers[i]) checkStatus(hydro_sim_buffer_copy(sim, stateBuffers[i], state), "hydro_sim_buffer_copy");
+      if (derivBuffers[i]) {
+        checkStatus(hydro_sim_buffer_zero(sim, derivBuffers[i]), "hydro_sim_buffer_zero");
+        callback(derivBuffers[i]);
+      }
+    }
+  }
+  checkStatus(hydro_sim_finish_step(sim, dt), "hydro_sim_finish_step");
+}
+
+void ForwardEuler::integrate(real dt, std::function<void(void*)> callback) {
+  hydro_sim* sim = solver->handle();
+  if (derivBuffers.empty()) {
+    derivBuffers.assign(1, nullptr);
+    checkStatus(hydro_sim_buffer_create(sim, &derivBuffers[0]), "hydro_sim_buffer_create");
+  }
+  void* state = nullptr;
+  checkStatus(hydro_sim_state_buffer(sim, &state), "hydro_sim_state_buffer");
+  checkStatus(hydro_sim_init_step(sim), "hydro_sim_init_step");
+  checkStatus(hydro_sim_buffer_zero(sim, derivBuffers[0]), "hydro_sim_buffer_zero");
+  callback(derivBuffers[0]);
+  checkStatus(hydro_sim_mult_add(sim, state, state, derivBuffers[0], dt), "hydro_sim_mult_add");
+  checkStatus(hydro_sim_finish_step(sim, dt), "hydro_sim_finish_step");
+}
+
 }  // namespace Integrator
 
 namespace Solver {
@@ -153,6 +218,10 @@ real Solver::calcTimestep() {
 }
 
 void Solver::step(real dt) { checkStatus(hydro_sim_step(sim, dt), "step"); }
+
+void Solver::calcDeriv(void* derivBuffer, real dt) {
+  checkStatus(hydro_sim_calc_deriv(sim, dt, derivBuffer), "calcDeriv");
+}
 
 void Solver::update() {
   boundary();

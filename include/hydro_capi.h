@@ -265,6 +265,66 @@ int hydro_sim_set_profiling(hydro_sim* sim, int enable);
  * was enabled, and their count.  Synchronises the stream. */
 int hydro_sim_stage_timing(hydro_sim* sim, double* avg_ms, long long* launches);
 
+/* ---- generic stage evaluation (host-side integrators) -------------------
+ *
+ * The reference's Integrator::integrate(dt, callback)
+ * (include/HydroGPU/Integrator/Integrator.h:13-17) keeps the Runge-Kutta
+ * combination on the host (RungeKutta.h:59-125, ForwardEuler.cpp:8-32) and
+ * calls back into the solver for each derivative (Roe::step / calcDeriv,
+ * src/Solver/Roe.cpp:94-112; HLL::step, src/Solver/HLL.cpp:234-241).  These
+ * entry points give such an integrator the same pieces: a derivative of the
+ * current state, and the buffer operations it combines stages with.  Buffers
+ * are device arrays in the simulation's storage layout (N state planes of
+ * `volume` elements of hydro_sim_element_size bytes, ghost cells included),
+ * owned by the simulation.  Plain storage only (not HYDRO_FLAG_KAHAN).
+ * hydro_sim_step / update / run remain the fused path for the same solvers.
+ * Order: hydro_sim_boundary, hydro_sim_calc_timestep, hydro_sim_init_step,
+ * then the integrator's stages (calc_deriv, copy, zero, mult_add on the
+ * state buffer), then hydro_sim_finish_step. */
+
+/* Bytes per element of the simulation's buffers: 8 (FP64) or 4 (FP32_MIXED). */
+size_t hydro_sim_element_size(const hydro_sim* sim);
+
+/* The coefficients alphas(i,k), betas(i,k) of an integrator
+ * (include/HydroGPU/Integrator/RungeKutta.h:129-294; ForwardEuler: 1 stage,
+ * alpha = beta = 1): *order stages, alpha[4*i+k], beta[4*i+k] (4x4 each). */
+int hydro_integrator_tableau(int integrator, int* order, double* alpha, double* beta);
+
+/* A zeroed device buffer of one state's size (Solver::CL::alloc); released
+ * with hydro_sim_buffer_destroy or when the simulation is destroyed. */
+int hydro_sim_buffer_create(hydro_sim* sim, void** buf);
+int hydro_sim_buffer_destroy(hydro_sim* sim, void* buf);
+
+/* The simulation's state buffer itself (Solver::stateBuffer).  Valid until
+ * the next fused update (hydro_sim_step / update / run), which may move the
+ * state to the other ping-pong buffer. */
+int hydro_sim_state_buffer(hydro_sim* sim, void** buf);
+
+/* Solver::initStep for a host-side step: freezes the start-of-step data
+ * (Roe eigenbasis, HLL wave speeds, SRHD primitives) of the current state. */
+int hydro_sim_init_step(hydro_sim* sim);
+
+/* Roe::calcDeriv / HLL::calcDeriv (src/Solver/Roe.cpp:101-112,
+ * res/CalcFluxDeriv.cl:7-44): deriv += -(flux differences)/dx over the
+ * interior cells for the state buffer's current contents, with the frozen
+ * data of hydro_sim_init_step and the full-step dt in the limiter
+ * (res/Roe.cl:206, 292).  Accumulates, as the reference's kernel does: the
+ * integrator zeroes `deriv` first.  `deriv` must come from
+ * hydro_sim_buffer_create. */
+int hydro_sim_calc_deriv(hydro_sim* sim, double dt, void* deriv);
+
+/* enqueueCopyBuffer, Solver::CL::zero (src/Solver/Solver.cpp:22-32) and
+ * multAdd a = b + c * k (res/Common.cl:201-210) on the simulation's buffers
+ * (created ones or the state buffer), in stream order. */
+int hydro_sim_buffer_copy(hydro_sim* sim, void* dst, const void* src);
+int hydro_sim_buffer_zero(hydro_sim* sim, void* buf);
+int hydro_sim_mult_add(hydro_sim* sim, void* a, const void* b, const void* c, double k);
+
+/* Ends a host-side step: advances the simulated time by dt and the step
+ * count, and runs the solver's tail (SRHD constrainState + updatePrimitives,
+ * src/Solver/SRHDRoe.cpp:62-73). */
+int hydro_sim_finish_step(hydro_sim* sim, double dt);
+
 /* ---- many uneven simulations in one launch ------------------------------ */
 
 /* A batch of independent Euler-Roe grids (1D or 2D, any mix of sizes) that

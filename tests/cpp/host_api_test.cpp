@@ -235,6 +235,42 @@ static int run_gpu() {
     };
     compare(app, 5, 1e-10, "MaxwellRoe wave 128 RK4");
   }
+  // the reference's host-side form, Integrator::integrate(dt, callback) with
+  // Solver::calcDeriv as the callback (Roe::step, src/Solver/Roe.cpp:94-99),
+  // for a Runge-Kutta tableau and for ForwardEuler
+  for (const char* integ : {"RungeKutta4", "ForwardEuler"}) {
+    HydroGPU::HydroGPUApp app;
+    app.solverName = "EulerRoe";
+    app.integratorName = integ;
+    app.slopeLimiter = "MonotizedCentral";
+    app.dim = 2;
+    app.size = {{48, 40, 1, 1}};
+    for (int a = 0; a < 2; ++a)
+      for (int m = 0; m < 2; ++m) app.boundaryMethods[a][m] = HydroGPU::BOUNDARY_METHOD_PERIODIC;
+    app.initState = kh;
+    auto solver = app.createSolver();
+    oracle_sim* o = oracle_create(&solver->desc(), 0);
+    std::vector<real> q0 = initial_state(app, *solver);
+    oracle_set_state(o, q0.data(), q0.size());
+    for (int i = 0; i < 3; ++i) {
+      solver->boundary();
+      double dt = 0;
+      HydroGPU::checkStatus(hydro_sim_calc_timestep(solver->handle(), &dt), "calc_timestep");
+      solver->integrator->integrate(dt, [&](void* deriv) { solver->calcDeriv(deriv, dt); });
+      const double dto = oracle_update(o);
+      CHECK(std::fabs(dt - dto) <= 1e-12 * dto, "%s callback form: dt %.17g vs %.17g", integ, dt, dto);
+    }
+    // the oracle's ForwardEuler ends with the zero-gravity pass's boundary
+    solver->boundary();
+    oracle_boundary(o);
+    std::vector<real> ref(q0.size());
+    oracle_get_state(o, ref.data(), ref.size());
+    const real e = relerr(solver->getState(), ref);
+    CHECK(e < 1e-10, "%s callback form: rel err %.3e", integ, e);
+    CHECK(solver->getSteps() == 3, "%s callback form: steps %lld", integ, solver->getSteps());
+    std::printf("%-34s %4d updates  rel err %.2e\n", (std::string(integ) + " integrate(dt, callback)").c_str(), 3, e);
+    oracle_destroy(o);
+  }
   std::printf("%s gpu checks\n", failures ? "FAILED" : "passed");
   return failures ? 1 : 0;
 }
