@@ -132,6 +132,21 @@ def test_blast_3d_full_size_properties(gpu_lib):
     assert noise_free < 1e-3  # only the 1e-4 seeded density noise breaks the symmetry
 
 
+def test_blast_3d_full_size_one_update_matches_oracle(gpu_lib):
+    # the 3D HLLC workload at its full size: one FP64 update (boundary, CFL,
+    # three TVD-RK3 stages) against the oracle, 1e-11 relative
+    kw = dict(xmin=[-1] * 3, xmax=[1] * 3, solverName="EulerHLLC", integratorName="RungeKutta3TVD",
+              slopeLimiter="MinMod", boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4)
+    d = config.make_desc([256] * 3, **kw)
+    q = ics.blast(d)
+    g = _mk(d)
+    g.set_state(q)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    assert g.update() == pytest.approx(o.update(), rel=1e-12)
+    assert _relerr(g.get_state(), o.get_state()) < 1e-11
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
 def test_3d_first_order_decomposition_invariant(gpu_lib, precision, monkeypatch):
     # the y-chunk height and the workgroup shape change which waves compute

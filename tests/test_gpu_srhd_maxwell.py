@@ -75,6 +75,29 @@ def test_srhd_mixed_quadrants(gpu_lib):
     assert g.health() == 0
 
 
+def test_srhd_full_size_quadrants(gpu_lib):
+    # the SRHD workload at its full size (1024^2 quadrants, RK2, gamma 5/3):
+    # one FP64 update against the oracle (state and primitives, 1e-9 as
+    # above), and the FP32-mixed path against the FP64 one (1e-3)
+    kw = dict(xmin=[0.0] * 2, xmax=[1.0] * 2, solverName="SRHDRoe", integratorName="RungeKutta2",
+              slopeLimiter="Superbee", boundaryMethods=[{"min": "FREEFLOW", "max": "FREEFLOW"}] * 2,
+              defs={"idealGas_heatCapacityRatio": 5.0 / 3.0})
+    d64 = config.make_desc([1024, 1024], **kw)
+    d32 = config.make_desc([1024, 1024], precision="fp32_mixed", **kw)
+    q = ics.srhd_quadrants(d64)
+    g, m = _mk(d64), _mk(d32)
+    g.set_state(q)
+    m.set_state(q)
+    o = oracle.OracleSim(d64)
+    o.set_state(q)
+    assert g.update() == pytest.approx(o.update(), rel=1e-10)
+    m.update()
+    assert _relerr(g.get_state(), o.get_state()) < 1e-9
+    assert _relerr(g.get_primitives(), o.get_primitives()) < 1e-9
+    assert _relerr(m.get_state(), g.get_state()) < 1e-3
+    assert g.health() == 0 and m.health() == 0
+
+
 MAXWELL_CASES = [
     ([128], "RungeKutta2", "Superbee", "FREEFLOW"),
     ([70, 40], "RungeKutta4", "MonotizedCentral", "PERIODIC"),
