@@ -385,6 +385,53 @@ void launch_zero_words(void* p, long long nwords, hipStream_t st) {
   hipLaunchKernelGGL(zero_words_kernel, dim3(stride_blocks(nwords)), dim3(256), 0, st, (uint32_t*)p, nwords);
 }
 
+// ---------------------------------------------------------------- layouts
+// The reference's array of structures state[s + N*cell] (FP64, what
+// Converter::toGPU / fromGPU move, src/Solver/Solver.cpp:333-343) to and from
+// the state planes of the storage type: FP32 storage takes hi = float(v) and,
+// when compensated, lo = float(v - hi); reading back gives hi + lo in FP64.
+template <typename S>
+__global__ void __launch_bounds__(256) aos_to_planes_kernel(const double* aos, int n, long long vol, long long plane,
+                                                            S* hi, float* lo) {
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < vol; c += (long long)gridDim.x * 256) {
+    for (int k = 0; k < n; ++k) {
+      const double v = aos[k + (long long)n * c];
+      const S h = S(v);
+      hi[k * plane + c] = h;
+      if (lo) lo[k * plane + c] = float(v - double(h));
+    }
+  }
+}
+template <typename S>
+__global__ void __launch_bounds__(256) planes_to_aos_kernel(const S* hi, const float* lo, int n, long long vol,
+                                                            long long plane, double* aos) {
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < vol; c += (long long)gridDim.x * 256) {
+    for (int k = 0; k < n; ++k) {
+      double v = double(hi[k * plane + c]);
+      if (lo) v += double(lo[k * plane + c]);
+      aos[k + (long long)n * c] = v;
+    }
+  }
+}
+void launch_aos_to_planes_f64(const double* aos, int n, long long vol, long long plane, double* u, hipStream_t st) {
+  hipLaunchKernelGGL((aos_to_planes_kernel<double>), dim3(stride_blocks(vol)), dim3(256), 0, st, aos, n, vol, plane,
+                     u, (float*)nullptr);
+}
+void launch_aos_to_planes_f32(const double* aos, int n, long long vol, long long plane, float* hi, float* lo,
+                              hipStream_t st) {
+  hipLaunchKernelGGL((aos_to_planes_kernel<float>), dim3(stride_blocks(vol)), dim3(256), 0, st, aos, n, vol, plane,
+                     hi, lo);
+}
+void launch_planes_to_aos_f64(const double* u, int n, long long vol, long long plane, double* aos, hipStream_t st) {
+  hipLaunchKernelGGL((planes_to_aos_kernel<double>), dim3(stride_blocks(vol)), dim3(256), 0, st, u,
+                     (const float*)nullptr, n, vol, plane, aos);
+}
+void launch_planes_to_aos_f32(const float* hi, const float* lo, int n, long long vol, long long plane, double* aos,
+                              hipStream_t st) {
+  hipLaunchKernelGGL((planes_to_aos_kernel<float>), dim3(stride_blocks(vol)), dim3(256), 0, st, hi, lo, n, vol,
+                     plane, aos);
+}
+
 template <typename S>
 static void con2prim(const Con2PrimArgs& a, S* u, float* lo, double* prim, hipStream_t st) {
   dim3 g((a.sx + 255) / 256, a.dim > 1 ? a.sy : 1, a.dim > 2 ? a.sz : 1), b(256);

@@ -74,6 +74,9 @@ struct hydro_sim {
   // and whether U[cur ^ 1] holds the frozen start-of-step state
   std::vector<void*> user_bufs;
   bool frozen = false;
+  // device staging of the reference's FP64 array-of-structures state, for
+  // the layout change on the GPU (allocated on first use)
+  double* aos_dev = nullptr;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
@@ -612,6 +615,7 @@ void hydro_sim_destroy(hydro_sim* s) {
   for (int j = 0; j < kMaxPartials; ++j)
     if (s->P[j]) hipFree(s->P[j]);
   for (void* p : s->user_bufs) hipFree(p);
+  if (s->aos_dev) hipFree(s->aos_dev);
   if (s->prim) hipFree(s->prim);
   if (s->blockmin) hipFree(s->blockmin);
   if (s->partial) hipFree(s->partial);
@@ -737,10 +741,32 @@ void* hydro_sim_stream(hydro_sim* s) { return s ? (void*)s->stream : nullptr; }
 // AoS (reference layout) -> the device planes of the current state buffer;
 // FP32 storage keeps the residual as the low word when compensated, or
 // takes the low words given (`low`, compensated storage only)
+static int ensure_aos_staging(hydro_sim* s) {
+  if (!s->aos_dev) HIP_TRY(hipMalloc((void**)&s->aos_dev, sizeof(double) * (size_t)s->N * (size_t)s->vol));
+  return HYDRO_OK;
+}
+
 static int upload_state(hydro_sim* s, const double* aos, const float* low = nullptr) {
   s->frozen = false;
   const int N = s->N;
   const long long V = s->vol;
+  if (!low) {
+    // the array of structures goes over as it is; the planes are built on the GPU
+    int rc = ensure_aos_staging(s);
+    if (rc) return rc;
+    // in stream order: the layout kernel reads the staging after the copy
+    HIP_TRY(hipMemcpyAsync(s->aos_dev, aos, sizeof(double) * (size_t)N * (size_t)V, hipMemcpyHostToDevice,
+                           s->stream));
+    if (s->mixed)
+      launch_aos_to_planes_f32(s->aos_dev, N, V, s->plane, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr,
+                               s->stream);
+    else
+      launch_aos_to_planes_f64(s->aos_dev, N, V, s->plane, (double*)s->U[s->cur], s->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->ghosts_pending = false;
+    return HYDRO_OK;
+  }
   if (s->mixed) {
     std::vector<float> hi((size_t)s->plane * N, 0.f), lo((size_t)s->plane * N, 0.f);
     for (long long c = 0; c < V; ++c)
@@ -837,28 +863,20 @@ int hydro_sim_get_state(hydro_sim* s, double* aos, size_t n) {
     const int rc = materialize_ghosts(s);
     if (rc) return rc;
   }
-  HIP_TRY(hipStreamSynchronize(s->stream));
   const int N = s->N;
   const long long V = s->vol;
-  if (s->mixed) {
-    std::vector<float> hi((size_t)s->plane * N), lo;
-    HIP_TRY(hipMemcpy(hi.data(), s->U[s->cur], hi.size() * sizeof(float), hipMemcpyDeviceToHost));
-    if (s->kahan) {
-      lo.resize(hi.size());
-      HIP_TRY(hipMemcpy(lo.data(), s->Ulo[s->cur], lo.size() * sizeof(float), hipMemcpyDeviceToHost));
-    }
-    for (long long c = 0; c < V; ++c)
-      for (int k = 0; k < N; ++k) {
-        double v = hi[(size_t)k * s->plane + c];
-        if (s->kahan) v += (double)lo[(size_t)k * s->plane + c];
-        aos[k + (size_t)N * c] = v;
-      }
-  } else {
-    std::vector<double> buf((size_t)s->plane * N);
-    HIP_TRY(hipMemcpy(buf.data(), s->U[s->cur], buf.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (long long c = 0; c < V; ++c)
-      for (int k = 0; k < N; ++k) aos[k + (size_t)N * c] = buf[(size_t)k * s->plane + c];
-  }
+  // the planes become the array of structures on the GPU (FP64: hi + lo for
+  // compensated storage), then one copy back
+  int rc = ensure_aos_staging(s);
+  if (rc) return rc;
+  if (s->mixed)
+    launch_planes_to_aos_f32((const float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, N, V, s->plane,
+                             s->aos_dev, s->stream);
+  else
+    launch_planes_to_aos_f64((const double*)s->U[s->cur], N, V, s->plane, s->aos_dev, s->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(aos, s->aos_dev, sizeof(double) * (size_t)N * (size_t)V, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
   return HYDRO_OK;
 }
 

@@ -150,6 +150,16 @@ void launch_mult_add_f64(double* a, const double* b, const double* c, double k, 
 void launch_mult_add_f32(float* a, const float* b, const float* c, float k, long long n, hipStream_t st);
 void launch_zero_words(void* p, long long nwords, hipStream_t st);
 
+// the reference's FP64 array of structures aos[s + n*cell] (a device
+// buffer) to / from the state planes (FP32: hi, and the low words when lo
+// is not null)
+void launch_aos_to_planes_f64(const double* aos, int n, long long vol, long long plane, double* u, hipStream_t st);
+void launch_aos_to_planes_f32(const double* aos, int n, long long vol, long long plane, float* hi, float* lo,
+                              hipStream_t st);
+void launch_planes_to_aos_f64(const double* u, int n, long long vol, long long plane, double* aos, hipStream_t st);
+void launch_planes_to_aos_f32(const float* hi, const float* lo, int n, long long vol, long long plane, double* aos,
+                              hipStream_t st);
+
 // FP64 interior sums, per block: partial[b * n + s]
 void launch_totals_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* u,
                        double* partial, int nblocks, hipStream_t st);
