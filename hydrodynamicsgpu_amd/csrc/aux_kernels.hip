@@ -12,10 +12,13 @@
 namespace hydro {
 
 template <typename S>
-__global__ void __launch_bounds__(256) boundary_kernel(BoundaryArgs b, S* planes, float* lo) {
+__global__ void __launch_bounds__(256) boundary_kernel(BoundaryArgs b, S* planes, float* lo, double* prim) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ghost_count(b.dim, b.sx, b.sy, b.sz)) return;
   boundary_cell<S>(b, planes, lo, g);
+  // SRHDRoe::boundary fills the primitives with the same maps
+  // (src/Solver/SRHDRoe.cpp:79-96); same launch, independent planes
+  if (prim) boundary_cell<double>(b, prim, nullptr, g);
 }
 
 template <typename S>
@@ -35,14 +38,15 @@ static long long host_ghost_count(int dim, int sx, int sy, int sz) {
   return 4LL * sx * sy + 4LL * sx * (sz - 4) + 4LL * (sy - 4) * (sz - 4);
 }
 
-void launch_boundary_f64(const BoundaryArgs& b, double* planes, hipStream_t st) {
+void launch_boundary_f64(const BoundaryArgs& b, double* planes, double* prim, hipStream_t st) {
   const long long g = host_ghost_count(b.dim, b.sx, b.sy, b.sz);
   hipLaunchKernelGGL((boundary_kernel<double>), dim3((unsigned)((g + 255) / 256)), dim3(256), 0, st, b, planes,
-                     (float*)nullptr);
+                     (float*)nullptr, prim);
 }
-void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, hipStream_t st) {
+void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, double* prim, hipStream_t st) {
   const long long g = host_ghost_count(b.dim, b.sx, b.sy, b.sz);
-  hipLaunchKernelGGL((boundary_kernel<float>), dim3((unsigned)((g + 255) / 256)), dim3(256), 0, st, b, planes, lo);
+  hipLaunchKernelGGL((boundary_kernel<float>), dim3((unsigned)((g + 255) / 256)), dim3(256), 0, st, b, planes, lo,
+                     prim);
 }
 
 void launch_copy_ghosts_f64(int dim, int sx, int sy, int sz, long long plane, int n, const double* src,
@@ -325,11 +329,28 @@ __global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, flo
   const long long cell = x + (long long)a.sx * (y + (long long)a.sy * z);
   const SRHDDefs defs{a.gamma, a.max_iter, a.stop_eps, a.vel_eps, a.pmin_eps, a.rho_min, a.rho_max,
                       a.eint_max, a.D_min, a.D_max, a.tau_min, a.tau_max};
+  // ghost cells (not written by the stages) first take the start-of-step
+  // values when the caller defers that copy to here
+  bool ghost = x < 2 || x > a.sx - 3;
+  if (DIM > 1) ghost = ghost || y < 2 || y > a.sy - 3;
+  if (DIM > 2) ghost = ghost || z < 2 || z > a.sz - 3;
+  const bool copy = ghost && a.ghost_src;
+  const S* src = copy ? (const S*)a.ghost_src : u;
+  const float* srclo = copy ? a.ghost_src_lo : lo;
   double q[N];
 #pragma unroll
   for (int s = 0; s < N; ++s) {
-    q[s] = double(u[s * a.plane + cell]);
-    if (lo) q[s] += double(lo[s * a.plane + cell]);
+    const S h = src[s * a.plane + cell];
+    q[s] = double(h);
+    float l = 0.f;
+    if (lo) {
+      l = srclo[s * a.plane + cell];
+      q[s] += double(l);
+    }
+    if (copy) {
+      u[s * a.plane + cell] = h;
+      if (lo) lo[s * a.plane + cell] = l;
+    }
   }
   const double D = fmin(fmax(q[0], a.D_min), a.D_max);
   const double tau = fmin(fmax(q[E], a.tau_min), a.tau_max);

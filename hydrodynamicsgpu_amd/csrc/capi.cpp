@@ -156,11 +156,12 @@ int enqueue_boundary(hydro_sim* s) {
     if (rc) return rc;
   }
   const BoundaryArgs b = boundary_args(s);
-  if (s->mixed) launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->stream);
-  else launch_boundary_f64(b, (double*)s->U[s->cur], s->stream);
   // SRHDRoe::boundary fills the primitives with the same kernels afterwards
-  // (src/Solver/SRHDRoe.cpp:79-96)
-  if (s->prim) launch_boundary_f64(b, s->prim, s->stream);
+  // (src/Solver/SRHDRoe.cpp:79-96): the same launch here
+  if (s->mixed)
+    launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->stream);
+  else
+    launch_boundary_f64(b, (double*)s->U[s->cur], s->prim, s->stream);
   HIP_TRY(hipGetLastError());
   return HYDRO_OK;
 }
@@ -384,6 +385,13 @@ int enqueue_srhd_tail(hydro_sim* s) {
   c.tau_min = s->d.srhd_tau_min;
   c.tau_max = s->d.srhd_tau_max;
   c.health = &s->sc->health;
+  // constrainState touches every cell, ghosts included: the deferred ghost
+  // copy from the start-of-step state happens in the same launch
+  if (s->ghosts_pending) {
+    c.ghost_src = s->U[s->cur ^ 1];
+    c.ghost_src_lo = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
+    s->ghosts_pending = false;
+  }
   if (s->mixed) launch_con2prim_f32(c, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->stream);
   else launch_con2prim_f64(c, (double*)s->U[s->cur], s->prim, s->stream);
   HIP_TRY(hipGetLastError());
@@ -423,12 +431,7 @@ int enqueue_stages(hydro_sim* s) {
   // ghost cells of the new state are those of u0 (the RK combination leaves
   // them unchanged: their derivative is zero); copied when first read
   s->cur ^= 1;
-  s->ghosts_pending = true;
-  // SRHD's constrainState touches every cell, ghosts included
-  if (s->d.equation == HYDRO_EQN_SRHD) {
-    const int rc = materialize_ghosts(s);
-    if (rc) return rc;
-  }
+  s->ghosts_pending = true;  // SRHD: copied inside the con2prim launch below
   // ForwardEuler's zero-gravity pass refreshes the ghosts after the update
   // (src/Solver/SelfGravitation.cpp:122 + src/Integrator/ForwardEuler.cpp:29-31)
   if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER &&

@@ -38,8 +38,9 @@ struct BoundaryArgs {
   signed char method[3][2];
   unsigned negmask[3][2];
 };
-void launch_boundary_f64(const BoundaryArgs& b, double* planes, hipStream_t st);
-void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, hipStream_t st);
+// prim: SRHD primitive planes filled by the same launch, or null
+void launch_boundary_f64(const BoundaryArgs& b, double* planes, double* prim, hipStream_t st);
+void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, double* prim, hipStream_t st);
 
 // copy ghost cells of every plane from src to dst (the cells the stage
 // kernels do not write)
@@ -73,6 +74,10 @@ struct Con2PrimArgs {
   double rho_min, rho_max, eint_max;
   double D_min, D_max, tau_min, tau_max;
   unsigned* health;
+  // when set: the state planes (and low words) whose ghost cells the new
+  // state takes before the clamp (the deferred ghost copy, in this launch)
+  const void* ghost_src;
+  const float* ghost_src_lo;
 };
 void launch_con2prim_f64(const Con2PrimArgs& a, double* u, double* prim, hipStream_t st);
 void launch_con2prim_f32(const Con2PrimArgs& a, float* u, float* lo, double* prim, hipStream_t st);
