@@ -1,10 +1,14 @@
 // capi.cpp -- C ABI of the MI355X update path (include/hydro_capi.h).
 //
 // Owns the device buffers of one grid and sequences the kernels of a step:
-//   boundary (per axis) -> [HLL: frozen wave speeds] -> CFL per-block minima ->
-//   finalise dt on the device -> S fused RK stages -> ghost copy.
-// dt never leaves the device on the run() path; the only host sync is the
-// one a caller asks for (update() returns dt, get_state, totals).
+//   boundary (every ghost cell, one launch) -> CFL minima per wave ->
+//   finalise dt on the device -> S fused RK stages (frozen start-of-step
+//   data recomputed per stage) -> [SRHD: constrain + con2prim].
+// The new state's ghost cells are copied only when something reads them.
+// dt never leaves the device on the run() path (captured graphs, or one
+// single-workgroup launch for small 1D grids); the only host sync is the
+// one a caller asks for (update() returns dt, get_state, totals).  The
+// generic entry points at the end serve host-side integrators.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
