@@ -67,6 +67,7 @@ struct hydro_sim {
   // stage decomposition
   int rows = 32, wpb = 4;
   int zwaves = 8;  // first-order 3D kernel: waves per workgroup
+  int zshare = 0;  // first-order 3D kernel: modes whose rows go through LDS (bit per CombineMode)
   // measurement: event pairs around fused stage launches
   bool profiling = false;
   // two captured updates per starting parity (ping-pong state), replayed by run()
@@ -342,6 +343,7 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
     a.rows = s->rows;
     a.nchunks = (s->sz[1] - 4 + a.rows - 1) / a.rows;
     a.zwaves = s->zwaves;
+    a.zshare = s->zshare;
     const int nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
     a.nwaves = a.nstrips * a.nchunks * nzb;
   } else {
@@ -681,6 +683,11 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
     s->zwaves = stage3d_r1_waves(!s->mixed);
     if (const char* e = getenv("HYDRO_3D_ZWAVES")) s->zwaves = atoi(e) == 4 ? 4 : 8;
+    // rows shared through LDS: measured faster for the FP64 first stage only
+    // (256^3 HLLC blast: 794 vs 891 us; FP32 and the later FP64 stages hold
+    // more registers than they save loads)
+    s->zshare = s->mixed ? 0 : (1 << CM_FIRST) | (1 << CM_SINGLE);
+    if (const char* e = getenv("HYDRO_3D_ZSHARE")) s->zshare = (int)strtol(e, nullptr, 0);
     const long long nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
     s->rows = pick_rows_3d_r1(strips * nzb, s->sz[1] - 4, ncu * (8 / s->zwaves));
   } else if (s->dim == 3) {

@@ -5,7 +5,9 @@
 // per wave; LDS exchange) and marches a chunk of z-planes with the z stencil
 // held in registers.  The first and last 2 rows of the workgroup are halo
 // rows that compute only what their neighbours need.
-// HLL/HLLC (first order, radius 1): stage3d_r1_kernel below.
+// HLL/HLLC (first order, radius 1): stage3d_r1_kernel below, and its
+// variant stage3d_r1s_kernel that shares the rows of a workgroup's planes
+// through LDS (chosen per combination mode).
 // Kernel arithmetic and reference semantics are those of the 2D kernel
 // (stage2d_impl.hpp).
 #include "device_common.hpp"
@@ -380,10 +382,203 @@ __global__ void __launch_bounds__(64 * W) stage3d_r1_kernel(StageArgs<T, S> a) {
   step(y1, false, 0);
 }
 
+// The same decomposition with the rows shared through LDS: each wave builds
+// its own row (stage state and frozen data) once and publishes it; the wave
+// above takes it as the lower neighbour of its z-face instead of loading and
+// building the plane below itself (only wave 0 loads the plane beneath the
+// workgroup).  The z-face flux then exists only after the row barrier, so a
+// wave hands it to the wave below one row later: row r-1 is completed at
+// step r, as in stage3d_r1_kernel, with the same arithmetic in the same
+// order (bit-identical results).  Loads and frozen-data builds per workgroup
+// and row: W + 1 instead of 2W.
+template <class Pol, typename T, typename S, bool KAHAN, int MODE, int W>
+__global__ void __launch_bounds__(64 * W) stage3d_r1s_kernel(StageArgs<T, S> a) {
+  constexpr int N = Pol::N;
+  constexpr int NS = Pol::NS, NF = Pol::NF;
+  using Frozen = typename Pol::Frozen;
+  using Comb = StageCombine<T, S, KAHAN, N, MODE>;
+  __shared__ T shr[2][W - 1][NS + NF][64];  // published rows of waves 0..W-2, by row parity
+  __shared__ T shz[2][W][N][64];            // lower z-face fluxes, by row parity
+
+  const int nb = gridDim.x;
+  const int per = nb >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (lb >= a.nwaves) return;  // nwaves counts workgroups here (uniform per block)
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int sx = a.sx, sy = a.sy, sz = a.sz;
+  const int nzb = (sz - 4 + W - 2) / (W - 1);
+  const int zblk = lb % nzb;
+  const int rest = lb / nzb;
+  const int strip = rest % a.nstrips;
+  const int ych = rest / a.nstrips;
+  const int p = 2 + zblk * (W - 1) + w;      // this wave's plane
+  const bool ow = w < W - 1 && p <= sz - 3;  // output plane (wave-uniform)
+  const bool zw = p <= sz - 2;               // its lower face is read by an output wave
+  const long long psz = (long long)sx * sy;
+  const int c = strip * kStripOut + lane;
+  const uint32_t cl = c < sx ? c : sx - 1;
+  const bool out_lane = lane >= 2 && lane < 62 && c <= sx - 3;
+
+  const PhysParams<T> pp = make_params<T>(a);
+  const T gamma = pp.gamma;
+  const double dtd = *a.dt;
+  const T idx = T(a.inv_dx[0]), idy = T(a.inv_dx[1]), idz = T(a.inv_dx[2]);
+  const Comb comb(a, dtd);
+
+  struct Raw {
+    S q0[N];
+    T dk[N];
+  };
+  using Stage = typename Pol::Stage;
+  struct Row {
+    Stage st;
+    Frozen fz;
+  };
+  auto fetch = [&](int plane, int row, Raw& r) {
+    const long long e = (long long)__builtin_amdgcn_readfirstlane(plane) * psz +
+                        (long long)__builtin_amdgcn_readfirstlane(row) * sx;
+    comb.load_u0(e, cl, r.q0);
+    comb.load_dk(e, cl, r.dk);
+  };
+  auto build = [&](const Raw& r, Row& R) {
+    T u[N];
+    comb.stage_state(r.q0, r.dk, u);
+    R.st = Pol::stage(u, gamma);
+    R.fz = Pol::template frozen<S>(r.q0, gamma);
+  };
+  auto flux = [&](const Row& L, const Row& R, int side, T F[N]) {
+    const auto fa = Pol::face_a(L.fz, R.fz, L.st, R.st, side, true, pp);
+    Pol::face_b(fa, nullptr, nullptr, T(0), side, 0, F);
+  };
+  auto xdiv = [&](const Row& R, T div[N]) {
+    T ps[NS], pk[NF], pkL[NF];
+    Pol::pack_stage(R.st, ps);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) ps[i] = lane_from_left(ps[i]);
+    Pol::pack(R.fz, pk);
+#pragma unroll
+    for (int i = 0; i < NF; ++i) pkL[i] = lane_from_left(pk[i]);
+    Row L;
+    L.st = Pol::unpack_stage(ps);
+    L.fz = Pol::unpack(pkL);
+    T F[N];
+    flux(L, R, 0, F);
+#pragma unroll
+    for (int s = 0; s < N; ++s) div[s] = (F[s] - lane_from_right(F[s])) * idx;
+  };
+
+  const int pown = p < sz ? p : sz - 1;  // clamped load planes (idle waves past the grid)
+  const int plow = p - 1 < sz ? p - 1 : sz - 1;
+  const int y0 = 2 + ych * a.rows;
+  int y1 = y0 + a.rows;
+  if (y1 > sy - 2) y1 = sy - 2;
+
+  // window: P = own row r-1 and its loads w1; Gp = flux through face r-1;
+  // xp = x-divergence of row r-1; Fzp = flux through row r-1's lower z-face
+  Row P;
+  Raw w1, nx, nz;
+  T Gp[N], xp[N], Fzp[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) Gp[s] = xp[s] = Fzp[s] = T(0);
+  if (ow) {
+    fetch(pown, y0 - 1, w1);
+    build(w1, P);
+  }
+  if (zw) {
+    fetch(pown, y0, nx);
+    if (w == 0) fetch(plow, y0, nz);
+  }
+  // one row step, one barrier.  zstep: row r is an output row (its z-face
+  // and x-face run); otherwise (r = y1) face r only closes row r-1.
+  auto step = [&](int r, bool zstep, int buf) {
+    const Raw cur = nx, curz = nz;
+    if (zw && zstep) {  // prefetch row r+1 (<= y1 <= sy-2)
+      fetch(pown, r + 1, nx);
+      if (w == 0 && r + 1 < y1) fetch(plow, r + 1, nz);
+    }
+    const long long eo = (long long)__builtin_amdgcn_readfirstlane(pown) * psz +
+                         (long long)__builtin_amdgcn_readfirstlane(r - 1) * sx;
+    constexpr bool kLateFin = MODE == CM_FINAL || MODE == CM_GENERIC;
+    typename Comb::Fin fin;
+    if (!kLateFin && ow && r > y0) comb.fetch_fin_rest(eo, cl, w1.q0, w1.dk, fin);
+    Row C;
+    if (zw) build(cur, C);
+    if (zstep && zw && w < W - 1) {  // publish the row for the wave above
+      T pk[NS + NF];
+      Pol::pack_stage(C.st, pk);
+      Pol::pack(C.fz, pk + NS);
+#pragma unroll
+      for (int i = 0; i < NS + NF; ++i) shr[buf][w][i][lane] = pk[i];
+    }
+    T G[N], xd[N];
+    if (ow) flux(P, C, 1, G);  // face r: rows r-1 | r
+    if (zstep && ow) xdiv(C, xd);
+    __syncthreads();
+    if (ow) {
+      if (r > y0 && out_lane) {
+        // row r-1: its upper z-face is the lower face the wave above computed
+        // one step ago
+        T acc[N], deriv[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) {
+          acc[s] = xp[s];
+          acc[s] += (Fzp[s] - shz[buf ^ 1][w + 1][s][lane]) * idz;
+          deriv[s] = acc[s] + (Gp[s] - G[s]) * idy;
+        }
+        if (kLateFin) comb.fetch_fin_rest(eo, cl, w1.q0, w1.dk, fin);
+        comb.finish(eo, c, fin, deriv);
+      }
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        Gp[s] = G[s];
+        xp[s] = xd[s];
+      }
+      P = C;
+      w1 = cur;
+    }
+    if (zstep && zw) {
+      // lower z-face of row r: the plane below is the wave beneath's row
+      // (wave 0: loaded)
+      Row Z;
+      if (w == 0) {
+        build(curz, Z);
+      } else {
+        T pk[NS + NF];
+#pragma unroll
+        for (int i = 0; i < NS + NF; ++i) pk[i] = shr[buf][w - 1][i][lane];
+        Z.st = Pol::unpack_stage(pk);
+        Z.fz = Pol::unpack(pk + NS);
+      }
+      T Fz[N];
+      flux(Z, C, 2, Fz);
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        shz[buf][w][s][lane] = Fz[s];
+        Fzp[s] = Fz[s];
+      }
+    }
+  };
+  // unrolled by two so the LDS buffer index is static
+  int r = y0;
+#pragma unroll 1
+  for (; r + 1 < y1; r += 2) {
+    step(r, true, 0);
+    step(r + 1, true, 1);
+  }
+  if (r < y1) {
+    step(r, true, 0);
+    ++r;
+  }
+  step(y1, false, (y1 - y0) & 1);
+}
+
 template <class Pol, typename T, typename S, bool KAHAN, int MODE>
 static void launch3_r1_mode(const StageArgs<T, S>& a, hipStream_t st) {
   const int blocks = (a.nwaves + 7) & ~7;  // multiple of 8 for the XCD remap
-  if (a.zwaves == 4)
+  if (a.zwaves == 4 && ((a.zshare >> MODE) & 1))
+    hipLaunchKernelGGL((stage3d_r1s_kernel<Pol, T, S, KAHAN, MODE, 4>), dim3(blocks), dim3(64 * 4), 0, st, a);
+  else if (a.zwaves == 4)
     hipLaunchKernelGGL((stage3d_r1_kernel<Pol, T, S, KAHAN, MODE, 4>), dim3(blocks), dim3(64 * 4), 0, st, a);
   else
     hipLaunchKernelGGL((stage3d_r1_kernel<Pol, T, S, KAHAN, MODE, 8>), dim3(blocks), dim3(64 * 8), 0, st, a);

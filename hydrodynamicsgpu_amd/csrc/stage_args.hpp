@@ -57,6 +57,7 @@ struct StageArgs {
   int nstrips, rows, nchunks;  // wave -> (strip, chunk) decomposition
   int nwaves;
   int zwaves;                // first-order 3D kernel: waves (z-planes) per workgroup
+  int zshare;                // first-order 3D kernel: bit MODE set -> rows shared through LDS (4 waves)
 };
 
 }  // namespace hydro

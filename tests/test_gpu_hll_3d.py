@@ -149,18 +149,21 @@ def test_blast_3d_full_size_one_update_matches_oracle(gpu_lib):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
 def test_3d_first_order_decomposition_invariant(gpu_lib, precision, monkeypatch):
-    # the y-chunk height and the workgroup shape change which waves compute
-    # the chunk- and block-edge faces, not their arithmetic: the results are
-    # bit-identical
+    # the y-chunk height, the workgroup shape and whether the plane below
+    # comes through LDS (rows shared between the waves of a workgroup)
+    # change which waves compute the chunk- and block-edge faces, not their
+    # arithmetic: the results are bit-identical
     from hydrodynamicsgpu_amd import capi
 
     d = config.make_desc([70, 45, 25], solverName="EulerHLLC", integratorName="RungeKutta3TVD",
                          boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4, precision=precision)
     q = _rand_state(d, 5)
     out = []
-    # (rows per chunk, waves per workgroup)
-    for rows, zw in ((None, None), ("8", None), ("41", None), (None, "8"), ("13", "8")):
-        for k, v in (("HYDRO_ROWS_PER_WAVE", rows), ("HYDRO_3D_ZWAVES", zw)):
+    keys = ("HYDRO_ROWS_PER_WAVE", "HYDRO_3D_ZWAVES", "HYDRO_3D_ZSHARE")
+    # (rows per chunk, waves per workgroup, shared-row modes)
+    for case in ((None, None, None), ("8", None, None), ("41", None, None), (None, "8", None), ("13", "8", None),
+                 (None, "4", "0"), (None, "4", "0x3F"), ("13", "4", "0x3F")):
+        for k, v in zip(keys, case):
             if v is None:
                 monkeypatch.delenv(k, raising=False)
             else:
@@ -170,7 +173,7 @@ def test_3d_first_order_decomposition_invariant(gpu_lib, precision, monkeypatch)
         g.run(3)
         out.append(g.get_state())
         g.close()
-    for k in ("HYDRO_ROWS_PER_WAVE", "HYDRO_3D_ZWAVES"):
+    for k in keys:
         monkeypatch.delenv(k, raising=False)
     for o in out[1:]:
         np.testing.assert_array_equal(o, out[0])
