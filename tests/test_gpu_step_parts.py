@@ -78,6 +78,29 @@ def test_cfl_accumulators_reset_between_calls(gpu_lib):
     assert dts[0] == dts[2] and dts[1] > dts[0]
 
 
+def test_calc_timestep_clips_against_the_clock(gpu_lib):
+    # with HYDRO_FLAG_CLIP_TO_TEND, calc_timestep clips against the
+    # simulation's own time (it leaves the clock as it is): the dt it returns
+    # is the one the next update takes; at t_end it is 0 and updates stop
+    # counting
+    d = config.make_desc([256], xmin=[-1], xmax=[1], integratorName="RungeKutta2TVD", cfl=0.5, t_end=0.01,
+                         flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND)
+    g = _mk(d)
+    g.set_state(ics.sod(d))
+    while True:
+        t, steps, _ = g.time()
+        g.boundary()
+        dt = g.calc_timestep()
+        assert g.time()[:2] == (t, steps)  # the clock did not move
+        assert t + dt <= 0.01 + 1e-15
+        if dt == 0.0:
+            break
+        assert g.update() == dt
+    assert g.time()[0] == 0.01
+    n = g.time()[1]
+    assert g.update() == 0.0 and g.time()[1] == n and g.health() == 0
+
+
 @pytest.mark.parametrize("bm", [["PERIODIC", "MIRROR"], ["FREEFLOW", "PERIODIC"], ["MIRROR", "FREEFLOW"]])
 def test_manual_step_sequence_matches_oracle(gpu_lib, bm):
     # boundary / calc_timestep / step called one by one, including two steps
