@@ -302,27 +302,35 @@ def config_legs(steps, warmup, with_cpu, cpu_seconds, only=None):
     # C1: Sod, 1024 cells incl. ghosts on [-1, 1], TVD-RK2, Superbee, CFL 0.5, t = 0.2
     n = 1024
     kw = dict(xmin=[-1], xmax=[1], integratorName="RungeKutta2TVD", slopeLimiter="Superbee", cfl=0.5, t_end=0.2)
-    d = config.make_desc([n], flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND, **kw)
-    q = ics.sod(d)
-    sim = capi.Sim(d)
-    sim.set_state(q)
-    sim.run(2)
-    sim.set_state(q)
-    sync_all(sim)
-    t0 = time.perf_counter()
-    while sim.time()[0] < 0.2 - 1e-12:
-        sim.run(64)
-    el = time.perf_counter() - t0
-    _, nsteps, _ = sim.time()
-    rho, _, _ = ics.primitives_euler(d, sim.get_state())
-    x = ics.cell_centers(d)[0]
+    x = ics.cell_centers(config.make_desc([n], **kw))[0]
     er, _, _ = sample((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4, x / 0.2)
-    l1 = float(np.sum(np.abs(rho - er)[2:n - 2]) * (2.0 / n))
-    sim.close()
+
+    def sod_to_t(precision, flags):
+        d = config.make_desc([n], precision=precision, flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND | flags,
+                             **kw)
+        q = ics.sod(d)
+        sim = capi.Sim(d)
+        sim.set_state(q)
+        sim.run(2)
+        sim.set_state(q)
+        sync_all(sim)
+        t0 = time.perf_counter()
+        while sim.time()[0] < 0.2 - 1e-12:
+            sim.run(64)
+        el = time.perf_counter() - t0
+        _, nsteps, _ = sim.time()
+        rho, _, _ = ics.primitives_euler(d, sim.get_state())
+        sim.close()
+        return d, q, nsteps, el, float(np.sum(np.abs(rho - er)[2:n - 2]) * (2.0 / n))
+
+    d, q, nsteps, el, l1 = sod_to_t("fp64", 0)
+    _, _, nsteps32, el32, l1_32 = sod_to_t("fp32_mixed", config.FLAG_KAHAN)
     c1 = {"workload": "euler1d_roe_sod_1024_rk2tvd_superbee_t0.2", "precision": "fp64", "steps": nsteps,
           "gpu_value": (n - 4) * nsteps / el, "unit": "cell-updates/s", "gpu_seconds_to_t": el, "l1_rho_vs_exact": l1,
+          "gpu_mixed_value": (n - 4) * nsteps32 / el32, "gpu_mixed_seconds_to_t": el32, "mixed_steps": nsteps32,
+          "l1_rho_vs_exact_mixed": l1_32,
           "note": "latency-bound: 1020 cells per step, every update in one single-workgroup launch per 64 steps, the host "
-                  "reads t between launches"}
+                  "reads t between launches; mixed = FP32 compensated storage"}
     if with_cpu:
         import oracle
 
