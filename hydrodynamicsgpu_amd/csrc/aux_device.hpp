@@ -253,11 +253,23 @@ struct DtSolver {
   }
 };
 
-// A wave's largest rate -> its smallest dt (NaN when a speed was not finite)
+// Partial CFL minima carry "some entry was NaN" in the sign bit: the value
+// is the minimum over the finite entries (>= 0, or +inf), as findMinTimestep's
+// min() skips NaN entries one by one (res/Common.cl:9-40); the flag only
+// feeds the health bit.  dt_flag encodes, dt_merge folds one into a running
+// (minimum, flag) pair.
+HYDRO_DEV double dt_flag(double dt, bool nan) { return nan ? copysign(dt, -1.0) : dt; }
+HYDRO_DEV void dt_merge(double v, double& m, bool& nan) {
+  nan |= signbit(v) || !(v == v);
+  m = fmin(m, fabs(v));
+}
+
+// A wave's largest rate -> its smallest dt, flagged when a speed was not
+// finite (the finite entries still count)
 HYDRO_DEV double wave_dt(double rate, bool bad) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) rate = fmax(rate, __shfl_xor(rate, o, 64));
-  return __any(bad) ? NAN : 1.0 / rate;
+  return dt_flag(1.0 / rate, __any(bad));
 }
 
 // ---------------------------------------------------------------- CFL, 1D/2D

@@ -158,13 +158,13 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
     m = fmin(m, r);
   }
   m = block_min(m, sh);
-  // any NaN in the block is reported as NaN for the finaliser's health check
+  // a NaN in the block is flagged for the finaliser's health check
   __shared__ int anybad;
   if (threadIdx.x == 0) anybad = 0;
   __syncthreads();
   if (bad) atomicOr(&anybad, 1);
   __syncthreads();
-  if (threadIdx.x == 0) a.blockmin[blockIdx.x] = anybad ? NAN : m;
+  if (threadIdx.x == 0) a.blockmin[blockIdx.x] = dt_flag(m, anybad != 0);
 }
 
 void launch_dt_euler_f64(const DtArgs& a, const double* u0, hipStream_t st) {
@@ -195,10 +195,7 @@ __global__ void __launch_bounds__(1024) finalize_dt_kernel(FinalizeArgs f) {
       v[k] = i < f.nblocks ? f.blockmin[i] : INFINITY;
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      nan |= !(v[k] == v[k]);
-      m = fmin(m, v[k]);
-    }
+    for (int k = 0; k < 8; ++k) dt_merge(v[k], m, nan);
   }
   __shared__ int anynan;
   if (threadIdx.x == 0) anynan = 0;

@@ -63,12 +63,8 @@ __global__ void __launch_bounds__(256) batch_dt_kernel(BatchDtArgs b, const S* u
     mn = INFINITY;
     bool bad = false;
 #pragma unroll 1
-    for (int st = task.a; st < task.a + task.b; ++st) {
-      const double v = dt_march_wave<S, 1, SOLVER>(a, u0, st, 0);
-      bad |= !(v == v);
-      mn = fmin(mn, v);
-    }
-    if (bad) mn = NAN;
+    for (int st = task.a; st < task.a + task.b; ++st) dt_merge(dt_march_wave<S, 1, SOLVER>(a, u0, st, 0), mn, bad);
+    mn = dt_flag(mn, bad);
   }
   if ((threadIdx.x & 63) == 0) b.taskmin[t] = mn;
 }
@@ -97,9 +93,7 @@ __global__ void __launch_bounds__(256) batch_finalize_kernel(BatchFinalizeArgs f
   bool nan = false;
   const int ntasks = f.use_fixed ? 0 : m.dt_ntasks;  // no CFL launch with a fixed dt
   for (int i = threadIdx.x; i < ntasks; i += blockDim.x) {
-    const double x = f.taskmin[m.dt_task0 + i];
-    nan |= !(x == x);
-    v = fmin(v, x);
+    dt_merge(f.taskmin[m.dt_task0 + i], v, nan);
   }
   if (nan) atomicOr(&anynan, 1);
   v = block_min(v, sh);

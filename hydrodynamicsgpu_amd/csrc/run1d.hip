@@ -49,11 +49,7 @@ __global__ void __launch_bounds__(kRun1dThreads) run1d_kernel(Run1dArgs<T, S> r)
     bool bad = false;
     if (r.march) {
 #pragma unroll 1
-      for (int w = wave; w < r.dt.nstrips; w += NW) {
-        const double v = dt_march_wave<S, 1, SOLVER>(r.dt, r.U[par], w, 0);
-        bad |= !(v == v);
-        m = fmin(m, v);
-      }
+      for (int w = wave; w < r.dt.nstrips; w += NW) dt_merge(dt_march_wave<S, 1, SOLVER>(r.dt, r.U[par], w, 0), m, bad);
     }
     if (lane == 0) {
       wmin[wave] = m;

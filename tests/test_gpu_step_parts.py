@@ -57,6 +57,33 @@ def test_cfl_matches_oracle(gpu_lib, solver, size, precision):
     assert g.calc_timestep() == pytest.approx(o.calc_timestep(), rel=rel)
 
 
+@pytest.mark.parametrize("solver", ["Roe", "HLL", "HLLC"])
+@pytest.mark.parametrize("size", [[300], [125, 64], [70, 33, 21]])
+def test_cfl_skips_nan_speeds_like_the_reference(gpu_lib, solver, size):
+    # cells with negative pressure have NaN sound speeds (per cell for
+    # HLL/HLLC, where the Roe average is negative too for Roe); min/max with
+    # NaN-skipping semantics absorb some (HLL's speed bounds), the reference's
+    # max(0, .) / min(0, .) read the rest as 0, and its min() over the
+    # per-cell dt counts every other cell (res/Roe.cl:51-57,
+    # res/Common.cl:9-40).  The GPU minimum must be the oracle's.
+    d = config.make_desc(size, solverName="Euler" + solver, cfl=0.5)
+    q = _rand_state(d, 11)
+    n = config.num_states(d)
+    cells = config.volume(d)
+    for c in (cells // 3, cells // 2 + 7, (2 * cells) // 3 + 1):
+        q[n * c + n - 1] = -1.0  # negative total energy: negative pressure
+    g = _mk(d)
+    g.set_state(q)
+    g.boundary()
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    o.boundary()
+    o.init_step()
+    dt_o = o.calc_timestep()
+    assert np.isfinite(dt_o)
+    assert g.calc_timestep() == pytest.approx(dt_o, rel=1e-13)
+
+
 def test_cfl_accumulators_reset_between_calls(gpu_lib):
     # the fused reduction leaves its accumulators reset: a smaller-speed state
     # after a larger-speed one gets its own (larger) dt, not the old minimum
