@@ -1,4 +1,7 @@
-"""Every integrator with every solver family, FP64, against the oracle.
+"""Matrices of configurations against the oracle: every integrator with
+every solver family; boundary methods mixed per side and axis (NONE
+included) with update() and run(); every NaN-safe limiter with the limited
+Roe-type fluxes of each equation.
 
 The twelve tableaus (ForwardEuler and the eleven Runge-Kutta forms of
 include/HydroGPU/Integrator/RungeKutta.h, each with the plan the library
@@ -80,3 +83,79 @@ def test_integrator_solver_matrix(gpu_lib, family, integ):
         m.update()
     assert _relerr(m.get_state(), g.get_state()) < 1e-4
     assert m.health() == 0
+
+
+BOUNDARY_SETS = [
+    # per axis (min, max); axes beyond the grid's dimension are ignored
+    [("NONE", "NONE"), ("NONE", "NONE"), ("NONE", "NONE")],
+    [("MIRROR", "NONE"), ("NONE", "FREEFLOW"), ("PERIODIC", "PERIODIC")],
+    [("FREEFLOW", "MIRROR"), ("MIRROR", "PERIODIC"), ("NONE", "MIRROR")],
+    [("PERIODIC", "PERIODIC"), ("FREEFLOW", "NONE"), ("MIRROR", "FREEFLOW")],
+]
+
+
+@pytest.mark.parametrize("bset", range(len(BOUNDARY_SETS)))
+@pytest.mark.parametrize("family", ["roe1d", "roe2d", "hll2d", "hllc3d", "roe3d", "srhd2d", "maxwell2d"])
+def test_boundary_matrix(gpu_lib, family, bset):
+    # NONE sides keep their ghost cells as they are; mixed methods per side
+    # and axis compose as the reference's per-axis passes do.  update() and
+    # run() against the oracle (full state, ghosts included).
+    from hydrodynamicsgpu_amd import capi
+
+    size, solver, lim, _ = FAMILIES[family]
+    kw = dict(xmin=[0.0] * 3, xmax=[1.0] * 3, solverName=solver, integratorName="RungeKutta2TVD", slopeLimiter=lim,
+              boundaryMethods=[{"min": a, "max": b} for a, b in BOUNDARY_SETS[bset]], cfl=0.4)
+    if solver == "SRHDRoe":
+        kw["defs"] = {"idealGas_heatCapacityRatio": 5.0 / 3.0}
+    d = config.make_desc(size, **kw)
+    q = _state(d, family)
+    g, r = capi.Sim(d), capi.Sim(d)
+    g.set_state(q)
+    r.set_state(q)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    for _ in range(3):
+        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+    r.run(3)
+    ref = o.get_state()
+    tol = 1e-9 if family.startswith("srhd") else 1e-10
+    assert _relerr(g.get_state(), ref) < tol
+    assert _relerr(r.get_state(), ref) < tol
+    assert g.health() == 0 and r.health() == 0
+
+
+# limiters whose formula is NaN for r = 0/0 (not wrapped in fmin/fmax in the
+# reference) are left out: smooth random states hit 0/0 in some field
+NAN_SAFE = [l for l in config.LIMITERS if l not in ("BeamWarming", "Fromm", "CHARM", "Ospre", "VanAlbada1",
+                                                    "VanAlbada2", "VanLeer", "BarthJespersen")]
+
+
+@pytest.mark.parametrize("lim", NAN_SAFE)
+@pytest.mark.parametrize("family", ["roe1d", "roe3d", "srhd2d", "maxwell2d"])
+def test_limiter_matrix(gpu_lib, family, lim):
+    # every NaN-safe limiter through the limited Roe-type fluxes of each
+    # equation (the compiled and the runtime-selected forms), FP64 against
+    # the oracle and FP32-mixed against FP64
+    from hydrodynamicsgpu_amd import capi
+
+    size, solver, _, bm = FAMILIES[family]
+    kw = dict(xmin=[0.0] * 3, xmax=[1.0] * 3, solverName=solver, integratorName="RungeKutta2", slopeLimiter=lim,
+              boundaryMethods=[{"min": bm, "max": bm}] * 3, cfl=0.4)
+    if solver == "SRHDRoe":
+        kw["defs"] = {"idealGas_heatCapacityRatio": 5.0 / 3.0}
+    d = config.make_desc(size, **kw)
+    q = _state(d, family)
+    g = capi.Sim(d)
+    g.set_state(q)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    for _ in range(2):
+        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+    tol = 1e-9 if family.startswith("srhd") else 1e-10
+    assert _relerr(g.get_state(), o.get_state()) < tol
+    m = capi.Sim(config.make_desc(size, precision="fp32_mixed", **kw))
+    m.set_state(q)
+    for _ in range(2):
+        m.update()
+    assert _relerr(m.get_state(), g.get_state()) < 1e-4
+    assert g.health() == 0 and m.health() == 0
