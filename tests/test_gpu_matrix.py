@@ -9,8 +9,8 @@ picks for it) times the solver families (Euler Roe 1D/2D/3D, Euler HLL,
 Euler HLLC 2D/3D, SRHD Roe, Maxwell Roe) on small grids with a boundary
 method per family: two updates, max |gpu - oracle| <= 1e-10 * max |oracle|
 (SRHD 1e-9: the con2prim Newton's stopping iteration can move with
-round-off).  FP32-mixed runs of the same matrix track the FP64 GPU runs to
-FP32 accuracy (relative 1e-4).
+round-off).  FP32-mixed runs of the same matrix, plain and with compensated
+storage, track the FP64 GPU runs to FP32 accuracy (relative 1e-4).
 """
 import numpy as np
 import pytest
@@ -33,13 +33,13 @@ FAMILIES = {
 }
 
 
-def _desc(family, integ, precision="fp64"):
+def _desc(family, integ, precision="fp64", flags=config.FLAG_REFERENCE):
     size, solver, lim, bm = FAMILIES[family]
     # RungeKutta2NonTVD's first stage steps back by 20 dt (RungeKutta.h:232-243):
     # with the first-order solvers it blows up within an update at CFL 0.4
     cfl = 0.02 if integ == "RungeKutta2NonTVD" else 0.4
     kw = dict(xmin=[0.0] * 3, xmax=[1.0] * 3, solverName=solver, integratorName=integ, slopeLimiter=lim,
-              boundaryMethods=[{"min": bm, "max": bm}] * 3, cfl=cfl, precision=precision)
+              boundaryMethods=[{"min": bm, "max": bm}] * 3, cfl=cfl, precision=precision, flags=flags)
     if solver == "SRHDRoe":
         kw["defs"] = {"idealGas_heatCapacityRatio": 5.0 / 3.0}
     return config.make_desc(size, **kw)
@@ -77,12 +77,13 @@ def test_integrator_solver_matrix(gpu_lib, family, integ):
     tol = 1e-9 if family.startswith("srhd") else 1e-10
     assert _relerr(g.get_state(), o.get_state()) < tol
     assert g.health() == 0
-    m = capi.Sim(_desc(family, integ, "fp32_mixed"))
-    m.set_state(q)
-    for _ in range(2):
-        m.update()
-    assert _relerr(m.get_state(), g.get_state()) < 1e-4
-    assert m.health() == 0
+    for flags in (config.FLAG_REFERENCE, config.FLAG_REFERENCE | config.FLAG_KAHAN):
+        m = capi.Sim(_desc(family, integ, "fp32_mixed", flags))
+        m.set_state(q)
+        for _ in range(2):
+            m.update()
+        assert _relerr(m.get_state(), g.get_state()) < 1e-4
+        assert m.health() == 0
 
 
 BOUNDARY_SETS = [
