@@ -1,6 +1,6 @@
 """Matrices of configurations against the oracle: every integrator with
 every solver family (fused path, and the host-side integrate(dt, callback)
-form); boundary methods mixed per side and axis (NONE included) with
+form); a fixed dt; boundary methods mixed per side and axis (NONE included) with
 update() and run(); every NaN-safe limiter with the limited Roe-type fluxes
 of each equation.
 
@@ -186,3 +186,32 @@ def test_host_side_integrator_matrix(gpu_lib, family, integ):
     tol = 1e-9 if family.startswith("srhd") else 1e-10
     assert _relerr(g.get_state(), o.get_state()) < tol
     assert g.health() == 0
+
+
+@pytest.mark.parametrize("family", list(FAMILIES))
+def test_fixed_dt_matrix(gpu_lib, family):
+    # useFixedDT: dt is fixedDT and the frozen data are those of each step's
+    # starting state (the oracle's option for that; the reference itself
+    # skips the Roe basis then, a documented difference)
+    from hydrodynamicsgpu_amd import capi
+
+    size, solver, lim, bm = FAMILIES[family]
+    kw = dict(xmin=[0.0] * 3, xmax=[1.0] * 3, solverName=solver, integratorName="RungeKutta3TVD", slopeLimiter=lim,
+              boundaryMethods=[{"min": bm, "max": bm}] * 3, useFixedDT=True, fixedDT=1e-3)
+    if solver == "SRHDRoe":
+        kw["defs"] = {"idealGas_heatCapacityRatio": 5.0 / 3.0}
+    d = config.make_desc(size, **kw)
+    q = _state(d, family)
+    g, r = capi.Sim(d), capi.Sim(d)
+    g.set_state(q)
+    r.set_state(q)
+    o = oracle.OracleSim(d, opts=oracle.ORACLE_OPT_FRESH_BASIS_FIXED_DT)
+    o.set_state(q)
+    for _ in range(3):
+        assert g.update() == 1e-3
+        o.update()
+    r.run(3)
+    tol = 1e-9 if family.startswith("srhd") else 1e-10
+    assert _relerr(g.get_state(), o.get_state()) < tol
+    assert _relerr(r.get_state(), o.get_state()) < tol
+    assert r.time()[0] == pytest.approx(3e-3, rel=1e-15)
