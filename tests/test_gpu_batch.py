@@ -160,3 +160,20 @@ def test_uneven_512_mix_one_step_properties(gpu_lib):
         ref, tm = _single(d, q, 2)
         np.testing.assert_array_equal(b.get_state(m), ref)
         assert b.time(m) == tm
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_clipped_members_stop_at_their_t_end(gpu_lib, precision):
+    # members reach t_end after different numbers of updates; later updates
+    # are no-ops for them (clock, step count and health stay), exactly as for
+    # the same grid run alone
+    mix = _mix(integratorName="RungeKutta2", slopeLimiter="Superbee", precision=precision,
+               flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND, t_end=0.004)
+    b = _batch(mix)
+    b.run(40)
+    for m, (d, q) in enumerate(mix):
+        ref, (t, n, dt) = _single(d, q, 40)
+        assert t == 0.004 and 0 < n < 40, (m, t, n)
+        np.testing.assert_array_equal(b.get_state(m), ref, err_msg="member %d" % m)
+        assert b.time(m) == (t, n, dt)
+        assert b.health(m) == 0
