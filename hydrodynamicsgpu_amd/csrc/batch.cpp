@@ -414,6 +414,9 @@ int hydro_batch_create(hydro_ctx* ctx, const hydro_sim_desc* descs, int count, h
     return cleanup(HYDRO_E_HIP, "memset");
   if (hipHostMalloc((void**)&b->host_sc, sizeof(double) * kBatchDtStride * count) != hipSuccess)
     return cleanup(HYDRO_E_HIP, "pinned alloc");
+  // the zero fills above ran on the null stream: complete them before the
+  // batch's own stream touches the arenas
+  if (hipDeviceSynchronize() != hipSuccess) return cleanup(HYDRO_E_HIP, "sync");
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess || ncu <= 0) ncu = 256;
   b->blocks = 4 * ncu;  // 4 waves per SIMD
@@ -468,18 +471,18 @@ int hydro_batch_set_state(hydro_batch* b, int member, const double* aos, size_t 
         hi[(size_t)k * M.plane + c] = h;
         lo[(size_t)k * M.plane + c] = (float)(v - (double)h);
       }
-    HIP_TRY(hipMemcpy((float*)b->U[b->cur] + M.off, hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync((float*)b->U[b->cur] + M.off, hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice, b->stream));
     if (b->kahan)
-      HIP_TRY(hipMemcpy(b->Ulo[b->cur] + M.off, lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIP_TRY(copy_sync(b->Ulo[b->cur] + M.off, lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice, b->stream));
   } else {
     std::vector<double> buf((size_t)M.plane * N, 0.0);
     for (long long c = 0; c < vol; ++c)
       for (int k = 0; k < N; ++k) buf[(size_t)k * M.plane + c] = aos[k + (size_t)N * c];
-    HIP_TRY(hipMemcpy((double*)b->U[b->cur] + M.off, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync((double*)b->U[b->cur] + M.off, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
   }
   // a new initial state restarts the member's clock
   double z[kBatchDtStride] = {0, 0, 0, 0};
-  HIP_TRY(hipMemcpy(b->sc + (size_t)member * kBatchDtStride, z, sizeof z, hipMemcpyHostToDevice));
+  HIP_TRY(copy_sync(b->sc + (size_t)member * kBatchDtStride, z, sizeof z, hipMemcpyHostToDevice, b->stream));
   return HYDRO_OK;
 }
 
@@ -494,10 +497,10 @@ int hydro_batch_get_state(hydro_batch* b, int member, double* aos, size_t n) {
   HIP_TRY(hipStreamSynchronize(b->stream));
   if (b->mixed) {
     std::vector<float> hi((size_t)M.plane * N), lo;
-    HIP_TRY(hipMemcpy(hi.data(), (float*)b->U[b->cur] + M.off, hi.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(hi.data(), (float*)b->U[b->cur] + M.off, hi.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     if (b->kahan) {
       lo.resize(hi.size());
-      HIP_TRY(hipMemcpy(lo.data(), b->Ulo[b->cur] + M.off, lo.size() * sizeof(float), hipMemcpyDeviceToHost));
+      HIP_TRY(copy_sync(lo.data(), b->Ulo[b->cur] + M.off, lo.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     }
     for (long long c = 0; c < vol; ++c)
       for (int k = 0; k < N; ++k) {
@@ -507,7 +510,7 @@ int hydro_batch_get_state(hydro_batch* b, int member, double* aos, size_t n) {
       }
   } else {
     std::vector<double> buf((size_t)M.plane * N);
-    HIP_TRY(hipMemcpy(buf.data(), (double*)b->U[b->cur] + M.off, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(buf.data(), (double*)b->U[b->cur] + M.off, buf.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     for (long long c = 0; c < vol; ++c)
       for (int k = 0; k < N; ++k) aos[k + (size_t)N * c] = buf[(size_t)k * M.plane + c];
   }

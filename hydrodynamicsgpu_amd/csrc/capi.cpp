@@ -790,13 +790,13 @@ static int upload_state(hydro_sim* s, const double* aos, const float* low = null
         hi[(size_t)k * s->plane + c] = h;
         lo[(size_t)k * s->plane + c] = low ? low[k + (size_t)N * c] : (float)(v - (double)h);
       }
-    HIP_TRY(hipMemcpy(s->U[s->cur], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
-    if (s->kahan) HIP_TRY(hipMemcpy(s->Ulo[s->cur], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(s->U[s->cur], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    if (s->kahan) HIP_TRY(copy_sync(s->Ulo[s->cur], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
   } else {
     std::vector<double> buf((size_t)s->plane * N, 0.0);
     for (long long c = 0; c < V; ++c)
       for (int k = 0; k < N; ++k) buf[(size_t)k * s->plane + c] = aos[k + (size_t)N * c];
-    HIP_TRY(hipMemcpy(s->U[s->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(s->U[s->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
   }
   s->ghosts_pending = false;
   return HYDRO_OK;
@@ -835,14 +835,14 @@ int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
       for (int a = 0; a < D; ++a) pr[(size_t)(1 + a) * s->plane + c] = v[a];
       pr[(size_t)E * s->plane + c] = eInt;
     }
-    HIP_TRY(hipMemcpy(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
     aos = conv.data();
   }
   const int rc = upload_state(s, aos);
   if (rc) return rc;
   // a new initial state restarts the clock
   DevScalars z{};
-  HIP_TRY(hipMemcpy(s->sc, &z, sizeof z, hipMemcpyHostToDevice));
+  HIP_TRY(copy_sync(s->sc, &z, sizeof z, hipMemcpyHostToDevice, s->stream));
   return HYDRO_OK;
 }
 
@@ -859,14 +859,14 @@ int hydro_sim_restore(hydro_sim* s, const double* aos, const float* lo, const do
     std::vector<double> pr((size_t)s->plane * s->N, 0.0);
     for (long long c = 0; c < s->vol; ++c)
       for (int k = 0; k < s->N; ++k) pr[(size_t)k * s->plane + c] = prims[k + (size_t)s->N * c];
-    HIP_TRY(hipMemcpy(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
   }
   const int rc = upload_state(s, aos, lo);
   if (rc) return rc;
   DevScalars z{};
   z.t = t;
   z.steps = steps;
-  HIP_TRY(hipMemcpy(s->sc, &z, sizeof z, hipMemcpyHostToDevice));
+  HIP_TRY(copy_sync(s->sc, &z, sizeof z, hipMemcpyHostToDevice, s->stream));
   return HYDRO_OK;
 }
 
@@ -910,8 +910,8 @@ int hydro_sim_get_storage(hydro_sim* s, double* hi, float* lo, size_t n) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   const int N = s->N;
   std::vector<float> h((size_t)s->plane * N), l((size_t)s->plane * N);
-  HIP_TRY(hipMemcpy(h.data(), s->U[s->cur], h.size() * sizeof(float), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(l.data(), s->Ulo[s->cur], l.size() * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_sync(h.data(), s->U[s->cur], h.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(copy_sync(l.data(), s->Ulo[s->cur], l.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
   for (long long c = 0; c < s->vol; ++c)
     for (int k = 0; k < N; ++k) {
       hi[k + (size_t)N * c] = h[(size_t)k * s->plane + c];
@@ -926,7 +926,7 @@ int hydro_sim_get_primitives(hydro_sim* s, double* aos, size_t n) {
   if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "size must be N * volume");
   HIP_TRY(hipStreamSynchronize(s->stream));
   std::vector<double> buf((size_t)s->plane * s->N);
-  HIP_TRY(hipMemcpy(buf.data(), s->prim, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_sync(buf.data(), s->prim, buf.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   for (long long c = 0; c < s->vol; ++c)
     for (int k = 0; k < s->N; ++k) aos[k + (size_t)s->N * c] = buf[(size_t)k * s->plane + c];
   return HYDRO_OK;
@@ -1114,10 +1114,13 @@ int hydro_sim_buffer_create(hydro_sim* s, void** buf) {
   const size_t bytes = esize(s) * (size_t)s->plane * s->N;
   void* p = nullptr;
   HIP_TRY(hipMalloc(&p, bytes));
-  const hipError_t e = hipMemset(p, 0, bytes);
+  // zeroed in the simulation's stream order (a null-stream memset is not
+  // ordered against the non-blocking stream that uses the buffer next)
+  hipError_t e = hipMemsetAsync(p, 0, bytes, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
     hipFree(p);
-    return fail(HYDRO_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return fail(HYDRO_E_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
   }
   s->user_bufs.push_back(p);
   *buf = p;

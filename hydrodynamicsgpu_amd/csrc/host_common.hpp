@@ -265,6 +265,16 @@ inline double kernel_dx(double dx, bool six) {
   return strtod(buf, nullptr);
 }
 
+// A host<->device copy ordered on the simulation's (non-blocking) stream and
+// complete on return: copies on the legacy null stream are not ordered
+// against such streams, and a pageable host-to-device hipMemcpy may return
+// before its transfer lands.
+inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
+}
+
 struct DevScalars {
   double dt;
   double t;
