@@ -95,14 +95,7 @@ struct MaxwellRoePolicy : StageIsState<6, T> {
     const T lam[N] = {-a.lam, -a.lam, T(0), T(0), a.lam, a.lam};
     T ft[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const T l = lam[k];
-      const bool up = l >= T(0);
-      const T num = up ? dqlo[k] : dqhi[k];
-      const T theta = up ? T(1) : T(-1);
-      // l qavg - 1/2 l dq (theta + phi (eps - theta)), eps = l dt/dx
-      ft[k] = l * (a.lq[k] - T(.5) * limited_wave<LIM, T>(lim, num, a.dq[k], theta, l * dt_dx));
-    }
+    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, lam[k], a.lq[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
     right(ft, side, a.se, a.su, F);
   }
 };

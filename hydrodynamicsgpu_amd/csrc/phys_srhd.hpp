@@ -233,14 +233,7 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
                                T F[N]) {
     T ft[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const T l = a.lam[k];
-      const bool up = l >= T(0);
-      const T num = up ? dqlo[k] : dqhi[k];
-      const T theta = up ? T(1) : T(-1);
-      // l qavg - 1/2 l dq (theta + phi (eps - theta)), eps = l dt/dx
-      ft[k] = l * (a.lq[k] - T(.5) * limited_wave<LIM, T>(lim, num, a.dq[k], theta, l * dt_dx));
-    }
+    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, a.lam[k], a.lq[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       T s = T(0);

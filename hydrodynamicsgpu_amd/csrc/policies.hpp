@@ -36,16 +36,15 @@ struct StageIsState {
   static HYDRO_DEV Stage unpack_stage(const T o[N]) { return stage(o, T(0)); }
 };
 
-// The limited characteristic term of the flux, dq (theta + phi(r) (eps -
-// theta)) with r = num / dq (res/Roe.cl:272-298).  The limiters with
-// compile-time piecewise-linear forms skip the division: with s = sign(dq),
-// b = |dq|, c = s num, dq phi(num / dq) = s phi~(c, b), phi written on the
-// pair (MC: max(0, min(2b, (b+c)/2, 2c)); Superbee: max(0, min(b, 2c),
-// min(2b, c)); MinMod: max(0, min(c, b))).  dq = 0 gives 0, which is what
-// the product gives for the reference's r = +-inf or 0/0 (phi stays finite
-// under fmin/fmax).
+// The limited slope dq phi(r), r = num / dq (res/Roe.cl:272-298).  The
+// limiters with compile-time piecewise-linear forms skip the division: with
+// s = sign(dq), b = |dq|, c = s num, dq phi(num / dq) = s phi~(c, b), phi
+// written on the pair (MC: max(0, min(2b, (b+c)/2, 2c)); Superbee:
+// max(0, min(b, 2c), min(2b, c)); MinMod: max(0, min(c, b))).  dq = 0 gives
+// 0, which is what the product gives for the reference's r = +-inf or 0/0
+// (phi stays finite under fmin/fmax).
 template <int LIM, typename T>
-HYDRO_DEV T limited_wave(int lim, T num, T d, T theta, T eps) {
+HYDRO_DEV T limited_slope(int lim, T num, T d) {
   if constexpr (LIM == LIM_MONOTIZEDCENTRAL || LIM == LIM_SUPERBEE || LIM == LIM_MINMOD) {
     (void)lim;
     const T b = tabs(d);
@@ -57,12 +56,25 @@ HYDRO_DEV T limited_wave(int lim, T num, T d, T theta, T eps) {
       m = tmax(T(0), tmax(tmin(b, T(2) * c), tmin(T(2) * b, c)));
     else
       m = tmax(T(0), tmin(c, b));
-    const T dphi = d < T(0) ? -m : m;
-    return d * theta + dphi * (eps - theta);
+    return d < T(0) ? -m : m;
   } else {
-    const T phi = limiter<LIM, T>(lim, tdiv(num, d));
-    return d * (theta + phi * (eps - theta));
+    return d * limiter<LIM, T>(lim, tdiv(num, d));
   }
+}
+
+// The characteristic flux of one wave k of an interface (res/Roe.cl:272-298):
+//   l qavg - 1/2 l dq (theta + phi(r) (eps - theta)),
+// theta = +1 for l >= 0 else -1, eps = l dt/dx, r from the upwind
+// neighbour's dq (dqlo for l >= 0, dqhi otherwise).  With l theta = |l| it
+// is evaluated as l qavg - 1/2 (|l| dq + dq phi(r) (l eps - |l|)).
+template <int LIM, typename T>
+HYDRO_DEV T wave_flux(int lim, T l, T lq, T d, T dqlo, T dqhi, T dt_dx) {
+  const bool up = l >= T(0);
+  const T num = up ? dqlo : dqhi;
+  const T al = tabs(l);
+  const T dphi = limited_slope<LIM, T>(lim, num, d);
+  const T w = l * (l * dt_dx) - al;
+  return l * lq - T(.5) * (al * d + dphi * w);
 }
 
 // ------------------------------------------------------------------ Roe
@@ -130,14 +142,7 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
     T lam[N], ft[N];
     P::eig(a.f, lam);
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const T l = lam[k];
-      const bool up = l >= T(0);
-      const T num = up ? dqlo[k] : dqhi[k];
-      const T theta = up ? T(1) : T(-1);
-      // l qavg - 1/2 l dq (theta + phi (eps - theta)), eps = l dt/dx
-      ft[k] = l * (a.lq[k] - T(.5) * limited_wave<LIM, T>(lim, num, a.dq[k], theta, l * dt_dx));
-    }
+    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, lam[k], a.lq[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
     P::right(a.f, ft, F);
     P::rotate(F, side);
   }
