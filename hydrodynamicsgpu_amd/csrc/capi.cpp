@@ -68,10 +68,14 @@ struct hydro_sim {
   int rows = 32, wpb = 4;
   int zwaves = 8;  // first-order 3D kernel: waves per workgroup
   int zshare = 0;  // first-order 3D kernel: modes whose rows go through LDS (bit per CombineMode)
+  int dt_rows = 0;  // 2D CFL strip march: rows per wave (0: chosen from the grid size)
   // measurement: event pairs around fused stage launches
   bool profiling = false;
   // two captured updates per starting parity (ping-pong state), replayed by run()
   hipGraphExec_t graph2[2] = {nullptr, nullptr};
+  // longer captures: graph_updates updates (even) per submission
+  int graph_updates = 2;
+  hipGraphExec_t graphN[2] = {nullptr, nullptr};
   bool graph_pending[2] = {false, false};  // ghosts_pending after the captured pair
   bool use_graphs = true;
   bool use_run1d = true;  // small 1D Euler grids: whole runs in one workgroup
@@ -191,6 +195,10 @@ DtMarchArgs march_args(const hydro_sim* s) {
     // up to kDtRows rows per wave, fewer when that leaves SIMDs idle
     a.rows = kDtRows;
     while (a.rows > 2 && (long long)a.nstrips * ((ny + a.rows - 1) / a.rows) < 4096) --a.rows;
+    if (s->dt_rows > 0) {
+      const int r = std::min(kDtRows, std::max(2, s->dt_rows));
+      if ((long long)a.nstrips * ((ny + r - 1) / r) <= kDtMaxWaves) a.rows = r;  // slots in blockmin
+    }
     a.nchunks = (int)((ny + a.rows - 1) / a.rows);
   } else {
     a.rows = 1;
@@ -611,6 +619,8 @@ void hydro_sim_destroy(hydro_sim* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   for (auto& g : s->graph2)
     if (g) hipGraphExecDestroy(g);
+  for (auto& g : s->graphN)
+    if (g) hipGraphExecDestroy(g);
   for (auto& p : s->ev) {
     hipEventDestroy(p.first);
     hipEventDestroy(p.second);
@@ -702,6 +712,8 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   if (const char* e = getenv("HYDRO_GRAPHS")) s->use_graphs = atoi(e) != 0;
   if (const char* e = getenv("HYDRO_RUN1D")) s->use_run1d = atoi(e) != 0;
   if (const char* e = getenv("HYDRO_WAVES_PER_BLOCK")) s->wpb = std::min(4, std::max(1, atoi(e)));
+  if (const char* e = getenv("HYDRO_DT_ROWS")) s->dt_rows = atoi(e);
+  if (const char* e = getenv("HYDRO_GRAPH_UPDATES")) s->graph_updates = std::max(2, atoi(e) & ~1);
 
   auto cleanup = [&](int code, const std::string& m) {
     hydro_sim_destroy(s);
@@ -978,38 +990,72 @@ int hydro_sim_update(hydro_sim* s, double* dt_out) {
   return HYDRO_OK;
 }
 
+// Records n updates (n even) into an instantiated graph.  The capture
+// records the launches without running them and leaves the host-side
+// bookkeeping where n updates leave it.
+static int capture_updates(hydro_sim* s, int n, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = HYDRO_OK;
+  for (int k = 0; k < n && !rc; ++k) rc = enqueue_update(s);
+  const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return HYDRO_OK;
+}
+
+// Captures the n-update graph for both buffer parities when the current
+// one is missing, so a later run that starts on the other parity (after an
+// odd number of updates) replays instead of capturing.  The captures only
+// record launches; the host-side parity and ghost bookkeeping are put back.
+static int ensure_graphs(hydro_sim* s, int n, hipGraphExec_t g[2]) {
+  const int par = s->cur;
+  if (g[par]) return HYDRO_OK;
+  const bool pending = s->ghosts_pending;
+  for (int k = 0; k < 2; ++k) {
+    const int p = par ^ k;
+    if (g[p]) continue;
+    s->cur = p;
+    const int rc = capture_updates(s, n, &g[p]);
+    s->graph_pending[p] = s->ghosts_pending;
+    s->cur = par;
+    s->ghosts_pending = pending;
+    if (rc) return rc;
+  }
+  return HYDRO_OK;
+}
+
 int hydro_sim_run(hydro_sim* s, int nsteps) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
   if (nsteps > 0 && run1d_eligible(s)) return s->mixed ? run1d<float>(s, nsteps) : run1d<double>(s, nsteps);
-  // Pairs of updates replay a captured hipGraph (one submission instead of
-  // a launch per kernel); an odd step, or profiling, runs the launches
-  // directly.  The pair returns the state to the same buffer parity and the
-  // same deferred-ghost bookkeeping, so the capture is valid for every replay.
+  // Runs of graph_updates updates, then pairs, replay captured hipGraphs (one
+  // submission instead of a launch per kernel); an odd step, or profiling,
+  // runs the launches directly.  An even number of updates returns the state
+  // to the same buffer parity and the same deferred-ghost bookkeeping, so a
+  // capture is valid for every replay.
   // (with a NONE side the ghost copy depends on the bookkeeping at replay
   // time, so those grids launch directly)
   const bool graphs = s->use_graphs && !s->profiling && nsteps >= 2 && boundary_covers_ghosts(s);
   int i = 0;
   if (graphs) {
     const int par = s->cur;
-    if (!s->graph2[par]) {
-      hipGraph_t g = nullptr;
-      HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_update(s);
-      if (!rc) rc = enqueue_update(s);
-      const hipError_t e = hipStreamEndCapture(s->stream, &g);
-      if (rc) {
-        if (g) hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      const hipError_t ei = hipGraphInstantiate(&s->graph2[par], g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      if (ei != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-      // the capture recorded the launches without running them and left
-      // the host-side bookkeeping where two updates leave it
-      s->graph_pending[par] = s->ghosts_pending;
+    const int big = s->graph_updates;
+    if (big > 2 && nsteps >= big) {
+      const int rc = ensure_graphs(s, big, s->graphN);
+      if (rc) return rc;
+      for (; i + big <= nsteps; i += big) HIP_TRY(hipGraphLaunch(s->graphN[par], s->stream));
     }
-    for (; i + 2 <= nsteps; i += 2) HIP_TRY(hipGraphLaunch(s->graph2[par], s->stream));
+    if (i + 2 <= nsteps) {
+      const int rc = ensure_graphs(s, 2, s->graph2);
+      if (rc) return rc;
+      for (; i + 2 <= nsteps; i += 2) HIP_TRY(hipGraphLaunch(s->graph2[par], s->stream));
+    }
     s->ghosts_pending = s->graph_pending[par];
   }
   for (; i < nsteps; ++i) {

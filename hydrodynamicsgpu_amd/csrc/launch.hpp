@@ -102,7 +102,7 @@ void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st);
 // CFL of Euler Roe/HLL/HLLC on 1D/2D grids (strip march): one minimum per
 // wave into wavemin[nwaves] (NaN where a speed was not finite), reduced by
 // launch_finalize_dt over nblocks = nwaves
-constexpr int kDtRows = 8;  // rows per wave of the strip-march CFL kernel (at most)
+constexpr int kDtRows = 10;  // rows per wave of the strip-march CFL kernel (at most)
 struct DtMarchArgs {
   int dim, sx, sy, sz;
   long long plane;

@@ -179,8 +179,10 @@ def test_state_after_run_includes_carried_ghosts(gpu_lib):
 @pytest.mark.parametrize("integ,bm,precision", [("RungeKutta4", "PERIODIC", "fp32_mixed"),
                                                  ("RungeKutta3TVD", "MIRROR", "fp64"),
                                                  ("ForwardEuler", "FREEFLOW", "fp64")])
-def test_graph_replays_match_direct_launches(gpu_lib, monkeypatch, integ, bm, precision):
-    # run() replays a captured pair of updates; several replays, with eager
+@pytest.mark.parametrize("graph_updates", ["2", "8"])
+def test_graph_replays_match_direct_launches(gpu_lib, monkeypatch, integ, bm, precision, graph_updates):
+    # run() replays captured runs of updates (graph_updates per graph, then
+    # pairs), for either starting buffer parity; several replays, with eager
     # work and synchronisation in between, must equal direct launches bit
     # for bit
     d = config.make_desc([90, 66], integratorName=integ, slopeLimiter="MonotizedCentral",
@@ -189,10 +191,11 @@ def test_graph_replays_match_direct_launches(gpu_lib, monkeypatch, integ, bm, pr
     monkeypatch.setenv("HYDRO_GRAPHS", "0")
     direct = _mk(d)
     monkeypatch.setenv("HYDRO_GRAPHS", "1")
+    monkeypatch.setenv("HYDRO_GRAPH_UPDATES", graph_updates)
     graphed = _mk(d)
     direct.set_state(q)
     graphed.set_state(q)
-    for n in (4, 3, 2, 5):
+    for n in (4, 3, 2, 5, 11, 17):
         direct.run(n)
         graphed.run(n)
         graphed.sync()
