@@ -253,25 +253,45 @@ int materialize_ghosts(hydro_batch* b) {
   return HYDRO_OK;
 }
 
+// Records two steps into an instantiated graph (launches only; the host-side
+// parity and ghost bookkeeping end where two steps leave them).
+int capture_pair(hydro_batch* b, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(b);
+  if (!rc) rc = enqueue_step(b);
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return HYDRO_OK;
+}
+
 int run_steps(hydro_batch* b, int nsteps, bool graphs) {
   int i = 0;
   if (graphs && nsteps >= 2) {
     const int par = b->cur;
     if (!b->graph2[par]) {
-      hipGraph_t g = nullptr;
-      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_step(b);
-      if (!rc) rc = enqueue_step(b);
-      const hipError_t e = hipStreamEndCapture(b->stream, &g);
-      if (rc) {
-        if (g) hipGraphDestroy(g);
-        return rc;
+      // both buffer parities at once, so a run that starts after an odd
+      // number of steps replays instead of capturing (every side of every
+      // member has a boundary method: the captured steps do not depend on
+      // the ghost bookkeeping)
+      const bool pending = b->ghosts_pending;
+      for (int k = 0; k < 2; ++k) {
+        const int p = par ^ k;
+        if (b->graph2[p]) continue;
+        b->cur = p;
+        const int rc = capture_pair(b, &b->graph2[p]);
+        b->graph_pending[p] = b->ghosts_pending;
+        b->cur = par;
+        b->ghosts_pending = pending;
+        if (rc) return rc;
       }
-      if (e != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      const hipError_t ei = hipGraphInstantiate(&b->graph2[par], g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      if (ei != hipSuccess) return fail(HYDRO_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-      b->graph_pending[par] = b->ghosts_pending;
     }
     for (; i + 2 <= nsteps; i += 2) HIP_TRY(hipGraphLaunch(b->graph2[par], b->stream));
     b->ghosts_pending = b->graph_pending[par];
