@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/hydro_capi.h"
+#include "cfl.hpp"
 #include "host_common.hpp"
 #include "launch.hpp"
 #include "stage_args.hpp"
@@ -34,6 +35,170 @@ using namespace hydro::host;
 
 }  // namespace
 
+
+
+namespace hydro {
+namespace host {
+
+CflGrid cfl_grid(const hydro_sim_desc& d, long long plane) {
+  CflGrid g;
+  g.d = &d;
+  g.dim = d.dim;
+  for (int a = 0; a < 3; ++a) g.sz[a] = a < d.dim ? d.size[a] : 1;
+  g.plane = plane;
+  for (int a = 0; a < 3; ++a) {
+    // src/HydroGPUApp.cpp:224-226: (xmax - xmin) / (float)size
+    const double szf = a < d.dim ? (double)(float)d.size[a] : 1.0;
+    const double dx = (d.xmax[a] - d.xmin[a]) / szf;
+    g.kdx[a] = kernel_dx(dx, (d.flags & HYDRO_FLAG_REF_DX_6DIGITS) != 0);
+  }
+  g.cfl = (double)(float)d.cfl;  // HydroGPUApp.h:84 stores cfl as float
+  g.mixed = d.precision == HYDRO_PRECISION_FP32_MIXED;
+  return g;
+}
+
+DtMarchArgs cfl_march_args(const CflGrid& g, double* minima) {
+  const hydro_sim_desc& d = *g.d;
+  DtMarchArgs a{};
+  a.dim = g.dim;
+  a.sx = g.sz[0];
+  a.sy = g.sz[1];
+  a.plane = g.plane;
+  a.kdx[0] = g.kdx[0];
+  a.kdx[1] = g.kdx[1];
+  a.gamma = d.gamma;
+  a.solver = d.equation == HYDRO_EQN_SRHD ? 3 : d.solver;
+  const int hi = d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
+  a.last_x = g.sz[0] - hi;
+  a.last_y = g.dim > 1 ? g.sz[1] - hi : 0;
+  a.nstrips = (a.last_x - 2) / 60 + 1;
+  if (g.dim > 1) {
+    const long long ny = a.last_y - 1;
+    // up to kDtRows rows per wave, fewer when that leaves SIMDs idle
+    a.rows = kDtRows;
+    while (a.rows > 2 && (long long)a.nstrips * ((ny + a.rows - 1) / a.rows) < 4096) --a.rows;
+    if (g.dt_rows > 0) {
+      const int r = std::min(kDtRows, std::max(2, g.dt_rows));
+      if ((long long)a.nstrips * ((ny + r - 1) / r) <= kDtMaxWaves) a.rows = r;  // slots in the minima
+    }
+    a.nchunks = (int)((ny + a.rows - 1) / a.rows);
+  } else {
+    a.rows = 1;
+    a.nchunks = 1;
+  }
+  a.nwaves = a.nstrips * a.nchunks;
+  a.wavemin = minima;
+  return a;
+}
+
+namespace {
+
+// which CFL kernel a grid uses: 1 = 1D/2D strip march, 2 = 3D strip march,
+// 3 = block minima, 0 = none (fixed dt)
+int cfl_path(const CflGrid& g) {
+  const hydro_sim_desc& d = *g.d;
+  if (d.use_fixed_dt) return 0;
+  const int hi = d.solver == HYDRO_SOLVER_ROE ? 3 : 2;
+  const long long strips = (g.sz[0] - hi - 2) / 60 + 1;
+  const bool srhd = d.equation == HYDRO_EQN_SRHD;
+  if (g.dim <= 2 && (d.equation == HYDRO_EQN_EULER || srhd)) {
+    const long long waves = strips * (g.dim > 1 ? (g.sz[1] - hi - 1 + kDtRows - 1) / kDtRows : 1);
+    if (waves <= kDtMaxWaves) return 1;
+  }
+  if (g.dim == 3 && d.equation == HYDRO_EQN_EULER && strips * (g.sz[2] - hi - 1) <= kDtMaxWaves) return 2;
+  return 3;
+}
+
+DtMarchArgs march3_args(const CflGrid& g, double* minima) {
+  const hydro_sim_desc& d = *g.d;
+  DtMarchArgs a{};
+  a.dim = 3;
+  a.sx = g.sz[0];
+  a.sy = g.sz[1];
+  a.sz = g.sz[2];
+  a.plane = g.plane;
+  for (int i = 0; i < 3; ++i) a.kdx[i] = g.kdx[i];
+  a.gamma = d.gamma;
+  a.solver = d.solver;
+  const int hi = d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
+  a.last_x = g.sz[0] - hi;
+  a.last_y = g.sz[1] - hi;
+  a.last_z = g.sz[2] - hi;
+  a.nstrips = (a.last_x - 2) / 60 + 1;
+  const long long ny = a.last_y - 1, nz = a.last_z - 1;
+  a.rows = 8;
+  while ((long long)a.nstrips * ((ny + a.rows - 1) / a.rows) * nz > kDtMaxWaves && a.rows < ny) a.rows *= 2;
+  a.nchunks = (int)((ny + a.rows - 1) / a.rows);
+  a.nwaves = (int)((long long)a.nstrips * a.nchunks * nz);
+  a.wavemin = minima;
+  return a;
+}
+
+}  // namespace
+
+long long cfl_minima(const CflGrid& g) {
+  switch (cfl_path(g)) {
+  case 1: return cfl_march_args(g, nullptr).nwaves;
+  case 2: return march3_args(g, nullptr).nwaves;
+  case 3: return g.dt_blocks;
+  default: return 0;
+  }
+}
+
+int enqueue_cfl(const CflGrid& g, const void* U, const double* prim, double* minima, FinalizeArgs f,
+                hipStream_t st) {
+  const hydro_sim_desc& d = *g.d;
+  f.blockmin = minima;
+  f.nblocks = 0;
+  switch (cfl_path(g)) {
+  case 1: {
+    // strip march, one minimum per wave, then the one-block tail
+    const DtMarchArgs a = cfl_march_args(g, minima);
+    if (d.equation == HYDRO_EQN_SRHD) launch_dt_march_srhd(a, prim, g.mixed, st);
+    else if (g.mixed) launch_dt_march_f32(a, (const float*)U, st);
+    else launch_dt_march_f64(a, (const double*)U, st);
+    f.nblocks = a.nwaves;
+    break;
+  }
+  case 2: {
+    // 3D strip march: waves = strips x row chunks x planes
+    const DtMarchArgs a = march3_args(g, minima);
+    if (g.mixed) launch_dt_march3_f32(a, (const float*)U, st);
+    else launch_dt_march3_f64(a, (const double*)U, st);
+    f.nblocks = a.nwaves;
+    break;
+  }
+  case 3: {
+    DtArgs a{};
+    a.dim = g.dim;
+    a.sx = g.sz[0];
+    a.sy = g.sz[1];
+    a.sz = g.sz[2];
+    a.plane = g.plane;
+    for (int i = 0; i < 3; ++i) a.kdx[i] = g.kdx[i];
+    a.gamma = d.gamma;
+    a.solver = d.solver;
+    a.eqn = d.equation;
+    a.prim = prim;
+    double mx[5];
+    maxwell_constants(d, mx);
+    a.maxwell_lambda = mx[4];
+    a.nblocks = g.dt_blocks;
+    a.blockmin = minima;
+    if (g.mixed) launch_dt_euler_f32(a, (const float*)U, st);
+    else launch_dt_euler_f64(a, (const double*)U, st);
+    f.nblocks = g.dt_blocks;
+    break;
+  }
+  default: break;
+  }
+  launch_finalize_dt(f, st);
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
+}  // namespace host
+}  // namespace hydro
 
 struct hydro_sim {
   hydro_ctx* ctx = nullptr;
@@ -69,6 +234,7 @@ struct hydro_sim {
   int zwaves = 8;  // first-order 3D kernel: waves per workgroup
   int zshare = 0;  // first-order 3D kernel: modes whose rows go through LDS (bit per CombineMode)
   int dt_rows = 0;  // 2D CFL strip march: rows per wave (0: chosen from the grid size)
+  CflGrid cg;       // the CFL pass's view of this grid
   // measurement: event pairs around fused stage launches
   bool profiling = false;
   // two captured updates per starting parity (ping-pong state), replayed by run()
@@ -176,38 +342,7 @@ int enqueue_boundary(hydro_sim* s) {
 }
 
 // The CFL strip march of a 1D/2D Euler or SRHD grid: waves = strips x row chunks
-DtMarchArgs march_args(const hydro_sim* s) {
-  DtMarchArgs a{};
-  a.dim = s->dim;
-  a.sx = s->sz[0];
-  a.sy = s->sz[1];
-  a.plane = s->plane;
-  a.kdx[0] = s->kdx[0];
-  a.kdx[1] = s->kdx[1];
-  a.gamma = s->d.gamma;
-  a.solver = s->d.equation == HYDRO_EQN_SRHD ? 3 : s->d.solver;
-  const int hi = s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
-  a.last_x = s->sz[0] - hi;
-  a.last_y = s->dim > 1 ? s->sz[1] - hi : 0;
-  a.nstrips = (a.last_x - 2) / 60 + 1;
-  if (s->dim > 1) {
-    const long long ny = a.last_y - 1;
-    // up to kDtRows rows per wave, fewer when that leaves SIMDs idle
-    a.rows = kDtRows;
-    while (a.rows > 2 && (long long)a.nstrips * ((ny + a.rows - 1) / a.rows) < 4096) --a.rows;
-    if (s->dt_rows > 0) {
-      const int r = std::min(kDtRows, std::max(2, s->dt_rows));
-      if ((long long)a.nstrips * ((ny + r - 1) / r) <= kDtMaxWaves) a.rows = r;  // slots in blockmin
-    }
-    a.nchunks = (int)((ny + a.rows - 1) / a.rows);
-  } else {
-    a.rows = 1;
-    a.nchunks = 1;
-  }
-  a.nwaves = a.nstrips * a.nchunks;
-  a.wavemin = s->blockmin;
-  return a;
-}
+DtMarchArgs march_args(const hydro_sim* s) { return cfl_march_args(s->cg, s->blockmin); }
 
 // The dt tail; advance_time false leaves the clock and the step count in a
 // scratch copy (calc_timestep)
@@ -228,77 +363,7 @@ FinalizeArgs finalize_args(hydro_sim* s, bool advance_time) {
 }
 
 int enqueue_dt(hydro_sim* s, bool advance_time) {
-  FinalizeArgs f = finalize_args(s, advance_time);
-  const int march_strips = (s->sz[0] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 2) / 60 + 1;
-  const long long march_waves =
-      (long long)march_strips * (s->dim > 1 ? (s->sz[1] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 1 + kDtRows - 1) / kDtRows : 1);
-  const bool srhd = s->d.equation == HYDRO_EQN_SRHD;
-  if (!s->d.use_fixed_dt && s->dim <= 2 && (s->d.equation == HYDRO_EQN_EULER || srhd) &&
-      march_waves <= kDtMaxWaves) {
-    // strip march, one minimum per wave, then the one-block tail
-    const DtMarchArgs a = march_args(s);
-    if (srhd) launch_dt_march_srhd(a, s->prim, s->mixed, s->stream);
-    else if (s->mixed) launch_dt_march_f32(a, (const float*)s->U[s->cur], s->stream);
-    else launch_dt_march_f64(a, (const double*)s->U[s->cur], s->stream);
-    f.nblocks = a.nwaves;
-    launch_finalize_dt(f, s->stream);
-    HIP_TRY(hipGetLastError());
-    return HYDRO_OK;
-  }
-  const long long strips3 = (s->sz[0] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 2) / 60 + 1;
-  const long long planes3 = s->sz[2] - (s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2) - 1;
-  if (!s->d.use_fixed_dt && s->dim == 3 && s->d.equation == HYDRO_EQN_EULER && strips3 * planes3 <= kDtMaxWaves) {
-    // 3D strip march: waves = strips x row chunks x planes
-    DtMarchArgs a{};
-    a.dim = 3;
-    a.sx = s->sz[0];
-    a.sy = s->sz[1];
-    a.sz = s->sz[2];
-    a.plane = s->plane;
-    for (int i = 0; i < 3; ++i) a.kdx[i] = s->kdx[i];
-    a.gamma = s->d.gamma;
-    a.solver = s->d.solver;
-    const int hi = s->d.solver == HYDRO_SOLVER_ROE ? 3 : 2;  // cells 2 .. S-hi
-    a.last_x = s->sz[0] - hi;
-    a.last_y = s->sz[1] - hi;
-    a.last_z = s->sz[2] - hi;
-    a.nstrips = (a.last_x - 2) / 60 + 1;
-    const long long ny = a.last_y - 1, nz = a.last_z - 1;
-    a.rows = 8;
-    while ((long long)a.nstrips * ((ny + a.rows - 1) / a.rows) * nz > kDtMaxWaves && a.rows < ny) a.rows *= 2;
-    a.nchunks = (int)((ny + a.rows - 1) / a.rows);
-    a.nwaves = (int)((long long)a.nstrips * a.nchunks * nz);
-    a.wavemin = s->blockmin;
-    if (s->mixed) launch_dt_march3_f32(a, (const float*)s->U[s->cur], s->stream);
-    else launch_dt_march3_f64(a, (const double*)s->U[s->cur], s->stream);
-    f.nblocks = a.nwaves;
-    launch_finalize_dt(f, s->stream);
-    HIP_TRY(hipGetLastError());
-    return HYDRO_OK;
-  }
-  if (!s->d.use_fixed_dt) {
-    DtArgs a{};
-    a.dim = s->dim;
-    a.sx = s->sz[0];
-    a.sy = s->sz[1];
-    a.sz = s->sz[2];
-    a.plane = s->plane;
-    for (int i = 0; i < 3; ++i) a.kdx[i] = s->kdx[i];
-    a.gamma = s->d.gamma;
-    a.solver = s->d.solver;
-    a.eqn = s->d.equation;
-    a.prim = s->prim;
-    double mx[5];
-    maxwell_constants(s->d, mx);
-    a.maxwell_lambda = mx[4];
-    a.nblocks = s->dt_blocks;
-    a.blockmin = s->blockmin;
-    if (s->mixed) launch_dt_euler_f32(a, (const float*)s->U[s->cur], s->stream);
-    else launch_dt_euler_f64(a, (const double*)s->U[s->cur], s->stream);
-  }
-  launch_finalize_dt(f, s->stream);
-  HIP_TRY(hipGetLastError());
-  return HYDRO_OK;
+  return enqueue_cfl(s->cg, s->U[s->cur], s->prim, s->blockmin, finalize_args(s, advance_time), s->stream);
 }
 
 template <typename T>
@@ -659,13 +724,11 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   s->plane = (s->vol + 63) & ~63LL;
   s->mixed = d->precision == HYDRO_PRECISION_FP32_MIXED;
   s->kahan = s->mixed && (d->flags & HYDRO_FLAG_KAHAN);
-  for (int a = 0; a < 3; ++a) {
-    // src/HydroGPUApp.cpp:224-226: (xmax - xmin) / (float)size
-    const double szf = a < d->dim ? (double)(float)d->size[a] : 1.0;
-    const double dx = (d->xmax[a] - d->xmin[a]) / szf;
-    s->kdx[a] = kernel_dx(dx, (d->flags & HYDRO_FLAG_REF_DX_6DIGITS) != 0);
+  {
+    const CflGrid g = cfl_grid(*d, s->plane);
+    for (int a = 0; a < 3; ++a) s->kdx[a] = g.kdx[a];
+    s->cfl = g.cfl;
   }
-  s->cfl = (double)(float)d->cfl;  // HydroGPUApp.h:84 stores cfl as float
   const char* plan_env = getenv("HYDRO_RK_PLAN");
   const bool allow_replay = !(plan_env && std::string(plan_env) == "partial");
   if (get_tableau(d->integrator, s->tab) && allow_replay && plan_replay(s->tab, s->plan)) {
@@ -714,6 +777,9 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   if (const char* e = getenv("HYDRO_WAVES_PER_BLOCK")) s->wpb = std::min(4, std::max(1, atoi(e)));
   if (const char* e = getenv("HYDRO_DT_ROWS")) s->dt_rows = atoi(e);
   if (const char* e = getenv("HYDRO_GRAPH_UPDATES")) s->graph_updates = std::max(2, atoi(e) & ~1);
+  s->cg = cfl_grid(s->d, s->plane);
+  s->cg.dt_rows = s->dt_rows;
+  s->cg.dt_blocks = s->dt_blocks;
 
   auto cleanup = [&](int code, const std::string& m) {
     hydro_sim_destroy(s);

@@ -107,27 +107,44 @@ def blast(desc, seed=4321):
     return build_state_euler(desc, rho, zero, zero, zero, p)
 
 
-def srhd_quadrants(desc, seed=777):
-    """Four-quadrant relativistic Riemann problem split at the domain centre.
-    Each quadrant draws rho in [.1,1], p in [.01,1] (log-uniform) and a
-    velocity of magnitude up to .995 (Lorentz factor up to ~10) from the
-    counter hash.  Returns the Newtonian (rho, rho v, E) the reference's
-    initVariables converts (res/SRHDCommon.cl:24-92)."""
-    x, y, z, _ = grid(desc)
-    cxm = 0.5 * (desc.xmin[0] + desc.xmax[0])
-    cym = 0.5 * (desc.xmin[1] + desc.xmax[1])
-    q = (x > cxm).astype(np.int64) + 2 * (y > cym).astype(np.int64)
+def srhd_quadrant_states(seed=777, w_min=3.0, w_max=10.0, spread=0.05):
+    """Per-quadrant (rho, p, W, vx, vy) of `srhd_quadrants`, quadrant index
+    (x > centre) + 2 (y > centre)."""
     qs = np.arange(4, dtype=np.uint64)
     u0 = uniform01(seed, qs, 1)
     u1 = uniform01(seed, qs, 2)
     u2 = uniform01(seed, qs, 3)
     u3 = uniform01(seed, qs, 4)
-    rho_q = 10.0 ** (-1.0 + u0)           # [0.1, 1]
-    p_q = 10.0 ** (-2.0 + 2.0 * u1)        # [0.01, 1]
-    speed = 0.995 * np.sqrt(u2)            # |v| <= .995
-    ang = 2.0 * np.pi * u3
-    vx_q = speed * np.cos(ang)
-    vy_q = speed * np.sin(ang)
+    g = uniform01(seed, np.arange(2, dtype=np.uint64), 5)
+    rho_q = 10.0 ** (-1.0 + u0)            # [0.1, 1]
+    p_q = 10.0 ** (-2.0 + 2.0 * u1)         # [0.01, 1]
+    W = w_min + (w_max - w_min) * u2        # Lorentz factor per quadrant
+    fast = min(int(4.0 * g[0]), 3)
+    W[fast] = w_max * (0.9 + 0.1 * u2[fast])
+    speed = np.sqrt(1.0 - 1.0 / (W * W))
+    ang = 2.0 * np.pi * g[1] + spread * (2.0 * u3 - 1.0)
+    return rho_q, p_q, W, speed * np.cos(ang), speed * np.sin(ang)
+
+
+def srhd_quadrants(desc, seed=777, w_min=3.0, w_max=10.0, spread=0.05):
+    """Four-quadrant relativistic Riemann problem split at the domain centre.
+    Each quadrant draws rho in [.1,1], p in [.01,1] (log-uniform) and a
+    Lorentz factor uniform in [w_min, w_max]; one quadrant, picked by the
+    hash, draws it in [.9 w_max, w_max], so the fastest flow has W >= 9 with
+    the defaults (W = 9.14 for seed 777).  The velocities share one seeded
+    direction, each quadrant turned from it by at most `spread` radians, so
+    neighbouring quadrants move at up to ~.9 c relative to each other
+    (relative Lorentz factor <= 1.4).  Quadrants that collide head-on at
+    these speeds drive the reference's Roe scheme (arithmetic-mean
+    linearisation, res/SRHDRoe.cl:21-312) to the density floor within its
+    first step, in the FP64 oracle as on the GPU, which is why the directions
+    are shared.  Returns the Newtonian (rho, rho v, E) the reference's
+    initVariables converts (res/SRHDCommon.cl:24-92)."""
+    x, y, z, _ = grid(desc)
+    cxm = 0.5 * (desc.xmin[0] + desc.xmax[0])
+    cym = 0.5 * (desc.xmin[1] + desc.xmax[1])
+    q = (x > cxm).astype(np.int64) + 2 * (y > cym).astype(np.int64)
+    rho_q, p_q, _, vx_q, vy_q = srhd_quadrant_states(seed, w_min, w_max, spread)
     rho = rho_q[q]
     p = p_q[q]
     vx = vx_q[q]

@@ -23,6 +23,7 @@ import pytest
 
 import oracle
 from hydrodynamicsgpu_amd import config, ics
+from srhd_states import quadrants_for
 
 pytestmark = pytest.mark.gpu
 
@@ -79,7 +80,7 @@ def _case(size, solver, integ, lim, bm, precision="fp64"):
         kw["defs"] = {"idealGas_heatCapacityRatio": 5.0 / 3.0}
     d = config.make_desc(size, **kw)
     if solver == "SRHDRoe":
-        q = ics.srhd_quadrants(d)
+        q = quadrants_for(d)
     elif solver == "MaxwellRoe":
         q = ics.maxwell_1(d) + 0.1 * np.random.default_rng(5).standard_normal(6 * config.volume(d))
     else:

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from hydrodynamicsgpu_amd import config, ics, io
+from srhd_states import quadrants_for
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +25,7 @@ def test_checkpoint_resume_is_exact(gpu_lib, tmp_path, precision, flags, solver,
               boundaryMethods=[{"min": "FREEFLOW", "max": "FREEFLOW"}] * 3, cfl=0.4)
     if solver == "SRHDRoe":
         d = config.make_desc(size, xmin=[0, 0], xmax=[1, 1], defs={"idealGas_heatCapacityRatio": 5 / 3}, **kw)
-        q = ics.srhd_quadrants(d)
+        q = quadrants_for(d)
     elif len(size) == 3:
         d = config.make_desc(size, xmin=[-1] * 3, xmax=[1] * 3, **kw)
         q = ics.blast(d)

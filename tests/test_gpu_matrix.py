@@ -9,8 +9,10 @@ include/HydroGPU/Integrator/RungeKutta.h, each with the plan the library
 picks for it) times the solver families (Euler Roe 1D/2D/3D, Euler HLL,
 Euler HLLC 2D/3D, SRHD Roe, Maxwell Roe) on small grids with a boundary
 method per family: two updates, max |gpu - oracle| <= 1e-10 * max |oracle|
-(SRHD 1e-9: the con2prim Newton's stopping iteration can move with
-round-off).  FP32-mixed runs of the same matrix, plain and with compensated
+(SRHD 1e-9, for the state and dt alike: the con2prim Newton's stopping
+iteration can move with round-off, and with the fastest quadrant at a
+Lorentz factor W ~ 9, 1 - v^2 ~ .012 magnifies round-off differences in the
+recovery about W^2-fold; other families' dt 1e-11).  FP32-mixed runs of the same matrix, plain and with compensated
 storage, track the FP64 GPU runs to FP32 accuracy (relative 1e-4).
 """
 import numpy as np
@@ -18,6 +20,7 @@ import pytest
 
 import oracle
 from hydrodynamicsgpu_amd import config, ics
+from srhd_states import quadrants_for
 
 pytestmark = pytest.mark.gpu
 
@@ -46,10 +49,14 @@ def _desc(family, integ, precision="fp64", flags=config.FLAG_REFERENCE):
     return config.make_desc(size, **kw)
 
 
+def _dt_tol(family):
+    return 1e-9 if family.startswith("srhd") else 1e-11
+
+
 def _state(d, family):
     rng = np.random.default_rng(len(family) * 7919)
     if family.startswith("srhd"):
-        return ics.srhd_quadrants(d)
+        return quadrants_for(d)
     if family.startswith("maxwell"):
         return ics.maxwell_1(d) + 0.1 * rng.standard_normal(6 * config.volume(d))
     shp = ics.grid(d)[0].shape
@@ -74,7 +81,7 @@ def test_integrator_solver_matrix(gpu_lib, family, integ):
     o = oracle.OracleSim(d)
     o.set_state(q)
     for _ in range(2):
-        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+        assert g.update() == pytest.approx(o.update(), rel=_dt_tol(family))
     tol = 1e-9 if family.startswith("srhd") else 1e-10
     assert _relerr(g.get_state(), o.get_state()) < tol
     assert g.health() == 0
@@ -117,7 +124,7 @@ def test_boundary_matrix(gpu_lib, family, bset):
     o = oracle.OracleSim(d)
     o.set_state(q)
     for _ in range(3):
-        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+        assert g.update() == pytest.approx(o.update(), rel=_dt_tol(family))
     r.run(3)
     ref = o.get_state()
     tol = 1e-9 if family.startswith("srhd") else 1e-10
@@ -152,7 +159,7 @@ def test_limiter_matrix(gpu_lib, family, lim):
     o = oracle.OracleSim(d)
     o.set_state(q)
     for _ in range(2):
-        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+        assert g.update() == pytest.approx(o.update(), rel=_dt_tol(family))
     tol = 1e-9 if family.startswith("srhd") else 1e-10
     assert _relerr(g.get_state(), o.get_state()) < tol
     m = capi.Sim(config.make_desc(size, precision="fp32_mixed", **kw))
@@ -182,7 +189,7 @@ def test_host_side_integrator_matrix(gpu_lib, family, integ):
         g.boundary()
         dt = g.calc_timestep()
         capi.integrate(g, dt)
-        assert dt == pytest.approx(o.update(), rel=1e-11)
+        assert dt == pytest.approx(o.update(), rel=_dt_tol(family))
     tol = 1e-9 if family.startswith("srhd") else 1e-10
     assert _relerr(g.get_state(), o.get_state()) < tol
     assert g.health() == 0

@@ -12,6 +12,7 @@ import pytest
 
 import oracle
 from hydrodynamicsgpu_amd import config, ics
+from srhd_states import quadrants_for
 
 pytestmark = pytest.mark.gpu
 
@@ -44,14 +45,14 @@ def test_srhd_fp64_parity(gpu_lib, size, integ, lim, bm):
     if len(size) == 1:
         q = ics.riemann(d, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1))
     else:
-        q = ics.srhd_quadrants(d) if len(size) == 2 else ics.riemann(d, (1.0, 0.3, 1.0), (0.2, -0.2, 0.05))
+        q = quadrants_for(d) if len(size) == 2 else ics.riemann(d, (1.0, 0.3, 1.0), (0.2, -0.2, 0.05))
     g = _mk(d)
     g.set_state(q)
     o = oracle.OracleSim(d)
     o.set_state(q)
     np.testing.assert_allclose(g.get_primitives(), o.get_primitives(), rtol=1e-14, atol=1e-14)
     for _ in range(5):
-        assert g.update() == pytest.approx(o.update(), rel=1e-10)
+        assert g.update() == pytest.approx(o.update(), rel=1e-9)
     assert _relerr(g.get_state(), o.get_state()) < 1e-9
     assert _relerr(g.get_primitives(), o.get_primitives()) < 1e-9
     assert g.health() == 0
@@ -75,10 +76,17 @@ def test_srhd_mixed_quadrants(gpu_lib):
     assert g.health() == 0
 
 
+def _max_lorentz(prims, dim):
+    P = np.asarray(prims).reshape(-1, dim + 2)
+    v2 = (P[:, 1:1 + dim] ** 2).sum(axis=1)
+    return float(1.0 / np.sqrt(1.0 - v2.max()))
+
+
 def test_srhd_full_size_quadrants(gpu_lib):
-    # the SRHD workload at its full size (1024^2 quadrants, RK2, gamma 5/3):
-    # one FP64 update against the oracle (state and primitives, 1e-9 as
-    # above), and the FP32-mixed path against the FP64 one (1e-3)
+    # the SRHD workload at its full size (1024^2 quadrants, RK2, gamma 5/3,
+    # fastest quadrant at W = 9.14): three FP64 updates against the oracle
+    # (state and primitives, 1e-9 as above), and the FP32-mixed path against
+    # the FP64 one (1e-3); no con2prim failure (health bit 2) in either
     kw = dict(xmin=[0.0] * 2, xmax=[1.0] * 2, solverName="SRHDRoe", integratorName="RungeKutta2",
               slopeLimiter="Superbee", boundaryMethods=[{"min": "FREEFLOW", "max": "FREEFLOW"}] * 2,
               defs={"idealGas_heatCapacityRatio": 5.0 / 3.0})
@@ -88,14 +96,42 @@ def test_srhd_full_size_quadrants(gpu_lib):
     g, m = _mk(d64), _mk(d32)
     g.set_state(q)
     m.set_state(q)
+    assert _max_lorentz(g.get_primitives(), 2) > 9.0
     o = oracle.OracleSim(d64)
     o.set_state(q)
-    assert g.update() == pytest.approx(o.update(), rel=1e-10)
-    m.update()
+    for _ in range(3):
+        assert g.update() == pytest.approx(o.update(), rel=1e-9)
+        m.update()
     assert _relerr(g.get_state(), o.get_state()) < 1e-9
     assert _relerr(g.get_primitives(), o.get_primitives()) < 1e-9
     assert _relerr(m.get_state(), g.get_state()) < 1e-3
+    assert _max_lorentz(m.get_primitives(), 2) > 9.0
     assert g.health() == 0 and m.health() == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_srhd_lorentz_guard_keeps_primitives(gpu_lib, precision):
+    # Forced from the host: a uniform conserved state D = 1, S = (4, 0),
+    # tau = 1 has |S| = 2 (tau + D), beyond any physical state.  With the
+    # velocity margin srhd_solvePrimVelEpsilon = 0 the Newton bracket starts
+    # at P = |S| - tau - D = 2, where |v| = |S| / (tau + D + P) = 1 exactly
+    # and the Lorentz factor would overflow (res/SRHDCommon.cl:163-165).
+    # The recovery stops there, keeps every cell's previous primitives and
+    # raises health bit 2; the uniform state has zero flux divergence, so
+    # it is unchanged.
+    d = config.make_desc([16, 16], xmin=[0.0] * 2, xmax=[1.0] * 2, solverName="SRHDRoe",
+                         integratorName="RungeKutta2", precision=precision,
+                         defs={"idealGas_heatCapacityRatio": 5.0 / 3.0, "srhd_solvePrimVelEpsilon": 0.0})
+    V = config.volume(d)
+    cons = np.tile([1.0, 4.0, 0.0, 1.0], V)
+    prims = np.tile([0.5, 0.3, 0.0, 1.0], V)
+    g = _mk(d)
+    g.restore(cons, 0.0, 0, prims=prims)
+    dt = g.update()
+    assert np.isfinite(dt) and dt > 0
+    assert g.health() & 4
+    np.testing.assert_array_equal(g.get_primitives(), prims)
+    np.testing.assert_array_equal(g.get_state(), cons)
 
 
 MAXWELL_CASES = [

@@ -77,3 +77,37 @@ def test_riemann_batch_mix_is_deterministic_and_filtered():
         pmin = min(left[2], right[2])
         assert 0.05 * pmin * (1 - 1e-9) <= ps <= 3.0 * pmin * (1 + 1e-9)
     assert dims == {1, 2}
+
+
+def test_srhd_quadrants_reach_lorentz_factor_nine():
+    # rho in [.1, 1], p in [.01, 1], W in [3, 10] with the fastest quadrant
+    # in [9, 10]; one shared direction within +-.05 rad
+    rho, p, W, vx, vy = ics.srhd_quadrant_states()
+    assert ((0.1 <= rho) & (rho <= 1.0)).all() and ((0.01 <= p) & (p <= 1.0)).all()
+    assert W.max() >= 9.0 and ((3.0 <= W) & (W <= 10.0)).all()
+    v = np.hypot(vx, vy)
+    np.testing.assert_allclose(1.0 / np.sqrt(1.0 - v * v), W, rtol=1e-10)
+    ang = np.arctan2(vy, vx)
+    assert np.ptp(np.unwrap(ang)) <= 0.1 + 1e-12
+    # relative Lorentz factor of every pair W1 W2 (1 - v1.v2) stays below 1.5
+    for i in range(4):
+        for j in range(i + 1, 4):
+            wrel = W[i] * W[j] * (1.0 - vx[i] * vx[j] - vy[i] * vy[j])
+            assert 1.0 <= wrel < 1.5
+
+
+def test_srhd_quadrants_stay_physical_in_the_oracle():
+    # 64^2, RK2, Superbee, CFL .5: every primitive stays finite and above the
+    # floors for 30 steps, and the fastest flow keeps W > 9
+    import oracle
+
+    d = config.make_desc([64, 64], xmin=[0, 0], xmax=[1, 1], solverName="SRHDRoe", integratorName="RungeKutta2",
+                         cfl=0.5, defs={"idealGas_heatCapacityRatio": 5.0 / 3.0})
+    o = oracle.OracleSim(d)
+    o.set_state(ics.srhd_quadrants(d))
+    for _ in range(30):
+        assert np.isfinite(o.update())
+    P = o.get_primitives().reshape(64, 64, 4)
+    assert np.isfinite(P).all() and P[..., 0].min() > 0.05 and P[..., 3].min() > 0.05
+    v2 = P[..., 1] ** 2 + P[..., 2] ** 2
+    assert 1.0 / np.sqrt(1.0 - v2.max()) > 9.0
