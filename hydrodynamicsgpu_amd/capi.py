@@ -80,6 +80,7 @@ def lib():
         L.hydro_batch_member_health.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_uint)]
         L.hydro_batch_set_decomposition.argtypes = [vp, ip, ip]
         L.hydro_batch_autotune.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), dp]
+        L.hydro_calc_timestep_noalloc.argtypes = [vp, vp, sz, vp, ctypes.c_double, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -267,7 +268,7 @@ EXPORTED_SYMBOLS = [
     "hydro_batch_create", "hydro_batch_destroy", "hydro_batch_set_state",
     "hydro_batch_get_state", "hydro_batch_run", "hydro_batch_sync", "hydro_batch_member_dt",
     "hydro_batch_interior_cells", "hydro_batch_member_time", "hydro_batch_member_health",
-    "hydro_batch_set_decomposition", "hydro_batch_autotune",
+    "hydro_batch_set_decomposition", "hydro_batch_autotune", "hydro_calc_timestep_noalloc",
 ]
 
 def tableau(integrator):

@@ -217,6 +217,33 @@ int hydro_sim_boundary(hydro_sim* sim);
  * start-of-step eigen data and returns cfl * min cell dt (FP64). */
 int hydro_sim_calc_timestep(hydro_sim* sim, double* dt_out);
 
+/* No counterpart in the reference.  The CFL timestep of
+ * hydro_sim_calc_timestep for a state the caller keeps in device memory,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls
+ * no allocating routine: its scratch (the CFL kernel's per-wave minima and
+ * the clock of the dt tail) is a file-scope device array of
+ * HYDRO_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU race.
+ *   d        descriptor as for hydro_sim_create (sizes include ghost cells)
+ *   state    device pointer: N state planes, cell x + SX*(y + SY*z), plane
+ *            stride `plane` elements (>= the cell count), double for FP64,
+ *            float for FP32_MIXED (the high words of compensated storage);
+ *            ghost cells filled (boundary applied), as the CFL reads them
+ *   prims    SRHD: device pointer to the FP64 primitive planes (same stride);
+ *            NULL for the other equations
+ *   t        simulated time, for HYDRO_FLAG_CLIP_TO_TEND
+ *   dt_out   device pointer, receives cfl * min cell dt (or the fixed dt)
+ *   flags_out device pointer or NULL: health bit 0 (see hydro_sim_health) is
+ *            OR-ed in
+ *   stream   hipStream_t, or NULL for the null stream; nothing is waited on
+ * Grids whose CFL pass would need more minima than the array holds are
+ * rejected with HYDRO_E_INVALID (none of the supported kernel choices
+ * needs more than 65536). */
+#define HYDRO_NOALLOC_SCRATCH_BYTES (256u + 8u * 65536u)
+int hydro_calc_timestep_noalloc(const hydro_sim_desc* d, const void* state, size_t plane, const double* prims,
+                                double t, double* dt_out, unsigned* flags_out, void* stream);
+
 /* Replaces Solver::step(dt) for the solver (src/Solver/Roe.cpp:94-99,
  * src/Solver/HLL.cpp:234-241) including the integrator stages
  * (include/HydroGPU/Integrator/RungeKutta.h:59-125,
