@@ -44,6 +44,8 @@ def parse():
                     help="headline path with compensated FP32 state storage (1) or plain FP32 (0)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the Sod L1 accuracy leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 512-grid leg")
+    ap.add_argument("--no-long-accuracy", action="store_true",
+                    help="skip the long-horizon error and drift legs (C2 t=.5, C3 t=.1, C4 t=.2 against 2x grids)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the HBM traffic measurement (rocprofv3 FETCH_SIZE / WRITE_SIZE child runs)")
     ap.add_argument("--no-configs", action="store_true",
@@ -206,11 +208,48 @@ def time_desc(d, q0, steps, warmup):
     return out
 
 
-def cpu_steps(d, q0, seconds, max_steps):
+def host_cpu():
+    """The host the CPU baseline runs on: logical CPUs (nproc), the CPUs this
+    process may run on, OMP_NUM_THREADS, and the CPU model and socket count
+    from /proc/cpuinfo (what lscpu reports)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity_cpus"] = None
+    info["omp_num_threads"] = os.environ.get("OMP_NUM_THREADS")
+    model, sockets = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model is None:
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    info["model"] = model
+    info["sockets"] = len(sockets) or None
+    return info
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU this process may use, or
+    OMP_NUM_THREADS when the environment sets it (a GPU job's share of a
+    shared host)."""
+    info = host_cpu()
+    share = info["affinity_cpus"] or info["nproc"] or 1
+    env = info["omp_num_threads"]
+    if env and env.isdigit() and int(env) > 0:
+        return min(int(env), share)
+    return share
+
+
+def cpu_steps(d, q0, seconds, max_steps, threads=None):
     """The FP64 oracle on the host cores: cell-updates/s over a bounded number of steps."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = threads or cpu_threads()
     oracle.lib().oracle_set_threads(threads)
     o = oracle.OracleSim(d)
     o.set_state(q0)
@@ -259,7 +298,7 @@ def traffic_leg(kahan):
 
     work = tempfile.mkdtemp(prefix="hydro_pmc_", dir="/tmp")
     base = ["--steps", "3", "--warmup", "1", "--no-cpu", "--no-fp64", "--no-accuracy", "--no-batch", "--no-configs",
-            "--no-traffic", "--kahan", str(int(kahan))]
+            "--no-traffic", "--no-long-accuracy", "--kahan", str(int(kahan))]
     probe = ["--copy-probe", "256"]
     f = _pmc("FETCH_SIZE", base, os.path.join(work, "f"))
     w = _pmc("WRITE_SIZE", base, os.path.join(work, "w"))
@@ -313,6 +352,7 @@ def config_legs(steps, warmup, with_cpu, cpu_seconds, only=None):
         sim.set_state(q)
         sim.run(2)
         sim.set_state(q)
+        tot0 = sim.totals()
         sync_all(sim)
         t0 = time.perf_counter()
         while sim.time()[0] < 0.2 - 1e-12:
@@ -320,30 +360,48 @@ def config_legs(steps, warmup, with_cpu, cpu_seconds, only=None):
         el = time.perf_counter() - t0
         _, nsteps, _ = sim.time()
         rho, _, _ = ics.primitives_euler(d, sim.get_state())
+        acc = {"l1": float(np.sum(np.abs(rho - er)[2:n - 2]) * (2.0 / n)),
+               "linf": float(np.max(np.abs(rho - er)[2:n - 2])),
+               "drift": _drift({"totals0": tot0, "totals1": sim.totals()})}
         sim.close()
-        return d, q, nsteps, el, float(np.sum(np.abs(rho - er)[2:n - 2]) * (2.0 / n))
+        return d, q, nsteps, el, acc
 
-    d, q, nsteps, el, l1 = sod_to_t("fp64", 0)
-    _, _, nsteps32, el32, l1_32 = sod_to_t("fp32_mixed", config.FLAG_KAHAN)
+    d, q, nsteps, el, acc64 = sod_to_t("fp64", 0)
+    _, _, nsteps32, el32, acc32 = sod_to_t("fp32_mixed", config.FLAG_KAHAN)
+    l1, l1_32 = acc64["l1"], acc32["l1"]
     c1 = {"workload": "euler1d_roe_sod_1024_rk2tvd_superbee_t0.2", "precision": "fp64", "steps": nsteps,
           "gpu_value": (n - 4) * nsteps / el, "unit": "cell-updates/s", "gpu_seconds_to_t": el, "l1_rho_vs_exact": l1,
           "gpu_mixed_value": (n - 4) * nsteps32 / el32, "gpu_mixed_seconds_to_t": el32, "mixed_steps": nsteps32,
           "l1_rho_vs_exact_mixed": l1_32,
+          "accuracy": {"t_end": 0.2, "mixed_form": "fp32_mixed+kahan", "reference": "exact Riemann solution",
+                       "l1": {"fp64": l1, "mixed": l1_32}, "l1_ratio": l1_32 / l1,
+                       "linf": {"fp64": acc64["linf"], "mixed": acc32["linf"]},
+                       "drift": {"fp64": acc64["drift"], "mixed": acc32["drift"]}},
           "note": "latency-bound: 1020 cells per step, every update in one single-workgroup launch per 64 steps, the host "
                   "reads t between launches; mixed = FP32 compensated storage"}
     if with_cpu:
         import oracle
 
-        o = oracle.OracleSim(d)
-        o.set_state(q)
-        ocount = 0
-        t0 = time.perf_counter()
-        while o.time < 0.2 - 1e-14:
-            o.update()
-            ocount += 1
-        c1["cpu_seconds_to_t"] = time.perf_counter() - t0
-        c1["cpu_value"] = (n - 4) * ocount / c1["cpu_seconds_to_t"]
-        c1["cpu_cores"] = 1
+        # 1020 cells per step: a thread team's fork and join per kernel can
+        # cost more than the work, so both one thread and the host share are
+        # timed and the faster is the baseline
+        runs = {}
+        for th in sorted({1, cpu_threads()}):
+            oracle.lib().oracle_set_threads(th)
+            o = oracle.OracleSim(d)
+            o.set_state(q)
+            ocount = 0
+            t0 = time.perf_counter()
+            while o.time < 0.2 - 1e-14:
+                o.update()
+                ocount += 1
+            runs[th] = (time.perf_counter() - t0, ocount)
+            o.close()
+        best = min(runs, key=lambda k: runs[k][0])
+        c1["cpu_seconds_to_t"] = runs[best][0]
+        c1["cpu_value"] = (n - 4) * runs[best][1] / runs[best][0]
+        c1["cpu_cores"] = best
+        c1["cpu_seconds_by_threads"] = {str(k): v[0] for k, v in runs.items()}
     res["C1"] = c1
     if only == "C1":
         return res
@@ -354,7 +412,7 @@ def _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only):
     """C3 and C4, each timed in FP64 and FP32-mixed (`only`: one of them)."""
     from hydrodynamicsgpu_amd import config, ics
 
-    def pair(name, workload, make, ic, N, S, b_extra=0, mixed_flags=0):
+    def pair(name, workload, make, ic, N, S, b_extra=0, mixed_flags=0, stage_extra=0):
         if only not in (None, name):
             return
         d64 = make("fp64", 0)
@@ -366,14 +424,23 @@ def _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only):
         v32 = cells * steps / g32["elapsed"]
         B32 = N * 4 * (3 * S - 1) + b_extra
         B64 = N * 8 * (3 * S - 1) + b_extra
-        a32 = cells * B32 / S / (g32["stage_ms"] * 1e-3) / 1e9
-        a64 = cells * B64 / S / (g64["stage_ms"] * 1e-3) / 1e9
+        # a stage launch moves its own algorithmic bytes: N b (3S - 1) / S
+        # of state, plus what the stage itself reads besides (SRHD: the FP64
+        # primitives of the start-of-step state, N * 8); the rest of b_extra
+        # belongs to the kernels between the stages and counts at step level
+        a32 = cells * (N * 4 * (3 * S - 1) / S + stage_extra) / (g32["stage_ms"] * 1e-3) / 1e9
+        a64 = cells * (N * 8 * (3 * S - 1) / S + stage_extra) / (g64["stage_ms"] * 1e-3) / 1e9
+        ms32 = g32["elapsed"] * 1e3 / steps
+        ms64 = g64["elapsed"] * 1e3 / steps
         diff = np.abs(g32["state"] - g64["state"]).reshape(-1, N)[:, 0]
         out = {"workload": workload, "interior_cells": cells, "gpu_fp64_value": v64, "gpu_mixed_value": v32,
                "unit": "cell-updates/s", "mixed_over_fp64": v32 / v64,
+               "ms_per_step": {"fp64": ms64, "mixed": ms32},
                "stage_kernel_avg_ms": {"fp64": g64["stage_ms"], "mixed": g32["stage_ms"]},
                "roofline_frac_mixed": a32 / HBM_PEAK_GBS, "roofline_frac_fp64": a64 / HBM_PEAK_GBS,
-               "B_mixed": B32, "B_fp64": B64, "l1_rho_mixed_minus_fp64": float(np.mean(diff)),
+               "roofline_frac_step_mixed": cells * B32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               "roofline_frac_step_fp64": cells * B64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               "B_mixed": B32, "B_fp64": B64, "l1_rho_mixed_minus_fp64_after_steps": float(np.mean(diff)),
                "health": {"fp64": g64["health"], "mixed": g32["health"]}}
         if with_cpu:
             ns, el, th = cpu_steps(d64, q0, cpu_seconds, 3)
@@ -394,14 +461,136 @@ def _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only):
                                 integratorName="RungeKutta2", defs={"idealGas_heatCapacityRatio": 5.0 / 3.0},
                                 precision=prec, flags=config.FLAG_REFERENCE | flags)
 
-    pair("C4", "srhd2d_roe_quadrants_1024^2_rk2", c4, ics.srhd_quadrants, 4, 2, b_extra=4 * 8 * 4)
+    pair("C4", "srhd2d_roe_quadrants_1024^2_rk2", c4, ics.srhd_quadrants, 4, 2, b_extra=4 * 8 * 4, stage_extra=4 * 8)
+    if "C4" in res:
+        from hydrodynamicsgpu_amd import capi
+
+        sim = capi.Sim(c4("fp64", 0))
+        sim.set_state(ics.srhd_quadrants(sim.desc))
+        P = sim.get_primitives().reshape(-1, 4)
+        sim.close()
+        res["C4"]["max_lorentz_factor_initial"] = float(1.0 / np.sqrt(1.0 - (P[:, 1] ** 2 + P[:, 2] ** 2).max()))
+    return res
+
+
+def _progress(msg):
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
+
+
+def _run_to(desc, q0, t_end, chunk=128):
+    """One grid from q0 to t_end (the last step clipped to it): the final
+    state, its FP64 conserved totals at the start and the end, and the clock."""
+    from hydrodynamicsgpu_amd import capi
+
+    sim = capi.Sim(desc)
+    sim.set_state(q0)
+    tot0 = sim.totals()
+    while sim.time()[0] < t_end * (1 - 1e-13):
+        sim.run(chunk)
+    t, steps, _ = sim.time()
+    out = dict(state=sim.get_state(), totals0=tot0, totals1=sim.totals(), t=t, steps=steps, health=sim.health())
+    sim.close()
+    return out
+
+
+def _interior_rho(desc, aos):
+    N = desc.dim + 2
+    shape = tuple(desc.size[a] for a in reversed(range(desc.dim)))
+    rho = np.asarray(aos).reshape(-1, N)[:, 0].reshape(shape)
+    return rho[(slice(2, -2),) * desc.dim]
+
+
+def _restrict(fine, dim, factor=2):
+    """Block average of a fine interior onto the coarse interior."""
+    shp = []
+    for n in fine.shape:
+        shp += [n // factor, factor]
+    return fine.reshape(shp).mean(axis=tuple(range(1, 2 * dim, 2)))
+
+
+def _drift(r):
+    t0, t1 = np.asarray(r["totals0"]), np.asarray(r["totals1"])
+    rel = [float((b - a) / abs(a)) if abs(a) > 1e-12 * abs(t0[0]) else None for a, b in zip(t0, t1)]
+    return {"relative": rel, "absolute": [float(b - a) for a, b in zip(t0, t1)]}
+
+
+def accuracy_legs(only=None):
+    """Error and conservation at long horizons, per configuration: the FP32
+    mixed and the FP64 GPU runs of the scheme against an FP64 run on a grid
+    twice as fine over the same interior region (the same initial
+    condition: the refined grid's noise is its parent cell's), restricted to
+    the base grid by block averaging; density L1 (integral of |e| over the
+    interior) and L-infinity; and the drift of the FP64 conserved totals
+    (interior sums) from t = 0 to t_end.  The FP64 GPU path stands for the
+    FP64 oracle here (it matches it to 1e-10 in the parity tests); running
+    the CPU oracle this far would take hours."""
+    from hydrodynamicsgpu_amd import config, ics
+
+    def kh(prec, flags, t_end):
+        return config.make_desc([2048, 2048], xmin=[-0.5, -0.5], xmax=[0.5, 0.5], integratorName="RungeKutta4",
+                                slopeLimiter="MonotizedCentral",
+                                boundaryMethods=[{"min": "PERIODIC", "max": "PERIODIC"}] * 2, cfl=0.5,
+                                precision=prec, flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND | flags,
+                                t_end=t_end)
+
+    def blast(prec, flags, t_end):
+        return config.make_desc([256, 256, 256], xmin=[-1] * 3, xmax=[1] * 3, solverName="EulerHLLC",
+                                integratorName="RungeKutta3TVD", slopeLimiter="MinMod",
+                                boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4, precision=prec,
+                                flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND | flags, t_end=t_end)
+
+    def quads(prec, flags, t_end):
+        return config.make_desc([1024, 1024], xmin=[0, 0], xmax=[1, 1], solverName="SRHDRoe",
+                                integratorName="RungeKutta2", defs={"idealGas_heatCapacityRatio": 5.0 / 3.0},
+                                precision=prec, flags=config.FLAG_REFERENCE | config.FLAG_CLIP_TO_TEND | flags,
+                                t_end=t_end)
+
+    legs = [
+        ("C2", kh, 0.5, lambda d, cells=None: ics.shear_kh(d, cells=cells), 0, "fp32_mixed"),
+        ("C3", blast, 0.1, lambda d, cells=None: ics.blast(d, cells=cells), config.FLAG_KAHAN,
+         "fp32_mixed+kahan"),
+        ("C4", quads, 0.2, lambda d, cells=None: ics.srhd_quadrants(d), 0, "fp32_mixed"),
+    ]
+    res = {}
+    for name, make, t_end, ic, mflags, mname in legs:
+        if only not in (None, name):
+            continue
+        t0 = time.perf_counter()
+        d64 = make("fp64", 0, t_end)
+        q = ic(d64)
+        r64 = _run_to(d64, q, t_end)
+        r32 = _run_to(make("fp32_mixed", mflags, t_end), q, t_end)
+        del q
+        df = ics.refined(d64)
+        qf = ic(df, cells=ics.parent_cells(d64, df))
+        rf = _run_to(df, qf, t_end)
+        del qf
+        ref = _restrict(_interior_rho(df, rf["state"]), d64.dim)
+        dV = float(np.prod([(d64.xmax[a] - d64.xmin[a]) / d64.size[a] for a in range(d64.dim)]))
+        e64 = _interior_rho(d64, r64["state"]) - ref
+        e32 = _interior_rho(d64, r32["state"]) - ref
+        l1_64, l1_32 = float(np.sum(np.abs(e64)) * dV), float(np.sum(np.abs(e32)) * dV)
+        dm = _interior_rho(d64, r32["state"]) - _interior_rho(d64, r64["state"])
+        res[name] = {
+            "t_end": t_end, "mixed_form": mname, "steps": {"fp64": r64["steps"], "mixed": r32["steps"],
+                                                          "reference_2x": rf["steps"]},
+            "reference": "FP64 GPU run at 2x resolution (%s cells incl. ghosts), block-averaged"
+                         % "x".join(str(df.size[a]) for a in range(df.dim)),
+            "l1": {"fp64": l1_64, "mixed": l1_32}, "l1_ratio": l1_32 / l1_64,
+            "linf": {"fp64": float(np.max(np.abs(e64))), "mixed": float(np.max(np.abs(e32)))},
+            "mixed_minus_fp64": {"l1": float(np.sum(np.abs(dm)) * dV), "linf": float(np.max(np.abs(dm)))},
+            "drift": {"fp64": _drift(r64), "mixed": _drift(r32), "reference_2x": _drift(rf)},
+            "health": {"fp64": r64["health"], "mixed": r32["health"], "reference_2x": rf["health"]},
+        }
+        del rf, r64, r32
+        _progress("accuracy %s to t=%g: %.1f s" % (name, t_end, time.perf_counter() - t0))
     return res
 
 
 def cpu_baseline(n, seconds, q0):
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     oracle.lib().oracle_set_threads(threads)
     d = make_desc(n, "fp64")
     o = oracle.OracleSim(d)
@@ -418,7 +607,7 @@ def cpu_baseline(n, seconds, q0):
     cells = (n - 4) ** 2
     return dict(value=cells * steps / el, unit="cell-updates/s", cores=threads, kind="port",
                 sample="%dx%d KH RK4 grid, %d steps after 1 untimed, FP64 oracle restatement, %d OpenMP threads"
-                       % (n, n, steps, threads))
+                       % (n, n, steps, threads), host=host_cpu())
 
 
 def copy_probe(mib):
@@ -521,10 +710,29 @@ def main():
         if t:
             out["roofline"]["traffic"] = t["bytes_per_launch"]
             out["roofline"]["traffic_detail"] = t
+    _progress("headline legs done")
     if not a.no_batch:
         out["batch"] = batch_leg(a.steps, a.warmup)
+        _progress("batch leg done")
     if not a.no_configs:
         out["configs"] = config_legs(a.steps, a.warmup, not a.no_cpu, min(a.cpu_seconds, 10.0))
+        _progress("configuration legs done")
+        c2 = {"workload": "euler2d_roe_kh_2048^2_rk4_mc", "interior_cells": cells, "gpu_mixed_value": value,
+              "unit": "cell-updates/s", "ms_per_step": {"mixed": out["ms_per_step"]},
+              "roofline_frac_mixed": out["roofline"]["frac"],
+              "roofline_frac_step_mixed": cells * B / (out["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+              "B_mixed": B, "B_fp64": N * 8 * (3 * S - 1)}
+        if "fp64" in out:
+            c2.update(gpu_fp64_value=out["fp64"]["value"], mixed_over_fp64=out["mixed_over_fp64"])
+            c2["ms_per_step"]["fp64"] = out["fp64"]["ms_per_step"]
+            c2["roofline_frac_step_fp64"] = (cells * c2["B_fp64"] / (out["fp64"]["ms_per_step"] * 1e-3) / 1e9
+                                             / HBM_PEAK_GBS)
+        out["configs"] = dict({"C2": c2}, **out["configs"])
+    if not a.no_long_accuracy:
+        acc = accuracy_legs()
+        out.setdefault("configs", {})
+        for k, v in acc.items():
+            out["configs"].setdefault(k, {})["accuracy"] = v
     if not a.no_accuracy:
         # L1 error at fixed resolution against the exact Sod solution: the
         # mixed path must stay within 2x of the FP64 path

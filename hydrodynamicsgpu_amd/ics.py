@@ -83,24 +83,60 @@ def sod(desc, gamma=None):
     return build_state_euler(desc, rho, zero, zero, zero, p, gamma)
 
 
-def shear_kh(desc, seed=1234):
+def refined(desc, factor=2):
+    """A grid `factor` times finer over the same interior region: the
+    interior cells of `desc` split factor^dim ways, 2 ghost cells per side
+    of the fine size (so the periodic period and the mirror walls stay
+    where they are).  Returns a copy of the descriptor."""
+    import copy
+
+    f = copy.deepcopy(desc)
+    for a in range(desc.dim):
+        n = desc.size[a]
+        dx = (desc.xmax[a] - desc.xmin[a]) / n
+        f.size[a] = factor * (n - 4) + 4
+        f.xmin[a] = desc.xmin[a] + 2 * dx - 2 * dx / factor
+        f.xmax[a] = desc.xmax[a] - 2 * dx + 2 * dx / factor
+    return f
+
+
+def parent_cells(coarse, fine, factor=2):
+    """Flat index of the coarse cell containing each fine cell (grid shape of
+    `fine`), for grids related by `refined`; fine ghost cells map to the
+    nearest coarse cell."""
+    idx = []
+    for a in range(3):
+        nf = fine.size[a]
+        i = np.arange(nf)
+        if a < fine.dim:
+            i = np.clip(2 + np.floor_divide(i - 2, factor), 0, coarse.size[a] - 1)
+        idx.append(i)
+    iz, iy, ix = np.meshgrid(idx[2], idx[1], idx[0], indexing="ij")
+    return ((iz * coarse.size[1] + iy) * coarse.size[0] + ix).astype(np.uint64)
+
+
+def shear_kh(desc, seed=1234, cells=None):
     """2D Kelvin-Helmholtz shear layer: rho=2, vx=+.5 for |y|<.25 else rho=1,
-    vx=-.5; p=2.5; vy = .01 sin(4 pi x) + 1e-3 (2U-1)."""
-    x, y, z, cells = grid(desc)
+    vx=-.5; p=2.5; vy = .01 sin(4 pi x) + 1e-3 (2U-1).  `cells` (grid shape)
+    names the hash counter of each cell, by default the cell's own index;
+    a refined grid passes `parent_cells` so that it carries the same
+    piecewise-constant noise as the grid it refines."""
+    x, y, z, own = grid(desc)
     inside = np.abs(y) < 0.25
     rho = np.where(inside, 2.0, 1.0)
     vx = np.where(inside, 0.5, -0.5)
-    u = uniform01(seed, cells)
+    u = uniform01(seed, own if cells is None else cells)
     vy = 0.01 * np.sin(4.0 * np.pi * x) + 1e-3 * (2.0 * u - 1.0)
     p = np.full_like(rho, 2.5)
     return build_state_euler(desc, rho, vx, vy, np.zeros_like(rho), p)
 
 
-def blast(desc, seed=4321):
-    """3D blast: rho = 1 + 1e-4 (2U-1), u=0, p=10 inside r<.2 else .1."""
-    x, y, z, cells = grid(desc)
+def blast(desc, seed=4321, cells=None):
+    """3D blast: rho = 1 + 1e-4 (2U-1), u=0, p=10 inside r<.2 else .1
+    (`cells` as for shear_kh)."""
+    x, y, z, own = grid(desc)
     r = np.sqrt(x * x + y * y + z * z)
-    u = uniform01(seed, cells)
+    u = uniform01(seed, own if cells is None else cells)
     rho = 1.0 + 1e-4 * (2.0 * u - 1.0)
     p = np.where(r < 0.2, 10.0, 0.1)
     zero = np.zeros_like(rho)
