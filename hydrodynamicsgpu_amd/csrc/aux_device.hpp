@@ -254,9 +254,11 @@ struct DtSolver {
 };
 
 // Partial CFL minima carry "some entry was NaN" in the sign bit: the value
-// is the minimum over the finite entries (>= 0, or +inf), as findMinTimestep's
-// min() skips NaN entries one by one (res/Common.cl:9-40); the flag only
-// feeds the health bit.  dt_flag encodes, dt_merge folds one into a running
+// is the minimum over the finite entries (>= 0, or +inf).  This port skips
+// NaN entries on purpose (fmin) and raises a health bit; the reference's
+// reduction (res/Common.cl:9-40) is a ternary (acc < el ? acc : el), whose
+// result with a NaN entry depends on where the NaN falls in the order of
+// the reduction.  The flag only feeds the health bit.  dt_flag encodes, dt_merge folds one into a running
 // (minimum, flag) pair.
 HYDRO_DEV double dt_flag(double dt, bool nan) { return nan ? copysign(dt, -1.0) : dt; }
 HYDRO_DEV void dt_merge(double v, double& m, bool& nan) {

@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_hydro.so")
+# ORACLE_LIB names another build of the same sources (the sanitizer build
+# of tests/test_sanitizers.py)
+_LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle_hydro.so")
 _lib = None
 
 ORACLE_OPT_FRESH_BASIS_FIXED_DT = 0x1
