@@ -233,6 +233,7 @@ struct hydro_sim {
   int rows = 32, wpb = 4;
   int zwaves = 8;  // first-order 3D kernel: waves per workgroup
   int zshare = 0;  // first-order 3D kernel: modes whose rows go through LDS (bit per CombineMode)
+  int zmarch = 0;  // first-order 3D kernel: z-march form with this many cells per lane (0: y-march form)
   int dt_rows = 0;  // 2D CFL strip march: rows per wave (0: chosen from the grid size)
   CflGrid cg;       // the CFL pass's view of this grid
   // measurement: event pairs around fused stage launches
@@ -417,8 +418,19 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
     a.nchunks = (s->sz[1] - 4 + a.rows - 1) / a.rows;
     a.zwaves = s->zwaves;
     a.zshare = s->zshare;
-    const int nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
-    a.nwaves = a.nstrips * a.nchunks * nzb;
+    a.zmarch = s->zmarch;
+    if (s->zmarch) {
+      // z-march: strips of 64 * cells-per-lane - 2 output columns, y-blocks
+      // of W - 2 output rows, z-chunks of `rows` planes
+      const int cpl = s->zmarch;
+      a.nstrips = (interior_x + 64 * cpl - 3) / (64 * cpl - 2);
+      a.nchunks = (s->sz[1] - 4 + s->zwaves - 3) / (s->zwaves - 2);
+      const int nzc = (s->sz[2] - 4 + a.rows - 1) / a.rows;
+      a.nwaves = a.nstrips * a.nchunks * nzc;
+    } else {
+      const int nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
+      a.nwaves = a.nstrips * a.nchunks * nzb;
+    }
   } else {
     // 3D Roe family: rows = z-planes per workgroup, nchunks = y-chunks, nwaves = workgroups
     const int ry = stage3d_rows_per_block(kernel_kind(s->d));
@@ -763,6 +775,27 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     if (const char* e = getenv("HYDRO_3D_ZSHARE")) s->zshare = (int)strtol(e, nullptr, 0);
     const long long nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
     s->rows = pick_rows_3d_r1(strips * nzb, s->sz[1] - 4, ncu * (8 / s->zwaves));
+    // the z-march form: FP32 two cells per lane in 8-wave workgroups (zm2)
+    // or one cell per lane in 16-wave workgroups (zm1); FP64 one cell per
+    // lane in 8-wave workgroups (LDS holds 94 KB per workgroup in each).
+    // Measured on the 256^3 HLLC blast (stage average, FP32 compensated /
+    // FP64): y-march 523 / 963 us, zm1 520 / 953, zm2 400 / -, zm1w8 540 /
+    // 955.  FP32 takes zm2 (two strips of 126 columns cover the 252
+    // interior columns, against five strips of 60); FP64 keeps the y-march.
+    std::string form = s->mixed ? "zm2" : "r1";
+    if (const char* e = getenv("HYDRO_3D_FORM")) form = e;
+    if (form == "zm1" || form == "zm2" || form == "zm1w8") {
+      s->zmarch = (form == "zm2" && s->mixed) ? 2 : 1;
+      s->zwaves = (form == "zm1" && s->mixed) ? 16 : 8;
+      const int cpl = s->zmarch;
+      const long long zstrips = (s->sz[0] - 4 + 64 * cpl - 3) / (64 * cpl - 2);
+      const long long yblocks = (s->sz[1] - 4 + s->zwaves - 3) / (s->zwaves - 2);
+      const int N = s->N;
+      const long long lds = (long long)s->zwaves * (2 * N + 4 + N) * 64 * cpl * (long long)esize(s);
+      const long long per_cu = std::max(1LL, std::min(160LL * 1024 / lds, 32LL / s->zwaves));
+      s->rows = pick_planes_zm(zstrips * yblocks, s->sz[2] - 4, ncu * per_cu);
+      if (const char* e = getenv("HYDRO_3D_PLANES")) s->rows = std::max(1, atoi(e));
+    }
   } else if (s->dim == 3) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
     const int ry = stage3d_rows_per_block(kernel_kind(*d));

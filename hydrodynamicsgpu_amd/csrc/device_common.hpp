@@ -31,6 +31,28 @@ HYDRO_DEV double lane_from_right(double v) {
   return __builtin_bit_cast(double, p);
 }
 
+// Full-wave DPP rotations (wave_ror:1 / wave_rol:1): lane l receives lane
+// l-1 (ror) or lane l+1 (rol), modulo 64, so lane 0 receives lane 63 and
+// lane 63 lane 0.
+HYDRO_DEV float lane_rot_from_left(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, true));
+}
+HYDRO_DEV float lane_rot_from_right(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, true));
+}
+HYDRO_DEV double lane_rot_from_left(double v) {
+  int2 p = __builtin_bit_cast(int2, v);
+  p.x = __builtin_amdgcn_update_dpp(0, p.x, 0x13C, 0xF, 0xF, true);
+  p.y = __builtin_amdgcn_update_dpp(0, p.y, 0x13C, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, p);
+}
+HYDRO_DEV double lane_rot_from_right(double v) {
+  int2 p = __builtin_bit_cast(int2, v);
+  p.x = __builtin_amdgcn_update_dpp(0, p.x, 0x134, 0xF, 0xF, true);
+  p.y = __builtin_amdgcn_update_dpp(0, p.y, 0x134, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, p);
+}
+
 // ---------------------------------------------------------------- math
 template <typename T> HYDRO_DEV T tmin(T a, T b);
 template <typename T> HYDRO_DEV T tmax(T a, T b);

@@ -237,6 +237,25 @@ inline int pick_rows_3d_r1(long long columns, int ny, int ncu) {
   return best;
 }
 
+// z-chunk height of the z-march 3D kernel: `columns` (strips x y-blocks)
+// workgroups per chunk row, `slots` workgroups resident at once; a chunk of
+// P planes marches P + 2 (one halo plane on each side), and the launch takes
+// ceil(workgroups / slots) rounds
+inline int pick_planes_zm(long long columns, int nz, long long slots) {
+  int best = nz;
+  double best_cost = 1e300;
+  for (int p = 4; p <= nz; ++p) {
+    const long long wg = columns * ((nz + p - 1) / p);
+    const long long rounds = (wg + slots - 1) / slots;
+    const double cost = (double)rounds * (std::min(p, nz) + 2.0);
+    if (cost < best_cost * (1.0 - 1e-9)) {
+      best_cost = cost;
+      best = p;
+    }
+  }
+  return best;
+}
+
 inline int pick_rows_2d(long long strips, int ny, int simds, int min_waves) {
   int best = 32;
   double best_cost = 1e300;

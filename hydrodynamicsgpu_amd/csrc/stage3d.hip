@@ -573,9 +573,282 @@ __global__ void __launch_bounds__(64 * W) stage3d_r1s_kernel(StageArgs<T, S> a) 
   step(y1, false, (y1 - y0) & 1);
 }
 
+// Neighbour exchange along x of per-lane cell arrays.  CPL = 1: lane l
+// holds strip cell l; CPL = 2: lane l holds cells l and 64 + l (two
+// coalesced 64-wide halves), so the left neighbour of cell 64 is lane 63's
+// first cell and the right neighbour of cell 63 is lane 0's second cell:
+// full-wave rotations with the halves swapped at the wrapping lane.
+template <int CPL>
+struct XLanes;
+template <>
+struct XLanes<1> {
+  template <typename T, int M>
+  static HYDRO_DEV void from_left(const T v[1][M], T o[1][M]) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) o[0][i] = lane_from_left(v[0][i]);
+  }
+  template <typename T, int M>
+  static HYDRO_DEV void from_right(const T v[1][M], T o[1][M]) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) o[0][i] = lane_from_right(v[0][i]);
+  }
+};
+template <>
+struct XLanes<2> {
+  template <typename T, int M>
+  static HYDRO_DEV void from_left(const T v[2][M], T o[2][M]) {
+    const bool last = (threadIdx.x & 63) == 63;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      o[0][i] = lane_from_left(v[0][i]);
+      o[1][i] = lane_rot_from_left(last ? v[0][i] : v[1][i]);
+    }
+  }
+  template <typename T, int M>
+  static HYDRO_DEV void from_right(const T v[2][M], T o[2][M]) {
+    const bool first = (threadIdx.x & 63) == 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      o[0][i] = lane_rot_from_right(first ? v[1][i] : v[0][i]);
+      o[1][i] = lane_from_right(v[1][i]);
+    }
+  }
+};
+
+// z-march form of the first-order stage (HLL, HLLC).  A workgroup of W
+// waves covers one strip of 64 * CPL columns (x: lanes, CPL cells per lane,
+// DPP exchange) and W consecutive rows, one per wave: rows yb-1 .. yb+W-2,
+// of which yb .. yb+W-3 are output rows (the first and last are halo rows).
+// It marches a chunk of z-planes with the z stencil in registers: per plane
+// every wave loads and builds its own row once, publishes it in LDS,
+// computes the flux through its lower y-face from the row of the wave
+// below and publishes that; an output row takes its upper y-face from the
+// wave above.  A plane's cells are complete one plane later, when the flux
+// through their upper z-face exists.  Two barriers per plane.  Each cell is
+// loaded and built once per workgroup (W / (W-2) per output cell), every
+// face is computed once, and the divergence is summed in the order of
+// stage3d_r1_kernel, so both give the same bits.
+template <class Pol, typename T, typename S, bool KAHAN, int MODE, int W, int CPL>
+__global__ void __launch_bounds__(64 * W) stage3d_zm_kernel(StageArgs<T, S> a) {
+  constexpr int N = Pol::N;
+  constexpr int NS = Pol::NS, NF = Pol::NF, NP = NS + NF;
+  constexpr int XW = 64 * CPL;  // strip width in cells
+  constexpr int XO = XW - 2;    // output columns per strip
+  using Frozen = typename Pol::Frozen;
+  using Stage = typename Pol::Stage;
+  using Comb = StageCombine<T, S, KAHAN, N, MODE>;
+  __shared__ T shr[W][NP][XW];  // each wave's row of the current plane, packed
+  __shared__ T shf[W][N][XW];   // flux through each wave's lower y-face
+
+  const int nb = gridDim.x;
+  const int per = nb >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (lb >= a.nwaves) return;  // nwaves counts workgroups here (uniform per block)
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int sx = a.sx, sy = a.sy, sz = a.sz;
+  const int nzc = (sz - 4 + a.rows - 1) / a.rows;
+  const int zch = lb % nzc;  // z fastest: neighbouring chunks share their halo planes in L2
+  const int rest = lb / nzc;
+  const int yblk = rest % a.nchunks;
+  const int strip = rest / a.nchunks;
+  const int y = 1 + yblk * (W - 2) + w;  // this wave's row
+  const int yl = y < sy ? y : sy - 1;
+  const bool orow = w >= 1 && w <= W - 2 && y <= sy - 3;  // output row (wave-uniform)
+  const long long psz = (long long)sx * sy;
+  const int base = 1 + strip * XO;
+  int c[CPL];
+  uint32_t cl[CPL];
+  bool oc[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int p = k * 64 + lane;
+    c[k] = base + p;
+    cl[k] = c[k] < sx ? c[k] : sx - 1;
+    oc[k] = p >= 1 && p <= XW - 2 && c[k] <= sx - 3;
+  }
+
+  const PhysParams<T> pp = make_params<T>(a);
+  const T gamma = pp.gamma;
+  const double dtd = *a.dt;
+  const T idx = T(a.inv_dx[0]), idy = T(a.inv_dx[1]), idz = T(a.inv_dx[2]);
+  const Comb comb(a, dtd);
+
+  struct Raw {
+    S q0[N];
+    T dk[N];
+  };
+  struct Cell {
+    Stage st;
+    Frozen fz;
+  };
+  auto erow = [&](int plane) {
+    return (long long)__builtin_amdgcn_readfirstlane(plane) * psz + (long long)__builtin_amdgcn_readfirstlane(yl) * sx;
+  };
+  auto fetch = [&](int plane, Raw r[CPL]) {
+    const long long e = erow(plane);
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      comb.load_u0(e, cl[k], r[k].q0);
+      comb.load_dk(e, cl[k], r[k].dk);
+    }
+  };
+  auto build = [&](const Raw r[CPL], Cell C[CPL]) {
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      T u[N];
+      comb.stage_state(r[k].q0, r[k].dk, u);
+      C[k].st = Pol::stage(u, gamma);
+      C[k].fz = Pol::template frozen<S>(r[k].q0, gamma);
+    }
+  };
+  auto pack = [&](const Cell& C, T o[NP]) {
+    Pol::pack_stage(C.st, o);
+    Pol::pack(C.fz, o + NS);
+  };
+  auto unpack = [&](const T o[NP], Cell& C) {
+    C.st = Pol::unpack_stage(o);
+    C.fz = Pol::unpack(o + NS);
+  };
+  auto flux = [&](const Cell& L, const Cell& R, int side, T F[N]) {
+    const auto fa = Pol::face_a(L.fz, R.fz, L.st, R.st, side, true, pp);
+    Pol::face_b(fa, nullptr, nullptr, T(0), side, 0, F);
+  };
+  auto xdiv = [&](const Cell C[CPL], T div[CPL][N]) {
+    T pk[CPL][NP], pl[CPL][NP];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) pack(C[k], pk[k]);
+    XLanes<CPL>::template from_left<T, NP>(pk, pl);
+    T F[CPL][N], Fr[CPL][N];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      Cell L;
+      unpack(pl[k], L);
+      flux(L, C[k], 0, F[k]);
+    }
+    XLanes<CPL>::template from_right<T, N>(F, Fr);
+#pragma unroll
+    for (int k = 0; k < CPL; ++k)
+#pragma unroll
+      for (int s = 0; s < N; ++s) div[k][s] = (F[k][s] - Fr[k][s]) * idx;
+  };
+
+  const int z0 = 2 + zch * a.rows;
+  int z1 = z0 + a.rows;
+  if (z1 > sz - 2) z1 = sz - 2;  // output planes z0 .. z1-1; z0-1 and z1 are halo planes
+
+  // registers: P = the row at plane zn-1 and its loads rp; per cell of
+  // plane zn-1: x-divergence xp, y-face difference yd, lower z-face flux fzl
+  Cell P[CPL];
+  Raw rp[CPL], nx[CPL];
+  T xp[CPL][N], yd[CPL][N], fzl[CPL][N];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k)
+#pragma unroll
+    for (int s = 0; s < N; ++s) xp[k][s] = yd[k][s] = fzl[k][s] = T(0);
+  fetch(z0 - 1, rp);
+  build(rp, P);
+  fetch(z0, nx);
+  constexpr bool kLateFin = MODE == CM_FINAL || MODE == CM_GENERIC;
+
+#pragma unroll 1
+  for (int zn = z0; zn <= z1; ++zn) {
+    Raw cur[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) cur[k] = nx[k];
+    if (zn < z1) fetch(zn + 1, nx);  // prefetch the next plane (<= z1 <= sz-2)
+    typename Comb::Fin fin[CPL];
+    const long long ep = erow(zn - 1);
+    if (!kLateFin && orow && zn > z0) {
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) comb.fetch_fin_rest(ep, cl[k], rp[k].q0, rp[k].dk, fin[k]);
+    }
+    Cell C[CPL];
+    build(cur, C);
+    // face zn: lower z-face of plane zn, upper z-face of plane zn-1
+    T Fz[CPL][N];
+    if (orow) {
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) flux(P[k], C[k], 2, Fz[k]);
+      if (zn > z0) {
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+          if (!oc[k]) continue;
+          T deriv[N];
+#pragma unroll
+          for (int s = 0; s < N; ++s) {
+            T acc = xp[k][s];
+            acc += (fzl[k][s] - Fz[k][s]) * idz;
+            deriv[s] = acc + yd[k][s] * idy;
+          }
+          if (kLateFin) comb.fetch_fin_rest(ep, cl[k], rp[k].q0, rp[k].dk, fin[k]);
+          comb.finish(ep, c[k], fin[k], deriv);
+        }
+      }
+    }
+    if (zn < z1) {  // plane zn is an output plane: its x- and y-faces
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        T pk[NP];
+        pack(C[k], pk);
+#pragma unroll
+        for (int i = 0; i < NP; ++i) shr[w][i][k * 64 + lane] = pk[i];
+      }
+      __syncthreads();
+      T G[CPL][N];
+      if (w >= 1) {
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+          T pk[NP];
+#pragma unroll
+          for (int i = 0; i < NP; ++i) pk[i] = shr[w - 1][i][k * 64 + lane];
+          Cell L;
+          unpack(pk, L);
+          flux(L, C[k], 1, G[k]);
+#pragma unroll
+          for (int s = 0; s < N; ++s) shf[w][s][k * 64 + lane] = G[k][s];
+        }
+      }
+      if (orow) xdiv(C, xp);
+      __syncthreads();
+      if (orow) {
+#pragma unroll
+        for (int k = 0; k < CPL; ++k)
+#pragma unroll
+          for (int s = 0; s < N; ++s) {
+            yd[k][s] = G[k][s] - shf[w + 1][s][k * 64 + lane];
+            fzl[k][s] = Fz[k][s];
+          }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      P[k] = C[k];
+      rp[k] = cur[k];
+    }
+  }
+}
+
 template <class Pol, typename T, typename S, bool KAHAN, int MODE>
 static void launch3_r1_mode(const StageArgs<T, S>& a, hipStream_t st) {
   const int blocks = (a.nwaves + 7) & ~7;  // multiple of 8 for the XCD remap
+  if (a.zmarch) {
+    // z-march form: (waves per workgroup, cells per lane) as chosen on the
+    // host; the FP32 forms with 1024 threads or 2 cells per lane fit the
+    // 160 KB of LDS, the FP64 form takes 8 waves and 1 cell per lane
+    if constexpr (sizeof(T) == 4) {
+      if (a.zwaves == 16 && a.zmarch == 1) {
+        hipLaunchKernelGGL((stage3d_zm_kernel<Pol, T, S, KAHAN, MODE, 16, 1>), dim3(blocks), dim3(64 * 16), 0, st, a);
+        return;
+      }
+      if (a.zwaves == 8 && a.zmarch == 2) {
+        hipLaunchKernelGGL((stage3d_zm_kernel<Pol, T, S, KAHAN, MODE, 8, 2>), dim3(blocks), dim3(64 * 8), 0, st, a);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((stage3d_zm_kernel<Pol, T, S, KAHAN, MODE, 8, 1>), dim3(blocks), dim3(64 * 8), 0, st, a);
+    return;
+  }
   if (a.zwaves == 4 && ((a.zshare >> MODE) & 1))
     hipLaunchKernelGGL((stage3d_r1s_kernel<Pol, T, S, KAHAN, MODE, 4>), dim3(blocks), dim3(64 * 4), 0, st, a);
   else if (a.zwaves == 4)

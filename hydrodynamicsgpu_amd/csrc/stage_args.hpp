@@ -58,6 +58,7 @@ struct StageArgs {
   int nwaves;
   int zwaves;                // first-order 3D kernel: waves (z-planes) per workgroup
   int zshare;                // first-order 3D kernel: bit MODE set -> rows shared through LDS (4 waves)
+  int zmarch;                // first-order 3D kernel: 0 y-march over z-planes; else z-march, cells per lane
 };
 
 }  // namespace hydro

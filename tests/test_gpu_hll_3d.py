@@ -147,22 +147,35 @@ def test_blast_3d_full_size_one_update_matches_oracle(gpu_lib):
     assert _relerr(g.get_state(), o.get_state()) < 1e-11
 
 
+@pytest.mark.parametrize("integ,kahan", [("RungeKutta3TVD", False), ("RungeKutta3TVD", True), ("ForwardEuler", False),
+                                         ("RungeKutta4", True), ("RungeKutta4_3_8thsRule", False)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
-def test_3d_first_order_decomposition_invariant(gpu_lib, precision, monkeypatch):
-    # the y-chunk height, the workgroup shape and whether the plane below
-    # comes through LDS (rows shared between the waves of a workgroup)
-    # change which waves compute the chunk- and block-edge faces, not their
-    # arithmetic: the results are bit-identical
+def test_3d_first_order_decomposition_invariant(gpu_lib, precision, integ, kahan, monkeypatch):
+    # the two forms of the first-order 3D stage (y-march over z-planes, and
+    # z-march over rows with one or two cells per lane) and their
+    # decompositions (y-chunk height, workgroup shape, rows shared through
+    # LDS, z-chunk height) change which waves compute the chunk- and
+    # block-edge faces, not their arithmetic: the results are bit-identical,
+    # for every combination mode (first / middle / final, single stage,
+    # partial sums) and with compensated storage
     from hydrodynamicsgpu_amd import capi
 
-    d = config.make_desc([70, 45, 25], solverName="EulerHLLC", integratorName="RungeKutta3TVD",
-                         boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4, precision=precision)
+    if kahan and precision == "fp64":
+        pytest.skip("compensated storage is an FP32 form")
+    flags = config.FLAG_REFERENCE | (config.FLAG_KAHAN if kahan else 0)
+    d = config.make_desc([70, 45, 25], solverName="EulerHLLC", integratorName=integ,
+                         boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"}] * 3, cfl=0.4, precision=precision,
+                         flags=flags)
     q = _rand_state(d, 5)
     out = []
-    keys = ("HYDRO_ROWS_PER_WAVE", "HYDRO_3D_ZWAVES", "HYDRO_3D_ZSHARE")
-    # (rows per chunk, waves per workgroup, shared-row modes)
-    for case in ((None, None, None), ("8", None, None), ("41", None, None), (None, "8", None), ("13", "8", None),
-                 (None, "4", "0"), (None, "4", "0x3F"), ("13", "4", "0x3F")):
+    keys = ("HYDRO_3D_FORM", "HYDRO_ROWS_PER_WAVE", "HYDRO_3D_ZWAVES", "HYDRO_3D_ZSHARE", "HYDRO_3D_PLANES")
+    cases = (("r1", None, None, None, None), ("r1", "8", None, None, None), ("r1", "41", None, None, None),
+             ("r1", None, "8", None, None), ("r1", "13", "8", None, None), ("r1", None, "4", "0", None),
+             ("r1", None, "4", "0x3F", None), ("r1", "13", "4", "0x3F", None),
+             ("zm1", None, None, None, None), ("zm1", None, None, None, "5"), ("zm1", None, None, None, "21"),
+             ("zm2", None, None, None, None), ("zm2", None, None, None, "7"), ("zm1w8", None, None, None, "3"),
+             (None, None, None, None, None))
+    for case in cases:
         for k, v in zip(keys, case):
             if v is None:
                 monkeypatch.delenv(k, raising=False)
@@ -175,5 +188,5 @@ def test_3d_first_order_decomposition_invariant(gpu_lib, precision, monkeypatch)
         g.close()
     for k in keys:
         monkeypatch.delenv(k, raising=False)
-    for o in out[1:]:
-        np.testing.assert_array_equal(o, out[0])
+    for c, o in zip(cases[1:], out[1:]):
+        np.testing.assert_array_equal(o, out[0], err_msg=str(c))
