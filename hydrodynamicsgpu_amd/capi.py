@@ -81,6 +81,8 @@ def lib():
         L.hydro_batch_set_decomposition.argtypes = [vp, ip, ip]
         L.hydro_batch_autotune.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), dp]
         L.hydro_calc_timestep_noalloc.argtypes = [vp, vp, sz, vp, ctypes.c_double, vp, vp, vp]
+        L.hydro_sim_decomposition.argtypes = [vp, ctypes.POINTER(ip)]
+        L.hydro_sim_autotune.argtypes = [vp, ip, ctypes.POINTER(ip), dp]
         _lib = L
     return _lib
 
@@ -249,6 +251,21 @@ class Sim:
         check(lib().hydro_sim_stage_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def decomposition(self):
+        """(rows, waves per workgroup, 3D form, its waves per workgroup) of
+        the stage kernels."""
+        out = (ctypes.c_int * 4)()
+        check(lib().hydro_sim_decomposition(self._h, out))
+        return tuple(out)
+
+    def autotune(self, updates=4):
+        """Time the candidate decompositions, keep the fastest (the state is
+        restored); returns (decomposition, ms per update)."""
+        out = (ctypes.c_int * 4)()
+        ms = ctypes.c_double()
+        check(lib().hydro_sim_autotune(self._h, int(updates), out, ctypes.byref(ms)))
+        return tuple(out), ms.value
+
     @property
     def stream(self):
         return lib().hydro_sim_stream(self._h)
@@ -269,6 +286,7 @@ EXPORTED_SYMBOLS = [
     "hydro_batch_get_state", "hydro_batch_run", "hydro_batch_sync", "hydro_batch_member_dt",
     "hydro_batch_interior_cells", "hydro_batch_member_time", "hydro_batch_member_health",
     "hydro_batch_set_decomposition", "hydro_batch_autotune", "hydro_calc_timestep_noalloc",
+    "hydro_sim_decomposition", "hydro_sim_autotune",
 ]
 
 def tableau(integrator):

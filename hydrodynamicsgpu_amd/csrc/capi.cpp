@@ -16,6 +16,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -234,6 +238,8 @@ struct hydro_sim {
   int zwaves = 8;  // first-order 3D kernel: waves per workgroup
   int zshare = 0;  // first-order 3D kernel: modes whose rows go through LDS (bit per CombineMode)
   int zmarch = 0;  // first-order 3D kernel: z-march form with this many cells per lane (0: y-march form)
+  int form3d = 0;  // first-order 3D kernel: one of the kForm3d* forms
+  int ncu = 256;   // compute units of the device
   int dt_rows = 0;  // 2D CFL strip march: rows per wave (0: chosen from the grid size)
   CflGrid cg;       // the CFL pass's view of this grid
   // measurement: event pairs around fused stage launches
@@ -292,6 +298,33 @@ BoundaryArgs boundary_args(const hydro_sim* s) {
       b.negmask[axis][mm] = neg;
     }
   return b;
+}
+
+// first-order 3D stage forms: y-march over 4 z-planes per workgroup;
+// z-march with one cell per lane (16-wave workgroups in FP32, 8 in FP64),
+// two cells per lane (FP32, 8 waves), one cell per lane in 8 waves
+enum { kForm3dYMarch = 0, kForm3dZm1 = 1, kForm3dZm2 = 2, kForm3dZm1w8 = 3 };
+
+// applies a form with its default decomposition (zwaves: the y-march form's
+// workgroup height, 0 for the default)
+void set_form3d(hydro_sim* s, int form, int zwaves = 0) {
+  s->form3d = form;
+  if (form == kForm3dYMarch) {
+    const long long strips = (s->sz[0] - 4 + 59) / 60;
+    s->zmarch = 0;
+    s->zwaves = zwaves ? zwaves : stage3d_r1_waves(!s->mixed);
+    const long long nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
+    s->rows = pick_rows_3d_r1(strips * nzb, s->sz[1] - 4, s->ncu * (8 / s->zwaves));
+    return;
+  }
+  s->zmarch = (form == kForm3dZm2 && s->mixed) ? 2 : 1;
+  s->zwaves = (form == kForm3dZm1 && s->mixed) ? 16 : 8;
+  const int cpl = s->zmarch;
+  const long long zstrips = (s->sz[0] - 4 + 64 * cpl - 3) / (64 * cpl - 2);
+  const long long yblocks = (s->sz[1] - 4 + s->zwaves - 3) / (s->zwaves - 2);
+  const long long lds = (long long)s->zwaves * (3 * s->N + 4) * 64 * cpl * (long long)esize(s);
+  const long long per_cu = std::max(1LL, std::min(160LL * 1024 / lds, 32LL / s->zwaves));
+  s->rows = pick_planes_zm(zstrips * yblocks, s->sz[2] - 4, s->ncu * per_cu);
 }
 
 // true when the boundary rewrites every ghost cell (no side is NONE)
@@ -721,6 +754,93 @@ void hydro_sim_destroy(hydro_sim* s) {
   delete s;
 }
 
+// ------------------------------------------------------------ autotuning
+// Work decompositions of the fused stage kernels: rows per wave (2D), rows
+// or planes per chunk (3D), waves per workgroup (2D), the first-order 3D
+// form.  The winners of hydro_sim_autotune are kept per grid shape for the
+// process and, with HYDRO_TUNE_FILE set, in that file (one line per shape:
+// key, then rows wpb form zwaves ms), which later processes read at create.
+namespace {
+
+struct TuneParams {
+  int rows, wpb, form, zwaves;
+  double ms;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneParams> g_tune;
+bool g_tune_loaded = false;
+
+std::string tune_key(const hydro_sim* s) {
+  std::ostringstream k;
+  k << "d" << s->dim << "_" << s->sz[0] << "x" << s->sz[1] << "x" << s->sz[2] << "_e" << s->d.equation << "_s"
+    << s->d.solver << "_l" << s->d.limiter << "_i" << s->d.integrator << "_p" << s->d.precision << "_k"
+    << (s->kahan ? 1 : 0);
+  return k.str();
+}
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* path = getenv("HYDRO_TUNE_FILE");
+  if (!path) return;
+  std::ifstream f(path);
+  std::string key;
+  TuneParams p{};
+  while (f >> key >> p.rows >> p.wpb >> p.form >> p.zwaves >> p.ms) g_tune[key] = p;
+}
+
+void tune_store(const std::string& key, const TuneParams& p) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_load_locked();
+  g_tune[key] = p;
+  if (const char* path = getenv("HYDRO_TUNE_FILE")) {
+    std::ofstream f(path, std::ios::app);
+    f << key << " " << p.rows << " " << p.wpb << " " << p.form << " " << p.zwaves << " " << p.ms << "\n";
+  }
+}
+
+bool tune_lookup(const std::string& key, TuneParams& p) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_load_locked();
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) return false;
+  p = it->second;
+  return true;
+}
+
+bool first_order_3d(const hydro_sim* s) {
+  return s->dim == 3 && (kernel_kind(s->d) == 1 || kernel_kind(s->d) == 2);
+}
+
+TuneParams current_params(const hydro_sim* s) {
+  return TuneParams{s->rows, s->wpb, s->form3d, s->zwaves, 0.0};
+}
+
+void drop_graphs(hydro_sim* s) {
+  for (auto& g : s->graph2)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  for (auto& g : s->graphN)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+void apply_params(hydro_sim* s, const TuneParams& p) {
+  if (first_order_3d(s)) {
+    const int form = (p.form == kForm3dZm2 && !s->mixed) ? kForm3dZm1w8 : p.form;
+    set_form3d(s, form, form == kForm3dYMarch ? p.zwaves : 0);
+  }
+  s->rows = std::max(1, p.rows);
+  s->wpb = std::min(4, std::max(1, p.wpb));
+  drop_graphs(s);
+}
+
+}  // namespace
+
 int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   if (!ctx || !out) return fail(HYDRO_E_INVALID, "null argument");
   int rc = validate(d);
@@ -760,21 +880,17 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu <= 0)
     ncu = 256;
+  s->ncu = ncu;
   if (s->dim == 2) {
     int min_waves = 3;
     if (const char* e = getenv("HYDRO_MIN_WAVES_PER_SIMD")) min_waves = std::max(1, atoi(e));
     s->rows = pick_rows_2d((s->sz[0] - 4 + 59) / 60, s->sz[1] - 4, 4 * ncu, min_waves);
   } else if (s->dim == 3 && (kernel_kind(*d) == 1 || kernel_kind(*d) == 2)) {
-    const long long strips = (s->sz[0] - 4 + 59) / 60;
-    s->zwaves = stage3d_r1_waves(!s->mixed);
-    if (const char* e = getenv("HYDRO_3D_ZWAVES")) s->zwaves = atoi(e) == 4 ? 4 : 8;
     // rows shared through LDS: measured faster for the FP64 first stage only
     // (256^3 HLLC blast: 794 vs 891 us; FP32 and the later FP64 stages hold
     // more registers than they save loads)
     s->zshare = s->mixed ? 0 : (1 << CM_FIRST) | (1 << CM_SINGLE);
     if (const char* e = getenv("HYDRO_3D_ZSHARE")) s->zshare = (int)strtol(e, nullptr, 0);
-    const long long nzb = (s->sz[2] - 4 + s->zwaves - 2) / (s->zwaves - 1);
-    s->rows = pick_rows_3d_r1(strips * nzb, s->sz[1] - 4, ncu * (8 / s->zwaves));
     // the z-march form: FP32 two cells per lane in 8-wave workgroups (zm2)
     // or one cell per lane in 16-wave workgroups (zm1); FP64 one cell per
     // lane in 8-wave workgroups (LDS holds 94 KB per workgroup in each).
@@ -782,19 +898,19 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     // FP64): y-march 523 / 963 us, zm1 520 / 953, zm2 400 / -, zm1w8 540 /
     // 955.  FP32 takes zm2 (two strips of 126 columns cover the 252
     // interior columns, against five strips of 60); FP64 keeps the y-march.
-    std::string form = s->mixed ? "zm2" : "r1";
-    if (const char* e = getenv("HYDRO_3D_FORM")) form = e;
-    if (form == "zm1" || form == "zm2" || form == "zm1w8") {
-      s->zmarch = (form == "zm2" && s->mixed) ? 2 : 1;
-      s->zwaves = (form == "zm1" && s->mixed) ? 16 : 8;
-      const int cpl = s->zmarch;
-      const long long zstrips = (s->sz[0] - 4 + 64 * cpl - 3) / (64 * cpl - 2);
-      const long long yblocks = (s->sz[1] - 4 + s->zwaves - 3) / (s->zwaves - 2);
-      const int N = s->N;
-      const long long lds = (long long)s->zwaves * (2 * N + 4 + N) * 64 * cpl * (long long)esize(s);
-      const long long per_cu = std::max(1LL, std::min(160LL * 1024 / lds, 32LL / s->zwaves));
-      s->rows = pick_planes_zm(zstrips * yblocks, s->sz[2] - 4, ncu * per_cu);
-      if (const char* e = getenv("HYDRO_3D_PLANES")) s->rows = std::max(1, atoi(e));
+    int form = s->mixed ? kForm3dZm2 : kForm3dYMarch;
+    if (const char* e = getenv("HYDRO_3D_FORM")) {
+      const std::string f = e;
+      form = f == "zm1" ? kForm3dZm1 : f == "zm2" ? kForm3dZm2 : f == "zm1w8" ? kForm3dZm1w8 : kForm3dYMarch;
+    }
+    set_form3d(s, form);
+    if (form == kForm3dYMarch) {
+      if (const char* e = getenv("HYDRO_3D_ZWAVES")) {
+        s->zwaves = atoi(e) == 4 ? 4 : 8;
+        set_form3d(s, form, s->zwaves);
+      }
+    } else if (const char* e = getenv("HYDRO_3D_PLANES")) {
+      s->rows = std::max(1, atoi(e));
     }
   } else if (s->dim == 3) {
     const long long strips = (s->sz[0] - 4 + 59) / 60;
@@ -813,6 +929,17 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   s->cg = cfl_grid(s->d, s->plane);
   s->cg.dt_rows = s->dt_rows;
   s->cg.dt_blocks = s->dt_blocks;
+  {
+    // a decomposition tuned earlier for this grid shape (this process or
+    // HYDRO_TUNE_FILE)
+    // (decomposition switches in the environment take precedence)
+    bool env = false;
+    for (const char* k : {"HYDRO_ROWS_PER_WAVE", "HYDRO_WAVES_PER_BLOCK", "HYDRO_3D_FORM", "HYDRO_3D_PLANES",
+                          "HYDRO_3D_ZWAVES", "HYDRO_MIN_WAVES_PER_SIMD"})
+      env = env || getenv(k) != nullptr;
+    TuneParams p;
+    if (!env && s->dim >= 2 && tune_lookup(tune_key(s), p)) apply_params(s, p);
+  }
 
   auto cleanup = [&](int code, const std::string& m) {
     hydro_sim_destroy(s);
@@ -1211,6 +1338,129 @@ int hydro_sim_stage_timing(hydro_sim* s, double* avg_ms, long long* launches) {
   }
   if (launches) *launches = (long long)s->ev.size();
   if (avg_ms) *avg_ms = s->ev.empty() ? 0.0 : tot / (double)s->ev.size();
+  return HYDRO_OK;
+}
+
+int hydro_sim_decomposition(hydro_sim* s, int* params_out) {
+  if (!s || !params_out) return fail(HYDRO_E_INVALID, "null argument");
+  params_out[0] = s->rows;
+  params_out[1] = s->wpb;
+  params_out[2] = first_order_3d(s) ? s->form3d : 0;
+  params_out[3] = first_order_3d(s) ? s->zwaves : 0;
+  return HYDRO_OK;
+}
+
+int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_per_update_out) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  if (updates <= 0) updates = 4;
+  if (s->dim == 1) {  // one row per wave; small grids run whole updates in one workgroup
+    if (params_out) hydro_sim_decomposition(s, params_out);
+    if (ms_per_update_out) *ms_per_update_out = 0.0;
+    return HYDRO_OK;
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  int rc = materialize_ghosts(s);
+  if (rc) return rc;
+  // the trial updates run on a copy: state, low words, primitives, clocks
+  const size_t bytes = esize(s) * (size_t)s->plane * s->N;
+  void* saveU = nullptr;
+  float* saveLo = nullptr;
+  double* savePrim = nullptr;
+  HIP_TRY(hipMalloc(&saveU, bytes));
+  HIP_TRY(hipMemcpyAsync(saveU, s->U[s->cur], bytes, hipMemcpyDeviceToDevice, s->stream));
+  if (s->kahan) {
+    HIP_TRY(hipMalloc((void**)&saveLo, sizeof(float) * s->plane * s->N));
+    HIP_TRY(hipMemcpyAsync(saveLo, s->Ulo[s->cur], sizeof(float) * s->plane * s->N, hipMemcpyDeviceToDevice,
+                           s->stream));
+  }
+  if (s->prim) {
+    HIP_TRY(hipMalloc((void**)&savePrim, sizeof(double) * s->plane * s->N));
+    HIP_TRY(hipMemcpyAsync(savePrim, s->prim, sizeof(double) * s->plane * s->N, hipMemcpyDeviceToDevice, s->stream));
+  }
+  DevScalars sc0[2];
+  HIP_TRY(hipMemcpyAsync(sc0, s->sc, sizeof sc0, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int cur0 = s->cur;
+  auto restore = [&]() -> int {
+    s->cur = cur0;
+    s->ghosts_pending = false;
+    HIP_TRY(hipMemcpyAsync(s->U[s->cur], saveU, bytes, hipMemcpyDeviceToDevice, s->stream));
+    if (saveLo)
+      HIP_TRY(hipMemcpyAsync(s->Ulo[s->cur], saveLo, sizeof(float) * s->plane * s->N, hipMemcpyDeviceToDevice,
+                             s->stream));
+    if (savePrim)
+      HIP_TRY(hipMemcpyAsync(s->prim, savePrim, sizeof(double) * s->plane * s->N, hipMemcpyDeviceToDevice,
+                             s->stream));
+    HIP_TRY(hipMemcpyAsync(s->sc, sc0, sizeof sc0, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return HYDRO_OK;
+  };
+
+  // candidates: the current choice first, then the grid around it
+  const TuneParams base = current_params(s);
+  std::vector<TuneParams> cand{base};
+  auto add = [&](TuneParams p) {
+    for (const auto& c : cand)
+      if (c.rows == p.rows && c.wpb == p.wpb && c.form == p.form && c.zwaves == p.zwaves) return;
+    cand.push_back(p);
+  };
+  if (s->dim == 2) {
+    for (int r : {base.rows, 8, 12, 16, 20, 24, 32})
+      for (int w : {1, 2, 4}) add(TuneParams{r, w, 0, base.zwaves, 0.0});
+  } else if (first_order_3d(s)) {
+    std::vector<int> forms{kForm3dYMarch, kForm3dZm1, kForm3dZm1w8};
+    if (s->mixed) forms.push_back(kForm3dZm2);
+    for (int f : forms) {
+      for (int zw : (f == kForm3dYMarch ? std::vector<int>{4, 8} : std::vector<int>{0})) {
+        set_form3d(s, f, zw);
+        const int r0 = s->rows;
+        add(TuneParams{r0, base.wpb, f, s->zwaves, 0.0});
+        if (f != kForm3dYMarch) {
+          add(TuneParams{std::max(4, r0 / 2), base.wpb, f, s->zwaves, 0.0});
+          add(TuneParams{std::min(s->sz[2] - 4, 2 * r0), base.wpb, f, s->zwaves, 0.0});
+        }
+      }
+    }
+  } else {
+    for (int r : {base.rows, 8, 16, 32, 64}) add(TuneParams{r, base.wpb, 0, base.zwaves, 0.0});
+  }
+
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  TuneParams best = base;
+  best.ms = 1e30;
+  for (const TuneParams& c : cand) {
+    apply_params(s, c);
+    if ((rc = restore())) break;
+    if ((rc = enqueue_update(s))) break;  // warm-up
+    if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = fail(HYDRO_E_HIP, "hipEventRecord"); break; }
+    for (int k = 0; k < updates && !rc; ++k) rc = enqueue_update(s);
+    if (rc) break;
+    if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(HYDRO_E_HIP, "hipEventRecord");
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const double per = ms / updates;
+    if (per < best.ms) {
+      best = c;
+      best.ms = per;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  apply_params(s, rc ? base : best);
+  const int rc2 = restore();
+  hipFree(saveU);
+  if (saveLo) hipFree(saveLo);
+  if (savePrim) hipFree(savePrim);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  tune_store(tune_key(s), best);
+  if (params_out) hydro_sim_decomposition(s, params_out);
+  if (ms_per_update_out) *ms_per_update_out = best.ms;
   return HYDRO_OK;
 }
 

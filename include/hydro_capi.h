@@ -280,6 +280,27 @@ int hydro_sim_totals(hydro_sim* sim, double* totals_out);
  * bit2 con2prim not converged. */
 int hydro_sim_health(hydro_sim* sim, unsigned* flags_out);
 
+/* No counterpart in the reference (its kernels' work sizes are fixed by the
+ * OpenCL NDRange).  The work decomposition of the fused stage kernels:
+ * params_out[4] = rows per wave (2D) or rows / planes per chunk (3D), waves
+ * per workgroup (2D), the first-order 3D form (0 y-march, 1 z-march one cell
+ * per lane, 2 z-march two cells per lane, 3 z-march one cell per lane in
+ * 8-wave workgroups) and that form's waves per workgroup. */
+int hydro_sim_decomposition(hydro_sim* sim, int* params_out);
+
+/* No counterpart in the reference.  Times `updates` updates (after one
+ * warm-up) of each candidate decomposition of this simulation's stage
+ * kernels -- 2D: rows per wave x waves per workgroup; 3D first order: form,
+ * cells per lane and planes per chunk; 3D Roe family: planes per chunk --
+ * restores the state, clock and primitives after every trial, and keeps the
+ * fastest.  Every decomposition gives the same bits.  The winner is kept per
+ * grid shape for the process and, when HYDRO_TUNE_FILE names a file,
+ * appended there; simulations created later with the same shape start from
+ * it (unless a decomposition switch is set in the environment).  1D grids
+ * have nothing to tune (returns the current choice).  params_out as
+ * hydro_sim_decomposition (may be NULL). */
+int hydro_sim_autotune(hydro_sim* sim, int updates, int* params_out, double* ms_per_update_out);
+
 /* Device HIP stream of the sim (hipStream_t as void*), for callers that want
  * to order their own work against it. */
 void* hydro_sim_stream(hydro_sim* sim);
