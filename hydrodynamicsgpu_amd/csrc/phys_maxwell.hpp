@@ -25,7 +25,7 @@ struct MaxwellRoePolicy : StageIsState<6, T> {
   static constexpr int NF = 1;
   struct FaceA {
     T dq[N];
-    T lq[N];
+    T lq2[N];
     T se, su, lam;
   };
 
@@ -82,7 +82,7 @@ struct MaxwellRoePolicy : StageIsState<6, T> {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       a.dq[k] = valid ? tr[k] - tl[k] : T(0);
-      a.lq[k] = T(.5) * (tr[k] + tl[k]);
+      a.lq2[k] = tr[k] + tl[k];
     }
     a.se = pp.mx[0];
     a.su = pp.mx[1];
@@ -95,7 +95,7 @@ struct MaxwellRoePolicy : StageIsState<6, T> {
     const T lam[N] = {-a.lam, -a.lam, T(0), T(0), a.lam, a.lam};
     T ft[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, lam[k], a.lq[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
+    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, lam[k], a.lq2[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
     right(ft, side, a.se, a.su, F);
   }
 };

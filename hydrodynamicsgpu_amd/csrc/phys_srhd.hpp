@@ -32,7 +32,7 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
     T lam[N];
     T R[N][N];   // right eigenvectors, rotated frame: R[row][col]
     T dq[N];
-    T lq[N];
+    T lq2[N];
   };
 
   template <typename P>
@@ -207,24 +207,24 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
     const Mean m = mean(cL, cR, side, gamma);
     T Lm[N][N];
     basis(m, a.lam, Lm, a.R);
-    T dlt[N], avg[N];
+    T dlt[N], sum[N];
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       dlt[s] = uR[s] - uL[s];
-      avg[s] = T(.5) * (uR[s] + uL[s]);
+      sum[s] = uR[s] + uL[s];
     }
     rot_in(dlt, side);
-    rot_in(avg, side);
+    rot_in(sum, side);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       T sd = T(0), sa = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
         sd += Lm[i][j] * dlt[j];
-        sa += Lm[i][j] * avg[j];
+        sa += Lm[i][j] * sum[j];
       }
       a.dq[i] = valid ? sd : T(0);
-      a.lq[i] = sa;
+      a.lq2[i] = sa;
     }
     return a;
   }
@@ -233,7 +233,7 @@ struct SRHDRoePolicy : StageIsState<DIM + 2, T> {
                                T F[N]) {
     T ft[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, a.lam[k], a.lq[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
+    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, a.lam[k], a.lq2[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       T s = T(0);

@@ -4,7 +4,7 @@
 // an interface into the interface flux in grid order.  It runs in two passes
 // so the Roe limiter can see its neighbouring interfaces:
 //   face_a: per-interface work that needs only the interface's two cells
-//           (Roe: frozen basis, deltaQTilde = L (qR - qL), L qavg);
+//           (Roe: frozen basis, deltaQTilde = L (qR - qL), L (qL + qR));
 //   face_b: the flux, given the neighbours' deltaQTilde (Roe) or nothing
 //           (HLL/HLLC, first order: res/EulerHLL.cl:317-335 keeps the
 //           limiter disabled).
@@ -36,45 +36,55 @@ struct StageIsState {
   static HYDRO_DEV Stage unpack_stage(const T o[N]) { return stage(o, T(0)); }
 };
 
-// The limited slope dq phi(r), r = num / dq (res/Roe.cl:272-298).  The
-// limiters with compile-time piecewise-linear forms skip the division: with
-// s = sign(dq), b = |dq|, c = s num, dq phi(num / dq) = s phi~(c, b), phi
-// written on the pair (MC: max(0, min(2b, (b+c)/2, 2c)); Superbee:
-// max(0, min(b, 2c), min(2b, c)); MinMod: max(0, min(c, b))).  dq = 0 gives
-// 0, which is what the product gives for the reference's r = +-inf or 0/0
-// (phi stays finite under fmin/fmax).
+// minmod(a, b): the argument of smaller magnitude when a and b have the same
+// sign, else 0 -- the median of (a, b, 0), one v_med3_f32 in FP32.
+template <typename T>
+HYDRO_DEV T minmod(T a, T b) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_fmed3f(a, b, T(0));
+  else return tmax(tmin(a, b), tmin(T(0), tmax(a, b)));
+}
+
+// Half the limited slope, dq phi(r) / 2 with r = num / dq (res/Roe.cl:272-298).
+// The limiters with compile-time piecewise-linear forms skip the division:
+// dq phi(num / dq) is homogeneous in (dq, num), so
+//   MC       (phi = max(0, min(2r, (1+r)/2, 2))):  minmod(dq, minmod((dq+num)/4, num))
+//   Superbee (phi = max(0, min(2r, 1), min(r, 2))): the larger in magnitude of
+//            minmod(num, dq/2) and minmod(num/2, dq), which carry dq's sign or are 0
+//   MinMod   (phi = max(0, min(1, r))):             minmod(dq, num) / 2
+// dq = 0 gives 0, which is what the product gives for the reference's r =
+// +-inf or 0/0 (phi stays finite under fmin/fmax).  The halving is exact.
 template <int LIM, typename T>
-HYDRO_DEV T limited_slope(int lim, T num, T d) {
-  if constexpr (LIM == LIM_MONOTIZEDCENTRAL || LIM == LIM_SUPERBEE || LIM == LIM_MINMOD) {
+HYDRO_DEV T half_limited_slope(int lim, T num, T d) {
+  if constexpr (LIM == LIM_MONOTIZEDCENTRAL) {
     (void)lim;
-    const T b = tabs(d);
-    const T c = d < T(0) ? -num : num;
-    T m;
-    if constexpr (LIM == LIM_MONOTIZEDCENTRAL)
-      m = tmax(T(0), tmin(T(2) * b, tmin(T(.5) * (b + c), T(2) * c)));
-    else if constexpr (LIM == LIM_SUPERBEE)
-      m = tmax(T(0), tmax(tmin(b, T(2) * c), tmin(T(2) * b, c)));
-    else
-      m = tmax(T(0), tmin(c, b));
-    return d < T(0) ? -m : m;
+    return minmod(d, minmod((d + num) * T(.25), num));
+  } else if constexpr (LIM == LIM_SUPERBEE) {
+    (void)lim;
+    const T a = minmod(num, T(.5) * d), b = minmod(T(.5) * num, d);
+    return copysign(tmax(tabs(a), tabs(b)), d);
+  } else if constexpr (LIM == LIM_MINMOD) {
+    (void)lim;
+    return T(.5) * minmod(d, num);
   } else {
-    return d * limiter<LIM, T>(lim, tdiv(num, d));
+    return (T(.5) * d) * limiter<LIM, T>(lim, tdiv(num, d));
   }
 }
 
 // The characteristic flux of one wave k of an interface (res/Roe.cl:272-298):
 //   l qavg - 1/2 l dq (theta + phi(r) (eps - theta)),
 // theta = +1 for l >= 0 else -1, eps = l dt/dx, r from the upwind
-// neighbour's dq (dqlo for l >= 0, dqhi otherwise).  With l theta = |l| it
-// is evaluated as l qavg - 1/2 (|l| dq + dq phi(r) (l eps - |l|)).
+// neighbour's dq (dqlo for l >= 0, dqhi otherwise).  With l theta = |l| and
+// lq2 = L (qL + qR) = 2 L qavg it is evaluated as
+//   (l/2) lq2 - (|l/2| dq + (dq phi(r) / 2) (l eps - |l|)),
+// every halving exact, so the product is the same expression term by term.
 template <int LIM, typename T>
-HYDRO_DEV T wave_flux(int lim, T l, T lq, T d, T dqlo, T dqhi, T dt_dx) {
+HYDRO_DEV T wave_flux(int lim, T l, T lq2, T d, T dqlo, T dqhi, T dt_dx) {
   const bool up = l >= T(0);
   const T num = up ? dqlo : dqhi;
-  const T al = tabs(l);
-  const T dphi = limited_slope<LIM, T>(lim, num, d);
-  const T w = l * (l * dt_dx) - al;
-  return l * lq - T(.5) * (al * d + dphi * w);
+  const T hl = T(.5) * l;
+  const T hd = half_limited_slope<LIM, T>(lim, num, d);
+  const T w = l * (l * dt_dx) - tabs(l);
+  return hl * lq2 - (tabs(hl) * d + hd * w);
 }
 
 // ------------------------------------------------------------------ Roe
@@ -91,7 +101,7 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
   struct FaceA {
     typename P::Face f;
     T dq[N];  // deltaQTilde of this interface (0 where the reference never writes it)
-    T lq[N];  // L * qavg
+    T lq2[N]; // L * (qL + qR), twice L * qavg
   };
 
   template <typename S>
@@ -113,22 +123,22 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
     return c;
   }
 
-  // res/Roe.cl:77-150 (deltaQTilde) and the L*qavg half of res/Roe.cl:264-270
+  // res/Roe.cl:77-150 (deltaQTilde) and the L qavg half of res/Roe.cl:264-270 (as L (qL + qR))
   static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const Stage& sL, const Stage& sR, int side,
                                 bool valid, const PhysParams<T>& pp) {
     const T gamma = pp.gamma;
     FaceA a;
     a.f = P::face(cL, cR, side, gamma);
-    T dlt[N], avg[N];
+    T dlt[N], sum[N];
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       dlt[s] = sR.u[s] - sL.u[s];
-      avg[s] = T(.5) * (sR.u[s] + sL.u[s]);
+      sum[s] = sR.u[s] + sL.u[s];
     }
     P::rotate(dlt, side);
-    P::rotate(avg, side);
+    P::rotate(sum, side);
     P::left(a.f, dlt, a.dq, gamma);
-    P::left(a.f, avg, a.lq, gamma);
+    P::left(a.f, sum, a.lq2, gamma);
     if (!valid) {
 #pragma unroll
       for (int s = 0; s < N; ++s) a.dq[s] = T(0);
@@ -142,7 +152,7 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
     T lam[N], ft[N];
     P::eig(a.f, lam);
 #pragma unroll
-    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, lam[k], a.lq[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
+    for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, lam[k], a.lq2[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
     P::right(a.f, ft, F);
     P::rotate(F, side);
   }
