@@ -37,6 +37,22 @@ HYDRO_DEV int ghost_source(int method, int k, int n) {
   return 2 * n - 5 - k;                  // n-1 <- n-4, n-2 <- n-3
 }
 
+// The same composition seen from a reader: coordinate k along `axis` of a
+// cell whose post-boundary value is wanted becomes the coordinate that holds
+// that value now, and the axis' reflect mask is folded into `neg`.  Interior
+// coordinates and sides without a method (NONE: the ghost keeps its value)
+// map to themselves.  A kernel that reads ghosts this way does not need the
+// boundary to have run, so both can share one launch.
+HYDRO_DEV int mapped_coord(const BoundaryArgs& b, int axis, int k, unsigned& neg) {
+  const int n = axis == 0 ? b.sx : (axis == 1 ? b.sy : b.sz);
+  if (k >= 2 && k < n - 2) return k;
+  const int side = k < 2 ? 0 : 1;
+  const int m = b.method[axis][side];
+  if (m < 0) return k;
+  neg ^= b.negmask[axis][side];
+  return ghost_source(m, k, n);
+}
+
 // ---------------------------------------------------------------- ghosts
 // Enumerates exactly the ghost cells (some coordinate in {0,1,S-2,S-1}) as
 // coordinates; 32-bit index arithmetic (a ghost count fits in 31 bits for
@@ -293,9 +309,12 @@ HYDRO_DEV double wave_dt(double rate, bool bad) {
 // speeds are the Marti-Mueller eigenvalues of the mean primitives
 // (res/SRHDRoe.cl:76-118), with the Roe form of the cell dt.
 // L: the element type in memory (the FP32-mixed SRHD march reads the FP64
-// primitives and computes its speeds in FP32).
-template <typename S, int DIM, int SOLVER, typename L = S>
-HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const L* u0, int strip, int chunk) {
+// primitives and computes its speeds in FP32).  MAP: ghost cells are read
+// through the boundary's composed map `gm` (mapped_coord), so the values are
+// the post-boundary ones while the boundary itself runs in the same launch.
+template <typename S, int DIM, int SOLVER, typename L = S, bool MAP = false>
+HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const L* u0, int strip, int chunk,
+                               const BoundaryArgs* gm = nullptr) {
   using D = DtSolver<S, DIM, SOLVER>;
   using Frozen = typename D::Frozen;
   constexpr int N = DIM + 2, NF = D::NF;
@@ -308,11 +327,20 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const L* u0, int strip, int
   const S gamma = S(a.gamma);
   const double idx0 = 1.0 / a.kdx[0];
   bool bad = false;
+  unsigned negc = 0;
+  uint32_t cs = cl;
+  if constexpr (MAP) cs = (uint32_t)mapped_coord(*gm, 0, (int)cl, negc);
 
   auto fetch = [&](int row, S q[N]) {
+    unsigned neg = negc;
+    if constexpr (MAP && DIM > 1) row = mapped_coord(*gm, 1, row, neg);
     const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;
 #pragma unroll
-    for (int s = 0; s < N; ++s) q[s] = S(ldp(u0, a.plane, s, e, cl));
+    for (int s = 0; s < N; ++s) q[s] = S(ldp(u0, a.plane, s, e, cs));
+    if constexpr (MAP) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) q[s] = ((neg >> s) & 1u) ? -q[s] : q[s];
+    }
   };
   auto x_rate = [&](const Frozen& f) {
     S pk[NF], pkL[NF];
@@ -369,8 +397,10 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const L* u0, int strip, int
 // planes read them too: L2 hits), takes the x-faces through DPP shifts, the
 // y-face against the previous row held in registers and both z-faces, and
 // keeps its minimum.  Cells 2 .. last_x / last_y / last_z of every axis.
-template <typename S, int SOLVER>
-HYDRO_DEV double dt_march3_wave(const DtMarchArgs& a, const S* u0, int strip, int chunk, int z) {
+// MAP: ghost reads through the boundary's composed map, as in dt_march_wave.
+template <typename S, int SOLVER, bool MAP = false>
+HYDRO_DEV double dt_march3_wave(const DtMarchArgs& a, const S* u0, int strip, int chunk, int z,
+                                const BoundaryArgs* gm = nullptr) {
   using D = DtSolver<S, 3, SOLVER>;
   using Frozen = typename D::Frozen;
   constexpr int N = 5, NF = D::NF;
@@ -384,12 +414,24 @@ HYDRO_DEV double dt_march3_wave(const DtMarchArgs& a, const S* u0, int strip, in
   const S gamma = S(a.gamma);
   const long long psz = (long long)sx * sy;
   bool bad = false;
+  unsigned negc = 0;
+  uint32_t cs = cl;
+  if constexpr (MAP) cs = (uint32_t)mapped_coord(*gm, 0, (int)cl, negc);
 
   auto fetch = [&](int plane, int row, S q[N]) {
+    unsigned neg = negc;
+    if constexpr (MAP) {
+      plane = mapped_coord(*gm, 2, plane, neg);
+      row = mapped_coord(*gm, 1, row, neg);
+    }
     const long long e = (long long)__builtin_amdgcn_readfirstlane(plane) * psz +
                         (long long)__builtin_amdgcn_readfirstlane(row) * sx;
 #pragma unroll
-    for (int s = 0; s < N; ++s) q[s] = ldp(u0, a.plane, s, e, cl);
+    for (int s = 0; s < N; ++s) q[s] = ldp(u0, a.plane, s, e, cs);
+    if constexpr (MAP) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) q[s] = ((neg >> s) & 1u) ? -q[s] : q[s];
+    }
   };
   struct Rows {
     S m[N], c[N], p[N];  // planes z-1, z, z+1

@@ -266,6 +266,66 @@ static void dt_march3(const DtMarchArgs& a, const S* u0, hipStream_t st) {
 void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st) { dt_march3<double>(a, u0, st); }
 void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st) { dt_march3<float>(a, u0, st); }
 
+// ---------------------------------------------------------------- step head
+// boundary + CFL march in one launch (HeadArgs).  The marching workgroups
+// read every ghost cell through the boundary's map, so the ghost-writing
+// workgroups (dispatched first) run beside them; each marching workgroup
+// leaves its minimum for the dt tail.
+template <typename S, int DIM, int SOLVER>
+__global__ void __launch_bounds__(256) step_head_kernel(HeadArgs h, S* planes, float* lo, double* prim) {
+  if ((int)blockIdx.x < h.bblocks) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ghost_count(h.b.dim, h.b.sx, h.b.sy, h.b.sz)) return;
+    boundary_cell<S>(h.b, planes, lo, g);
+    if (prim) boundary_cell<double>(h.b, prim, nullptr, g);
+    return;
+  }
+  const int mb = (int)blockIdx.x - h.bblocks;
+  const int wave = __builtin_amdgcn_readfirstlane(mb * 4 + (int)(threadIdx.x >> 6));
+  double m = INFINITY;
+  if (wave < h.a.nwaves) {
+    if constexpr (DIM == 3) {
+      const int per_plane = h.a.nstrips * h.a.nchunks;
+      const int z = 2 + wave / per_plane;
+      const int rest = wave % per_plane;
+      m = dt_march3_wave<S, SOLVER, true>(h.a, planes, rest % h.a.nstrips, rest / h.a.nstrips, z, &h.b);
+    } else if constexpr (SOLVER == 3) {
+      m = dt_march_wave<S, DIM, 3, double, true>(h.a, prim, wave % h.a.nstrips, wave / h.a.nstrips, &h.b);
+    } else {
+      m = dt_march_wave<S, DIM, SOLVER, S, true>(h.a, planes, wave % h.a.nstrips, wave / h.a.nstrips, &h.b);
+    }
+  }
+  __shared__ double wmin[4];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = INFINITY;
+    bool nan = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dt_merge(wmin[i], bm, nan);
+    h.a.wavemin[mb] = dt_flag(bm, nan);
+  }
+}
+
+template <typename S>
+static void step_head(const HeadArgs& h, S* planes, float* lo, double* prim, hipStream_t st) {
+  dim3 g(h.bblocks + h.mblocks), b(256);
+  const int dim = h.a.dim, sv = h.a.solver;
+#define LAUNCH(D, SV) hipLaunchKernelGGL((step_head_kernel<S, D, SV>), g, b, 0, st, h, planes, lo, prim)
+  if (dim == 3) {
+    if (sv == 0) LAUNCH(3, 0); else if (sv == 1) LAUNCH(3, 1); else LAUNCH(3, 2);
+  } else if (dim == 2) {
+    if (sv == 0) LAUNCH(2, 0); else if (sv == 1) LAUNCH(2, 1); else if (sv == 2) LAUNCH(2, 2); else LAUNCH(2, 3);
+  } else {
+    if (sv == 0) LAUNCH(1, 0); else if (sv == 1) LAUNCH(1, 1); else if (sv == 2) LAUNCH(1, 2); else LAUNCH(1, 3);
+  }
+#undef LAUNCH
+}
+void launch_step_head(const HeadArgs& h, void* planes, float* lo, double* prim, bool fp32, hipStream_t st) {
+  if (fp32) step_head<float>(h, (float*)planes, lo, prim, st);
+  else step_head<double>(h, (double*)planes, nullptr, prim, st);
+}
+
 // ---------------------------------------------------------------- totals
 template <typename S>
 __global__ void __launch_bounds__(256) totals_kernel(int dim, int sx, int sy, int sz, long long plane, int n,

@@ -252,6 +252,7 @@ struct hydro_sim {
   bool graph_pending[2] = {false, false};  // ghosts_pending after the captured pair
   bool use_graphs = true;
   bool use_run1d = true;  // small 1D Euler grids: whole runs in one workgroup
+  bool use_head = true;   // boundary + CFL march + dt tail of an update in one launch
   // generic stage evaluation: buffers handed out to a host-side integrator,
   // and whether U[cur ^ 1] holds the frozen start-of-step state
   std::vector<void*> user_bufs;
@@ -662,10 +663,45 @@ int enqueue_deriv(hydro_sim* s, double dt, void* deriv) {
   return HYDRO_OK;
 }
 
+// The head of an update: boundary and CFL march as one launch, then the dt
+// tail; for the strip-march CFL paths (1D/2D Euler and SRHD, 3D Euler)
+bool head_eligible(const hydro_sim* s) {
+  const int path = cfl_path(s->cg);
+  return s->use_head && (path == 1 || path == 2);
+}
+
+int enqueue_head(hydro_sim* s) {
+  if (boundary_covers_ghosts(s)) {
+    s->ghosts_pending = false;
+  } else {
+    const int rc = materialize_ghosts(s);
+    if (rc) return rc;
+  }
+  HeadArgs h{};
+  h.b = boundary_args(s);
+  h.a = cfl_path(s->cg) == 1 ? cfl_march_args(s->cg, s->blockmin) : march3_args(s->cg, s->blockmin);
+  h.mblocks = (h.a.nwaves + 3) / 4;
+  long long ghosts = 4;
+  if (s->dim == 2) ghosts = 4LL * s->sz[0] + 4LL * (s->sz[1] - 4);
+  if (s->dim == 3)
+    ghosts = 4LL * s->sz[0] * s->sz[1] + 4LL * s->sz[0] * (s->sz[2] - 4) + 4LL * (s->sz[1] - 4) * (s->sz[2] - 4);
+  h.bblocks = (int)((ghosts + 255) / 256);
+  launch_step_head(h, s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->mixed, s->stream);
+  FinalizeArgs f = finalize_args(s, true);
+  f.nblocks = h.mblocks;
+  launch_finalize_dt(f, s->stream);
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
 int enqueue_update(hydro_sim* s) {
-  int rc = enqueue_boundary(s);
-  if (rc) return rc;
-  rc = enqueue_dt(s, true);
+  int rc;
+  if (head_eligible(s)) {
+    rc = enqueue_head(s);
+  } else {
+    rc = enqueue_boundary(s);
+    if (!rc) rc = enqueue_dt(s, true);
+  }
   if (rc) return rc;
   return enqueue_stages(s);
 }
@@ -923,6 +959,7 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
   if (const char* e = getenv("HYDRO_ROWS_PER_WAVE")) s->rows = std::max(1, atoi(e));
   if (const char* e = getenv("HYDRO_GRAPHS")) s->use_graphs = atoi(e) != 0;
   if (const char* e = getenv("HYDRO_RUN1D")) s->use_run1d = atoi(e) != 0;
+  if (const char* e = getenv("HYDRO_STEP_HEAD")) s->use_head = atoi(e) != 0;
   if (const char* e = getenv("HYDRO_WAVES_PER_BLOCK")) s->wpb = std::min(4, std::max(1, atoi(e)));
   if (const char* e = getenv("HYDRO_DT_ROWS")) s->dt_rows = atoi(e);
   if (const char* e = getenv("HYDRO_GRAPH_UPDATES")) s->graph_updates = std::max(2, atoi(e) & ~1);

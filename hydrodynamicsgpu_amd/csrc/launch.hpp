@@ -121,6 +121,19 @@ void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, h
 void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
 
+// The head of an update in one launch: boundary and CFL strip march (1D/2D,
+// or 3D Euler).  Workgroups 0 .. bblocks-1 write the ghost cells (and the
+// SRHD primitives' ghosts when prim is set); the mblocks others march,
+// reading ghost cells through the boundary's composed map so that they need
+// not wait for it, and leave one minimum per workgroup in a.wavemin for the
+// dt tail (launch_finalize_dt over mblocks minima).
+struct HeadArgs {
+  BoundaryArgs b;
+  DtMarchArgs a;
+  int bblocks, mblocks;
+};
+void launch_step_head(const HeadArgs& h, void* planes, float* lo, double* prim, bool fp32, hipStream_t st);
+
 // Whole updates of a small 1D Euler grid in one workgroup (run1d.hip): per
 // step the boundary, the CFL march, the dt tail and every RK stage, with
 // workgroup barriers between them.  st[k] are the stage arguments with
