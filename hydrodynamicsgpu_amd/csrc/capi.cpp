@@ -252,7 +252,7 @@ struct hydro_sim {
   bool graph_pending[2] = {false, false};  // ghosts_pending after the captured pair
   bool use_graphs = true;
   bool use_run1d = true;  // small 1D Euler grids: whole runs in one workgroup
-  bool use_head = true;   // boundary + CFL march + dt tail of an update in one launch
+  bool use_head = true;   // boundary + CFL march of an update in one launch
   // generic stage evaluation: buffers handed out to a host-side integrator,
   // and whether U[cur ^ 1] holds the frozen start-of-step state
   std::vector<void*> user_bufs;
