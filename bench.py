@@ -331,7 +331,8 @@ def config_legs(steps, warmup, with_cpu, cpu_seconds, only=None):
     """The other configurations: Sod 1D (FP64, to t = 0.2, L1 vs the exact
     solution), the 3D HLLC blast (256^3, TVD-RK3, mixed with Kahan vs FP64)
     and the SRHD quadrants (1024^2, RK2, FP32 fluxes + FP64 con2prim vs FP64).
-    B per cell-update: N b (3S - 1), SRHD + 4 N 8 for the FP64 primitives."""
+    B per cell-update: N b (3S - 1), SRHD + 4 N b for the primitives (stored
+    in the state's precision b)."""
     from hydrodynamicsgpu_amd import capi, config, ics
     from hydrodynamicsgpu_amd.riemann import sample
 
@@ -413,6 +414,8 @@ def _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only):
     from hydrodynamicsgpu_amd import config, ics
 
     def pair(name, workload, make, ic, N, S, b_extra=0, mixed_flags=0, stage_extra=0):
+        # b_extra / stage_extra: values (not bytes) per cell-update / per
+        # stage launch and cell beyond the state, at the state's width b
         if only not in (None, name):
             return
         d64 = make("fp64", 0)
@@ -422,14 +425,14 @@ def _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only):
         g32 = time_desc(make("fp32_mixed", mixed_flags), q0, steps, warmup)
         v64 = cells * steps / g64["elapsed"]
         v32 = cells * steps / g32["elapsed"]
-        B32 = N * 4 * (3 * S - 1) + b_extra
-        B64 = N * 8 * (3 * S - 1) + b_extra
+        B32 = (N * (3 * S - 1) + b_extra) * 4
+        B64 = (N * (3 * S - 1) + b_extra) * 8
         # a stage launch moves its own algorithmic bytes: N b (3S - 1) / S
-        # of state, plus what the stage itself reads besides (SRHD: the FP64
-        # primitives of the start-of-step state, N * 8); the rest of b_extra
+        # of state, plus what the stage itself reads besides (SRHD: the
+        # primitives of the start-of-step state, N b); the rest of b_extra
         # belongs to the kernels between the stages and counts at step level
-        a32 = cells * (N * 4 * (3 * S - 1) / S + stage_extra) / (g32["stage_ms"] * 1e-3) / 1e9
-        a64 = cells * (N * 8 * (3 * S - 1) / S + stage_extra) / (g64["stage_ms"] * 1e-3) / 1e9
+        a32 = cells * (N * (3 * S - 1) / S + stage_extra) * 4 / (g32["stage_ms"] * 1e-3) / 1e9
+        a64 = cells * (N * (3 * S - 1) / S + stage_extra) * 8 / (g64["stage_ms"] * 1e-3) / 1e9
         ms32 = g32["elapsed"] * 1e3 / steps
         ms64 = g64["elapsed"] * 1e3 / steps
         diff = np.abs(g32["state"] - g64["state"]).reshape(-1, N)[:, 0]
@@ -461,7 +464,7 @@ def _config_pairs(res, steps, warmup, with_cpu, cpu_seconds, only):
                                 integratorName="RungeKutta2", defs={"idealGas_heatCapacityRatio": 5.0 / 3.0},
                                 precision=prec, flags=config.FLAG_REFERENCE | flags)
 
-    pair("C4", "srhd2d_roe_quadrants_1024^2_rk2", c4, ics.srhd_quadrants, 4, 2, b_extra=4 * 8 * 4, stage_extra=4 * 8)
+    pair("C4", "srhd2d_roe_quadrants_1024^2_rk2", c4, ics.srhd_quadrants, 4, 2, b_extra=4 * 4, stage_extra=4)
     if "C4" in res:
         from hydrodynamicsgpu_amd import capi
 

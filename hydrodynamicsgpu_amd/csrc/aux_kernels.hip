@@ -12,13 +12,13 @@
 namespace hydro {
 
 template <typename S>
-__global__ void __launch_bounds__(256) boundary_kernel(BoundaryArgs b, S* planes, float* lo, double* prim) {
+__global__ void __launch_bounds__(256) boundary_kernel(BoundaryArgs b, S* planes, float* lo, S* prim) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ghost_count(b.dim, b.sx, b.sy, b.sz)) return;
   boundary_cell<S>(b, planes, lo, g);
   // SRHDRoe::boundary fills the primitives with the same maps
   // (src/Solver/SRHDRoe.cpp:79-96); same launch, independent planes
-  if (prim) boundary_cell<double>(b, prim, nullptr, g);
+  if (prim) boundary_cell<S>(b, prim, nullptr, g);
 }
 
 template <typename S>
@@ -43,7 +43,7 @@ void launch_boundary_f64(const BoundaryArgs& b, double* planes, double* prim, hi
   hipLaunchKernelGGL((boundary_kernel<double>), dim3((unsigned)((g + 255) / 256)), dim3(256), 0, st, b, planes,
                      (float*)nullptr, prim);
 }
-void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, double* prim, hipStream_t st) {
+void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, float* prim, hipStream_t st) {
   const long long g = host_ghost_count(b.dim, b.sx, b.sy, b.sz);
   hipLaunchKernelGGL((boundary_kernel<float>), dim3((unsigned)((g + 255) / 256)), dim3(256), 0, st, b, planes, lo,
                      prim);
@@ -74,6 +74,11 @@ HYDRO_DEV void load_q(const S* u, long long plane, long long cell, S q[DIM + 2])
 // HLL/HLLC: res/EulerHLL.cl:142-175 on the frozen wave speeds of
 // res/EulerHLL.cl:17-115 / res/EulerHLLC.cl:15-108, cells 2..S-2; the right
 // face of cell S-2 is never written by the reference and reads as 0.
+// an SRHD primitive of the block-minima CFL kernel, FP64 or FP32 planes
+HYDRO_DEV double prim_at(const DtArgs& a, long long i) {
+  return a.prim_f32 ? double(((const float*)a.prim)[i]) : ((const double*)a.prim)[i];
+}
+
 template <typename S, int DIM>
 __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
   __shared__ double sh[4];
@@ -110,9 +115,9 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
         double pc[DIM + 2], pm[DIM + 2], pp[DIM + 2];
 #pragma unroll
         for (int s = 0; s < DIM + 2; ++s) {
-          pc[s] = a.prim[s * a.plane + cell];
-          pm[s] = a.prim[s * a.plane + cell - step[side]];
-          pp[s] = a.prim[s * a.plane + cell + step[side]];
+          pc[s] = prim_at(a, s * a.plane + cell);
+          pm[s] = prim_at(a, s * a.plane + cell - step[side]);
+          pp[s] = prim_at(a, s * a.plane + cell + step[side]);
         }
         const auto fc = P::template frozen<double>(pc, a.gamma);
         const auto mL = P::mean(P::template frozen<double>(pm, a.gamma), fc, side, a.gamma);
@@ -232,7 +237,13 @@ static void dt_march(const DtMarchArgs& a, const S* u0, hipStream_t st) {
 void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st) { dt_march<double>(a, u0, st); }
 void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st) { dt_march<float>(a, u0, st); }
 
-// SRHD: the FP64 primitive planes, speeds in FP64 or (FP32-mixed) FP32
+// SRHD: the primitive planes (FP64, or FP32 with FP32 speeds), speeds in
+// FP64 or (FP32-mixed) FP32
+void launch_dt_march_srhd(const DtMarchArgs& a, const float* prim, hipStream_t st) {
+  dim3 g((a.nwaves + 3) / 4), b(256);
+  if (a.dim == 1) hipLaunchKernelGGL((dt_march_kernel<float, 1, 3, float>), g, b, 0, st, a, prim);
+  else hipLaunchKernelGGL((dt_march_kernel<float, 2, 3, float>), g, b, 0, st, a, prim);
+}
 void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, hipStream_t st) {
   dim3 g((a.nwaves + 3) / 4), b(256);
   if (fp32) {
@@ -272,12 +283,12 @@ void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st)
 // workgroups (dispatched first) run beside them; each marching workgroup
 // leaves its minimum for the dt tail.
 template <typename S, int DIM, int SOLVER>
-__global__ void __launch_bounds__(256) step_head_kernel(HeadArgs h, S* planes, float* lo, double* prim) {
+__global__ void __launch_bounds__(256) step_head_kernel(HeadArgs h, S* planes, float* lo, S* prim) {
   if ((int)blockIdx.x < h.bblocks) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ghost_count(h.b.dim, h.b.sx, h.b.sy, h.b.sz)) return;
     boundary_cell<S>(h.b, planes, lo, g);
-    if (prim) boundary_cell<double>(h.b, prim, nullptr, g);
+    if (prim) boundary_cell<S>(h.b, prim, nullptr, g);
     return;
   }
   const int mb = (int)blockIdx.x - h.bblocks;
@@ -290,7 +301,7 @@ __global__ void __launch_bounds__(256) step_head_kernel(HeadArgs h, S* planes, f
       const int rest = wave % per_plane;
       m = dt_march3_wave<S, SOLVER, true>(h.a, planes, rest % h.a.nstrips, rest / h.a.nstrips, z, &h.b);
     } else if constexpr (SOLVER == 3) {
-      m = dt_march_wave<S, DIM, 3, double, true>(h.a, prim, wave % h.a.nstrips, wave / h.a.nstrips, &h.b);
+      m = dt_march_wave<S, DIM, 3, S, true>(h.a, prim, wave % h.a.nstrips, wave / h.a.nstrips, &h.b);
     } else {
       m = dt_march_wave<S, DIM, SOLVER, S, true>(h.a, planes, wave % h.a.nstrips, wave / h.a.nstrips, &h.b);
     }
@@ -308,7 +319,7 @@ __global__ void __launch_bounds__(256) step_head_kernel(HeadArgs h, S* planes, f
 }
 
 template <typename S>
-static void step_head(const HeadArgs& h, S* planes, float* lo, double* prim, hipStream_t st) {
+static void step_head(const HeadArgs& h, S* planes, float* lo, S* prim, hipStream_t st) {
   dim3 g(h.bblocks + h.mblocks), b(256);
   const int dim = h.a.dim, sv = h.a.solver;
 #define LAUNCH(D, SV) hipLaunchKernelGGL((step_head_kernel<S, D, SV>), g, b, 0, st, h, planes, lo, prim)
@@ -321,9 +332,9 @@ static void step_head(const HeadArgs& h, S* planes, float* lo, double* prim, hip
   }
 #undef LAUNCH
 }
-void launch_step_head(const HeadArgs& h, void* planes, float* lo, double* prim, bool fp32, hipStream_t st) {
-  if (fp32) step_head<float>(h, (float*)planes, lo, prim, st);
-  else step_head<double>(h, (double*)planes, nullptr, prim, st);
+void launch_step_head(const HeadArgs& h, void* planes, float* lo, void* prim, bool fp32, hipStream_t st) {
+  if (fp32) step_head<float>(h, (float*)planes, lo, (float*)prim, st);
+  else step_head<double>(h, (double*)planes, nullptr, (double*)prim, st);
 }
 
 // ---------------------------------------------------------------- totals
@@ -378,7 +389,7 @@ namespace hydro {
 // One thread per cell: x from the block's 256-column slice, y and z from the
 // grid (no index division).
 template <typename S, int DIM>
-__global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, float* lo, double* prim) {
+__global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, float* lo, S* prim) {
   constexpr int N = DIM + 2, E = DIM + 1;
   const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (x >= a.sx) return;
@@ -427,11 +438,13 @@ __global__ void __launch_bounds__(256) con2prim_kernel(Con2PrimArgs a, S* u, flo
   if (!act) return;
   double p[N];
 #pragma unroll
-  for (int s = 0; s < N; ++s) p[s] = prim[s * a.plane + cell];
-  // FP32 storage: warm-started Newton (same root, see srhd_con2prim)
+  for (int s = 0; s < N; ++s) p[s] = double(prim[s * a.plane + cell]);
+  // FP32 storage: warm-started Newton (same root, see srhd_con2prim); the
+  // primitives are stored in the state's precision (FP32: what the FP32
+  // fluxes and speeds read)
   if (srhd_con2prim<DIM, sizeof(S) == 4>(defs, q, p)) {
 #pragma unroll
-    for (int s = 0; s < N; ++s) prim[s * a.plane + cell] = p[s];
+    for (int s = 0; s < N; ++s) prim[s * a.plane + cell] = S(p[s]);
   } else {
     atomicOr(a.health, 4u);
   }
@@ -511,7 +524,7 @@ void launch_planes_to_aos_f32(const float* hi, const float* lo, int n, long long
 }
 
 template <typename S>
-static void con2prim(const Con2PrimArgs& a, S* u, float* lo, double* prim, hipStream_t st) {
+static void con2prim(const Con2PrimArgs& a, S* u, float* lo, S* prim, hipStream_t st) {
   dim3 g((a.sx + 255) / 256, a.dim > 1 ? a.sy : 1, a.dim > 2 ? a.sz : 1), b(256);
   if (a.dim == 1) hipLaunchKernelGGL((con2prim_kernel<S, 1>), g, b, 0, st, a, u, lo, prim);
   else if (a.dim == 2) hipLaunchKernelGGL((con2prim_kernel<S, 2>), g, b, 0, st, a, u, lo, prim);
@@ -520,7 +533,7 @@ static void con2prim(const Con2PrimArgs& a, S* u, float* lo, double* prim, hipSt
 void launch_con2prim_f64(const Con2PrimArgs& a, double* u, double* prim, hipStream_t st) {
   con2prim<double>(a, u, nullptr, prim, st);
 }
-void launch_con2prim_f32(const Con2PrimArgs& a, float* u, float* lo, double* prim, hipStream_t st) {
+void launch_con2prim_f32(const Con2PrimArgs& a, float* u, float* lo, float* prim, hipStream_t st) {
   con2prim<float>(a, u, lo, prim, st);
 }
 

@@ -149,7 +149,7 @@ long long cfl_minima(const CflGrid& g) {
   }
 }
 
-int enqueue_cfl(const CflGrid& g, const void* U, const double* prim, double* minima, FinalizeArgs f,
+int enqueue_cfl(const CflGrid& g, const void* U, const void* prim, bool prim_f64, double* minima, FinalizeArgs f,
                 hipStream_t st) {
   const hydro_sim_desc& d = *g.d;
   f.blockmin = minima;
@@ -158,7 +158,8 @@ int enqueue_cfl(const CflGrid& g, const void* U, const double* prim, double* min
   case 1: {
     // strip march, one minimum per wave, then the one-block tail
     const DtMarchArgs a = cfl_march_args(g, minima);
-    if (d.equation == HYDRO_EQN_SRHD) launch_dt_march_srhd(a, prim, g.mixed, st);
+    if (d.equation == HYDRO_EQN_SRHD && prim_f64) launch_dt_march_srhd(a, (const double*)prim, g.mixed, st);
+    else if (d.equation == HYDRO_EQN_SRHD) launch_dt_march_srhd(a, (const float*)prim, st);
     else if (g.mixed) launch_dt_march_f32(a, (const float*)U, st);
     else launch_dt_march_f64(a, (const double*)U, st);
     f.nblocks = a.nwaves;
@@ -184,6 +185,7 @@ int enqueue_cfl(const CflGrid& g, const void* U, const double* prim, double* min
     a.solver = d.solver;
     a.eqn = d.equation;
     a.prim = prim;
+    a.prim_f32 = prim && !prim_f64;
     double mx[5];
     maxwell_constants(d, mx);
     a.maxwell_lambda = mx[4];
@@ -224,7 +226,7 @@ struct hydro_sim {
   int ndbuf = 0;
   bool replay = false;
   void* P[kMaxPartials] = {nullptr, nullptr, nullptr};
-  double* prim = nullptr;  // SRHD primitive planes (FP64 in both precisions)
+  void* prim = nullptr;  // SRHD primitive planes (the state's precision: double, or float when mixed)
   double* blockmin = nullptr;
   double* partial = nullptr;
   DevScalars* sc = nullptr;
@@ -369,9 +371,9 @@ int enqueue_boundary(hydro_sim* s) {
   // SRHDRoe::boundary fills the primitives with the same kernels afterwards
   // (src/Solver/SRHDRoe.cpp:79-96): the same launch here
   if (s->mixed)
-    launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->stream);
+    launch_boundary_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, (float*)s->prim, s->stream);
   else
-    launch_boundary_f64(b, (double*)s->U[s->cur], s->prim, s->stream);
+    launch_boundary_f64(b, (double*)s->U[s->cur], (double*)s->prim, s->stream);
   HIP_TRY(hipGetLastError());
   return HYDRO_OK;
 }
@@ -397,8 +399,38 @@ FinalizeArgs finalize_args(hydro_sim* s, bool advance_time) {
   return f;
 }
 
+// SRHD primitive planes: stored in the state's precision (the FP32-mixed
+// form's fluxes and speeds read them as FP32; con2prim computes in FP64)
+size_t prim_bytes(const hydro_sim* s) {
+  return (s->mixed ? sizeof(float) : sizeof(double)) * (size_t)s->plane * (size_t)s->N;
+}
+
+int upload_prims(hydro_sim* s, const std::vector<double>& planes) {
+  if (!s->mixed)
+    return copy_sync(s->prim, planes.data(), prim_bytes(s), hipMemcpyHostToDevice, s->stream) == hipSuccess
+               ? HYDRO_OK
+               : fail(HYDRO_E_HIP, "copy (primitives)");
+  std::vector<float> f(planes.begin(), planes.end());
+  return copy_sync(s->prim, f.data(), prim_bytes(s), hipMemcpyHostToDevice, s->stream) == hipSuccess
+             ? HYDRO_OK
+             : fail(HYDRO_E_HIP, "copy (primitives)");
+}
+
+int download_prims(hydro_sim* s, std::vector<double>& planes) {
+  planes.assign((size_t)s->plane * s->N, 0.0);
+  if (!s->mixed)
+    return copy_sync(planes.data(), s->prim, prim_bytes(s), hipMemcpyDeviceToHost, s->stream) == hipSuccess
+               ? HYDRO_OK
+               : fail(HYDRO_E_HIP, "copy (primitives)");
+  std::vector<float> f(planes.size());
+  if (copy_sync(f.data(), s->prim, prim_bytes(s), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+    return fail(HYDRO_E_HIP, "copy (primitives)");
+  std::copy(f.begin(), f.end(), planes.begin());
+  return HYDRO_OK;
+}
+
 int enqueue_dt(hydro_sim* s, bool advance_time) {
-  return enqueue_cfl(s->cg, s->U[s->cur], s->prim, s->blockmin, finalize_args(s, advance_time), s->stream);
+  return enqueue_cfl(s->cg, s->U[s->cur], s->prim, !s->mixed, s->blockmin, finalize_args(s, advance_time), s->stream);
 }
 
 template <typename T>
@@ -433,7 +465,7 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   a.plane = s->plane;
   for (int i = 0; i < 3; ++i) a.inv_dx[i] = 1.0 / s->kdx[i];
   a.gamma = s->d.gamma;
-  a.prim = s->prim;
+  a.prim = (const T*)s->prim;
   maxwell_constants(s->d, a.mx);
   a.lim = s->d.limiter;
   const int interior_x = s->sz[0] - 4;
@@ -517,8 +549,10 @@ int enqueue_srhd_tail(hydro_sim* s) {
     c.ghost_src_lo = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
     s->ghosts_pending = false;
   }
-  if (s->mixed) launch_con2prim_f32(c, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->stream);
-  else launch_con2prim_f64(c, (double*)s->U[s->cur], s->prim, s->stream);
+  if (s->mixed)
+    launch_con2prim_f32(c, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, (float*)s->prim, s->stream);
+  else
+    launch_con2prim_f64(c, (double*)s->U[s->cur], (double*)s->prim, s->stream);
   HIP_TRY(hipGetLastError());
   return HYDRO_OK;
 }
@@ -1005,9 +1039,9 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     hipMemset(s->P[j], 0, bytes);
   }
   if (d->equation == HYDRO_EQN_SRHD) {
-    if (hipMalloc((void**)&s->prim, sizeof(double) * s->plane * s->N) != hipSuccess)
+    if (hipMalloc((void**)&s->prim, prim_bytes(s)) != hipSuccess)
       return cleanup(HYDRO_E_HIP, "out of device memory (primitives)");
-    hipMemset(s->prim, 0, sizeof(double) * s->plane * s->N);
+    hipMemset(s->prim, 0, prim_bytes(s));
   }
   if (hipMalloc((void**)&s->blockmin, sizeof(double) * std::max(s->dt_blocks, kDtMaxWaves)) != hipSuccess)
     return cleanup(HYDRO_E_HIP, "alloc");
@@ -1110,7 +1144,10 @@ int hydro_sim_set_state(hydro_sim* s, const double* aos, size_t n) {
       for (int a = 0; a < D; ++a) pr[(size_t)(1 + a) * s->plane + c] = v[a];
       pr[(size_t)E * s->plane + c] = eInt;
     }
-    HIP_TRY(copy_sync(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    {
+      const int rc = upload_prims(s, pr);
+      if (rc) return rc;
+    }
     aos = conv.data();
   }
   const int rc = upload_state(s, aos);
@@ -1134,7 +1171,8 @@ int hydro_sim_restore(hydro_sim* s, const double* aos, const float* lo, const do
     std::vector<double> pr((size_t)s->plane * s->N, 0.0);
     for (long long c = 0; c < s->vol; ++c)
       for (int k = 0; k < s->N; ++k) pr[(size_t)k * s->plane + c] = prims[k + (size_t)s->N * c];
-    HIP_TRY(copy_sync(s->prim, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    const int rc = upload_prims(s, pr);
+    if (rc) return rc;
   }
   const int rc = upload_state(s, aos, lo);
   if (rc) return rc;
@@ -1200,8 +1238,11 @@ int hydro_sim_get_primitives(hydro_sim* s, double* aos, size_t n) {
   if (!s->prim) return fail(HYDRO_E_UNSUPPORTED, "primitives exist for SRHD only");
   if (n != (size_t)s->N * (size_t)s->vol) return fail(HYDRO_E_INVALID, "size must be N * volume");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<double> buf((size_t)s->plane * s->N);
-  HIP_TRY(copy_sync(buf.data(), s->prim, buf.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  std::vector<double> buf;
+  {
+    const int rc = download_prims(s, buf);
+    if (rc) return rc;
+  }
   for (long long c = 0; c < s->vol; ++c)
     for (int k = 0; k < s->N; ++k) aos[k + (size_t)s->N * c] = buf[(size_t)k * s->plane + c];
   return HYDRO_OK;
@@ -1411,8 +1452,8 @@ int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_pe
                            s->stream));
   }
   if (s->prim) {
-    HIP_TRY(hipMalloc((void**)&savePrim, sizeof(double) * s->plane * s->N));
-    HIP_TRY(hipMemcpyAsync(savePrim, s->prim, sizeof(double) * s->plane * s->N, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMalloc((void**)&savePrim, prim_bytes(s)));
+    HIP_TRY(hipMemcpyAsync(savePrim, s->prim, prim_bytes(s), hipMemcpyDeviceToDevice, s->stream));
   }
   DevScalars sc0[2];
   HIP_TRY(hipMemcpyAsync(sc0, s->sc, sizeof sc0, hipMemcpyDeviceToHost, s->stream));
@@ -1426,8 +1467,7 @@ int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_pe
       HIP_TRY(hipMemcpyAsync(s->Ulo[s->cur], saveLo, sizeof(float) * s->plane * s->N, hipMemcpyDeviceToDevice,
                              s->stream));
     if (savePrim)
-      HIP_TRY(hipMemcpyAsync(s->prim, savePrim, sizeof(double) * s->plane * s->N, hipMemcpyDeviceToDevice,
-                             s->stream));
+      HIP_TRY(hipMemcpyAsync(s->prim, savePrim, prim_bytes(s), hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->sc, sc0, sizeof sc0, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return HYDRO_OK;

@@ -35,9 +35,10 @@ DtMarchArgs cfl_march_args(const CflGrid& g, double* minima);
 long long cfl_minima(const CflGrid& g);
 
 // CFL kernel + dt tail on `st`: U are the state planes (float for the
-// FP32-mixed form), prim the SRHD primitive planes; f.blockmin and
+// FP32-mixed form), prim the SRHD primitive planes (FP64 when prim_f64,
+// else FP32: the FP32-mixed simulation's storage); f.blockmin and
 // f.nblocks are filled here
-int enqueue_cfl(const CflGrid& g, const void* U, const double* prim, double* minima, FinalizeArgs f,
+int enqueue_cfl(const CflGrid& g, const void* U, const void* prim, bool prim_f64, double* minima, FinalizeArgs f,
                 hipStream_t st);
 
 }  // namespace host

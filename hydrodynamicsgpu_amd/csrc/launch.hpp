@@ -40,7 +40,7 @@ struct BoundaryArgs {
 };
 // prim: SRHD primitive planes filled by the same launch, or null
 void launch_boundary_f64(const BoundaryArgs& b, double* planes, double* prim, hipStream_t st);
-void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, double* prim, hipStream_t st);
+void launch_boundary_f32(const BoundaryArgs& b, float* planes, float* lo, float* prim, hipStream_t st);
 
 // copy ghost cells of every plane from src to dst (the cells the stage
 // kernels do not write)
@@ -57,7 +57,8 @@ struct DtArgs {
   double gamma;
   int solver;            // 0 Roe, 1 HLL, 2 HLLC
   int eqn;               // 0 Euler, 1 SRHD, 2 Maxwell
-  const double* prim;    // SRHD primitive planes
+  const void* prim;      // SRHD primitive planes
+  int prim_f32;          // ... in FP32 (else FP64)
   double maxwell_lambda; // Maxwell: 1/sqrt(eps mu)
   int nblocks;
   double* blockmin;      // [nblocks]
@@ -80,7 +81,7 @@ struct Con2PrimArgs {
   const float* ghost_src_lo;
 };
 void launch_con2prim_f64(const Con2PrimArgs& a, double* u, double* prim, hipStream_t st);
-void launch_con2prim_f32(const Con2PrimArgs& a, float* u, float* lo, double* prim, hipStream_t st);
+void launch_con2prim_f32(const Con2PrimArgs& a, float* u, float* lo, float* prim, hipStream_t st);
 void launch_dt_euler_f64(const DtArgs& a, const double* u0, hipStream_t st);
 void launch_dt_euler_f32(const DtArgs& a, const float* u0, hipStream_t st);
 
@@ -108,15 +109,17 @@ struct DtMarchArgs {
   long long plane;
   double kdx[3];
   double gamma;
-  int solver;                   // 0 Roe, 1 HLL, 2 HLLC, 3 SRHD Roe (reads the FP64 primitives)
+  int solver;                   // 0 Roe, 1 HLL, 2 HLLC, 3 SRHD Roe (reads the primitives)
   int last_x, last_y, last_z;   // last cell with a dt: S-3 (Roe), S-2 (HLL)
   int nstrips, rows, nchunks, nwaves;
   double* wavemin;
 };
 void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
-// SRHD (solver 3): reads the FP64 primitives; fp32: speeds in FP32 (FP32-mixed)
+// SRHD (solver 3) over FP64 primitive planes; fp32: speeds in FP32
 void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, hipStream_t st);
+// SRHD over FP32 primitive planes (the FP32-mixed simulation's storage), FP32 speeds
+void launch_dt_march_srhd(const DtMarchArgs& a, const float* prim, hipStream_t st);
 // 3D (Euler): waves = strips x row chunks x planes (nchunks counts row chunks)
 void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
@@ -132,7 +135,7 @@ struct HeadArgs {
   DtMarchArgs a;
   int bblocks, mblocks;
 };
-void launch_step_head(const HeadArgs& h, void* planes, float* lo, double* prim, bool fp32, hipStream_t st);
+void launch_step_head(const HeadArgs& h, void* planes, float* lo, void* prim, bool fp32, hipStream_t st);
 
 // Whole updates of a small 1D Euler grid in one workgroup (run1d.hip): per
 // step the boundary, the CFL march, the dt tail and every RK stage, with

@@ -101,7 +101,7 @@ int hydro_calc_timestep_noalloc(const hydro_sim_desc* d, const void* state, size
   f.t = &clock->t;
   f.steps = &clock->steps;
   f.health = flags_out ? flags_out : &clock->health;
-  return enqueue_cfl(g, state, prims, minima, f, st);
+  return enqueue_cfl(g, state, prims, true, minima, f, st);
 }
 
 }  // extern "C"

@@ -66,7 +66,7 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
   struct Raw {
     S q0[N];
     T dk[N];
-    double pr[Pol::kFrozenPrims ? N : 1];
+    S pr[Pol::kFrozenPrims ? N : 1];
   };
 
   auto fetch = [&](int row, Raw& w) {
@@ -82,7 +82,7 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
     T u[N];
     comb.stage_state(w.q0, w.dk, u);
     R.st = Pol::stage(u, gamma);
-    if constexpr (Pol::kFrozenPrims) R.fz = Pol::template frozen<double>(w.pr, gamma);
+    if constexpr (Pol::kFrozenPrims) R.fz = Pol::template frozen<S>(w.pr, gamma);
     else R.fz = Pol::template frozen<S>(w.q0, gamma);
   };
   auto load_row = [&](int row, Row& R) {

@@ -74,10 +74,10 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
     comb.stage_state(q0, dk, u);
     st = Pol::stage(u, gamma);
     if constexpr (Pol::kFrozenPrims) {
-      double pr[N];
+      S pr[N];
 #pragma unroll
       for (int s = 0; s < N; ++s) pr[s] = ldp(a.prim, pl, s, e, cl);
-      fz = Pol::template frozen<double>(pr, gamma);
+      fz = Pol::template frozen<S>(pr, gamma);
     } else {
       fz = Pol::template frozen<S>(q0, gamma);
     }

@@ -27,7 +27,7 @@ enum CombineMode {
 template <typename T, typename S>
 struct StageArgs {
   const S* u0;          // start-of-step state planes (FP64, or FP32 high words)
-  const double* prim;   // SRHD: start-of-step primitive planes (FP64), else null
+  const S* prim;        // SRHD: start-of-step primitive planes (storage precision), else null
   const float* u0lo;    // FP32 low words (compensated storage), or null
   const T* dk;          // d_k planes, null for k == 0
   T* dnext;             // d_{k+1} planes (non-final stage)

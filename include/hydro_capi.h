@@ -185,7 +185,10 @@ int hydro_sim_set_state(hydro_sim* sim, const double* aos, size_t n);
 /* Replaces Solver::Converter::fromGPU (src/Solver/Solver.cpp:339-343). */
 int hydro_sim_get_state(hydro_sim* sim, double* aos, size_t n);
 
-/* SRHD primitives (rho, v, eInt) in the reference layout (src/Solver/SRHDRoe.cpp:9-13). */
+/* SRHD primitives (rho, v, eInt) in the reference layout (src/Solver/SRHDRoe.cpp:9-13).
+ * They are stored in the state's precision: FP64, or FP32 for the FP32-mixed
+ * form (recovered in FP64 by con2prim, then rounded for the FP32 fluxes and
+ * speeds that read them), widened to double here. */
 int hydro_sim_get_primitives(hydro_sim* sim, double* aos, size_t n);
 
 /* The stored form of the state: for compensated FP32 storage
@@ -201,7 +204,8 @@ int hydro_sim_get_storage(hydro_sim* sim, double* hi, float* lo, size_t n);
  * hydro_sim_set_state): `lo` are the low words of compensated FP32 storage
  * (NULL: split as hydro_sim_set_state does; must be NULL for the other
  * forms), `prims` the SRHD primitives from hydro_sim_get_primitives (must
- * be NULL for the other equations).  Sets the simulated time and step count.
+ * be NULL for the other equations; rounded to FP32 for the FP32-mixed
+ * form, which gives back exactly what hydro_sim_get_primitives read).  Sets the simulated time and step count.
  * No counterpart in the reference, which has no checkpoints; it resumes a
  * run saved by the checkpoint writers (hydrodynamicsgpu_amd/io.py,
  * HydroGPU::Solver::Solver::saveCheckpoint). */
