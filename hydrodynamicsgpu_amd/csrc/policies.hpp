@@ -234,8 +234,29 @@ struct HLLPolicy {
     c.p = (gamma - T(1)) * c.rho * (eT - T(.5) * vsq);
     c.H = eT + c.p * inv;
     c.cs = tsqrt((gamma - T(1)) * (c.H - T(.5) * vsq));
-    c.w = tsqrt(c.rho);
+    c.w = SOLVER == 1 ? tsqrt(c.rho) : T(0);  // only HLL's Roe averages weigh by sqrt(rho)
     return c;
+  }
+  // What a face reads of its neighbour's frozen data: HLLC's wave-speed
+  // estimate needs rho, p, c and the normal velocity only; HLL's Roe
+  // averages need all of it.  Kernels that move a neighbour's frozen data
+  // (lane shifts, LDS) move these NFS values.
+  static constexpr int NFS = SOLVER == 2 ? 4 : NF;
+  static HYDRO_DEV void pack_side(const Frozen& c, int side, T o[NFS]) {
+    if constexpr (SOLVER == 2) {
+      o[0] = c.rho; o[1] = c.p; o[2] = c.cs; o[3] = c.v[side];
+    } else {
+      pack(c, o);
+    }
+  }
+  static HYDRO_DEV Frozen unpack_side(const T o[NFS], int side) {
+    if constexpr (SOLVER == 2) {
+      Frozen c{};
+      c.rho = o[0]; c.p = o[1]; c.cs = o[2]; c.v[side] = o[3];
+      return c;
+    } else {
+      return unpack(o);
+    }
   }
   static HYDRO_DEV void pack(const Frozen& c, T o[NF]) {
     o[0] = c.rho; o[1] = c.p; o[2] = c.cs; o[3] = c.w; o[4] = c.H;

@@ -631,7 +631,7 @@ struct XLanes<2> {
 template <class Pol, typename T, typename S, bool KAHAN, int MODE, int W, int CPL>
 __global__ void __launch_bounds__(64 * W) stage3d_zm_kernel(StageArgs<T, S> a) {
   constexpr int N = Pol::N;
-  constexpr int NS = Pol::NS, NF = Pol::NF, NP = NS + NF;
+  constexpr int NS = Pol::NS, NP = NS + Pol::NFS;  // a neighbour's stage data + the frozen data a face reads
   constexpr int XW = 64 * CPL;  // strip width in cells
   constexpr int XO = XW - 2;    // output columns per strip
   using Frozen = typename Pol::Frozen;
@@ -702,13 +702,13 @@ __global__ void __launch_bounds__(64 * W) stage3d_zm_kernel(StageArgs<T, S> a) {
       C[k].fz = Pol::template frozen<S>(r[k].q0, gamma);
     }
   };
-  auto pack = [&](const Cell& C, T o[NP]) {
+  auto pack = [&](const Cell& C, int side, T o[NP]) {
     Pol::pack_stage(C.st, o);
-    Pol::pack(C.fz, o + NS);
+    Pol::pack_side(C.fz, side, o + NS);
   };
-  auto unpack = [&](const T o[NP], Cell& C) {
+  auto unpack = [&](const T o[NP], int side, Cell& C) {
     C.st = Pol::unpack_stage(o);
-    C.fz = Pol::unpack(o + NS);
+    C.fz = Pol::unpack_side(o + NS, side);
   };
   auto flux = [&](const Cell& L, const Cell& R, int side, T F[N]) {
     const auto fa = Pol::face_a(L.fz, R.fz, L.st, R.st, side, true, pp);
@@ -717,13 +717,13 @@ __global__ void __launch_bounds__(64 * W) stage3d_zm_kernel(StageArgs<T, S> a) {
   auto xdiv = [&](const Cell C[CPL], T div[CPL][N]) {
     T pk[CPL][NP], pl[CPL][NP];
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) pack(C[k], pk[k]);
+    for (int k = 0; k < CPL; ++k) pack(C[k], 0, pk[k]);
     XLanes<CPL>::template from_left<T, NP>(pk, pl);
     T F[CPL][N], Fr[CPL][N];
 #pragma unroll
     for (int k = 0; k < CPL; ++k) {
       Cell L;
-      unpack(pl[k], L);
+      unpack(pl[k], 0, L);
       flux(L, C[k], 0, F[k]);
     }
     XLanes<CPL>::template from_right<T, N>(F, Fr);
@@ -790,7 +790,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_zm_kernel(StageArgs<T, S> a) {
 #pragma unroll
       for (int k = 0; k < CPL; ++k) {
         T pk[NP];
-        pack(C[k], pk);
+        pack(C[k], 1, pk);
 #pragma unroll
         for (int i = 0; i < NP; ++i) shr[w][i][k * 64 + lane] = pk[i];
       }
@@ -803,7 +803,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_zm_kernel(StageArgs<T, S> a) {
 #pragma unroll
           for (int i = 0; i < NP; ++i) pk[i] = shr[w - 1][i][k * 64 + lane];
           Cell L;
-          unpack(pk, L);
+          unpack(pk, 1, L);
           flux(L, C[k], 1, G[k]);
 #pragma unroll
           for (int s = 0; s < N; ++s) shf[w][s][k * 64 + lane] = G[k][s];
