@@ -391,26 +391,30 @@ HYDRO_DEV double dt_march_wave(const DtMarchArgs& a, const L* u0, int strip, int
 }
 
 // ---------------------------------------------------------------- CFL, 3D
-// The CFL of the Euler solvers on 3D grids as a strip march: a wave owns 64
-// columns of one z-plane and a chunk of rows, reads for every row the row
-// itself and the rows of the planes below and above (the waves of those
-// planes read them too: L2 hits), takes the x-faces through DPP shifts, the
-// y-face against the previous row held in registers and both z-faces, and
-// keeps its minimum.  Cells 2 .. last_x / last_y / last_z of every axis.
+// The CFL of the Euler solvers on 3D grids as a z-march: a wave owns 64
+// columns (60 output columns) of R consecutive rows and marches a
+// chunk of z-planes.  Per plane it loads the rows (and one halo row below
+// and above), builds each cell's frozen data once, takes the x-faces through
+// DPP shifts and the y-faces between consecutive rows, and the z-face
+// against the previous plane's frozen data held in registers; a plane's
+// rates are complete one plane later, with the minimal speed of its upper
+// z-face.  Every face speed is computed once per wave (the y halo rows and
+// the planes below and above the chunk once more).  Cells 2 .. last_x /
+// last_y / last_z of every axis; per cell the same face speeds and side
+// rates as the reference's calcCellTimestep, so the minimum is the same.
 // MAP: ghost reads through the boundary's composed map, as in dt_march_wave.
 template <typename S, int SOLVER, bool MAP = false>
-HYDRO_DEV double dt_march3_wave(const DtMarchArgs& a, const S* u0, int strip, int chunk, int z,
+HYDRO_DEV double dt_march3_wave(const DtMarchArgs& a, const S* u0, int strip, int ychunk, int zchunk,
                                 const BoundaryArgs* gm = nullptr) {
   using D = DtSolver<S, 3, SOLVER>;
   using Frozen = typename D::Frozen;
-  constexpr int N = 5, NF = D::NF;
+  constexpr int N = 5, NF = D::NF, R = sizeof(S) == 4 ? kDt3Rows : kDt3RowsF64;
   const int lane = threadIdx.x & 63;
   const int sx = a.sx, sy = a.sy, sz = a.sz;
   const int c = strip * 60 + lane;
   const uint32_t cl = c < sx ? c : sx - 1;
   const bool out_col = lane >= 2 && lane < 62 && c <= a.last_x;
   const bool right_ok_x = c + 1 <= sx - 2;
-  const bool right_ok_z = z + 1 <= sz - 2;
   const S gamma = S(a.gamma);
   const long long psz = (long long)sx * sy;
   bool bad = false;
@@ -433,59 +437,81 @@ HYDRO_DEV double dt_march3_wave(const DtMarchArgs& a, const S* u0, int strip, in
       for (int s = 0; s < N; ++s) q[s] = ((neg >> s) & 1u) ? -q[s] : q[s];
     }
   };
-  struct Rows {
-    S m[N], c[N], p[N];  // planes z-1, z, z+1
-  };
-  auto fetch3 = [&](int row, Rows& r) {
-    fetch(z - 1, row, r.m);
-    fetch(z, row, r.c);
-    fetch(z + 1, row, r.p);
-  };
   const double idx[3] = {1.0 / a.kdx[0], 1.0 / a.kdx[1], 1.0 / a.kdx[2]};
-  auto x_rate = [&](const Frozen& f) {
-    S pk[NF], pkL[NF];
-    D::pack(f, pk);
-#pragma unroll
-    for (int i = 0; i < NF; ++i) pkL[i] = lane_from_left(pk[i]);
-    S lmin, lmax;
-    D::speeds(D::unpack(pkL), f, 0, gamma, lmin, lmax);  // face c: cells c-1, c
-    const S lminR = lane_from_right(lmin);                // face c+1
-    return D::side_rate(lmax, lminR, right_ok_x, idx[0], bad);
-  };
-  auto z_rate = [&](const Rows& r, const Frozen& fc) {
-    S lmin0, lmax0, lmin1, lmax1;
-    D::speeds(D::frozen(r.m, gamma), fc, 2, gamma, lmin0, lmax0);  // lower face
-    D::speeds(fc, D::frozen(r.p, gamma), 2, gamma, lmin1, lmax1);  // upper face
-    return D::side_rate(lmax0, lmin1, right_ok_z, idx[2], bad);
-  };
 
-  const int y0 = 2 + chunk * a.rows;
-  int y1 = y0 + a.rows;
-  if (y1 > a.last_y + 1) y1 = a.last_y + 1;
-  double m = 0.0;  // largest rate of the wave's cells
-  // step r: face r (rows r-1 | r) closes row r-1, whose x and z terms the
-  // previous step left; the loads of row r+1 are issued before row r's work
-  S q[N];
-  fetch(z, y0 - 1, q);
-  Rows nx;
-  fetch3(y0, nx);
-  Frozen prev = D::frozen(q, gamma);
-  S lmaxY_prev = S(0);
-  double xz_prev = 0.0;
-#pragma unroll 1
-  for (int r = y0; r <= y1; ++r) {
-    const Rows cur = nx;
-    if (r < y1) fetch3(r + 1, nx);  // r + 1 <= y1 <= sy - 1
-    const Frozen fc = D::frozen(cur.c, gamma);
-    S lminY, lmaxY;
-    D::speeds(prev, fc, 1, gamma, lminY, lmaxY);  // face r
-    if (r > y0) {
-      const double yd = D::side_rate(lmaxY_prev, lminY, r <= sy - 2, idx[1], bad);
-      if (out_col) m = fmax(m, fmax(xz_prev, yd));
+  // rows y0 .. y0+nr-1 are this wave's output rows; row slot i holds row
+  // y0-1+i (0: the halo row below, nr+1: the halo row above; slots past it
+  // repeat row y0+nr and are not used)
+  const int y0 = 2 + ychunk * R;
+  const int nr = a.last_y - y0 + 1 < R ? a.last_y - y0 + 1 : R;
+  const int z0 = 2 + zchunk * a.zrows;
+  const int z1 = a.last_z < z0 + a.zrows - 1 ? a.last_z : z0 + a.zrows - 1;  // output planes z0 .. z1
+  auto slot_row = [&](int i) { return i <= nr + 1 ? y0 - 1 + i : y0 + nr; };
+
+  Frozen fp[R];         // frozen data of the previous plane, output rows
+  double pend[R];       // rate of the previous plane's cells without its upper z-face
+  S lmaxz[R];           // largest speed of the previous plane's lower z-face
+  {
+    S q[N];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      fetch(z0 - 1, slot_row(i + 1), q);
+      fp[i] = D::frozen(q, gamma);
+      pend[i] = 0.0;
+      lmaxz[i] = S(0);
     }
-    if (r < y1) xz_prev = fmax(x_rate(fc), z_rate(cur, fc));
-    lmaxY_prev = lmaxY;
-    prev = fc;
+  }
+  double m = 0.0;  // largest rate of the wave's cells
+#pragma unroll 1
+  for (int z = z0; z <= z1 + 1; ++z) {
+    const bool out_plane = z <= z1;
+    S q[R + 2][N];
+#pragma unroll
+    for (int i = 0; i < R + 2; ++i)
+      if (out_plane || (i >= 1 && i <= R)) fetch(z, slot_row(i), q[i]);
+    // z-faces between planes z-1 and z: the upper face of plane z-1 (whose
+    // rates it completes) and the lower face of plane z
+    Frozen fz[R];
+    S lminz[R], lmaxz_new[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      fz[i] = D::frozen(q[i + 1], gamma);
+      D::speeds(fp[i], fz[i], 2, gamma, lminz[i], lmaxz_new[i]);
+    }
+    if (z > z0) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        if (i >= nr) break;
+        const double zr = D::side_rate(lmaxz[i], lminz[i], z <= sz - 2, idx[2], bad);
+        if (out_col) m = fmax(m, fmax(pend[i], zr));
+      }
+    }
+    if (!out_plane) break;
+    // x- and y-faces of plane z
+    Frozen lo = D::frozen(q[0], gamma);
+    S lminy, lmaxy;
+    D::speeds(lo, fz[0], 1, gamma, lminy, lmaxy);  // lower y-face of row y0
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const Frozen up = i + 1 < R ? fz[i + 1] : D::frozen(q[R + 1], gamma);
+      S lminy_up, lmaxy_up;
+      D::speeds(fz[i], up, 1, gamma, lminy_up, lmaxy_up);  // upper y-face of row y0+i
+      if (i < nr) {
+        S pk[NF], pkL[NF];
+        D::pack(fz[i], pk);
+#pragma unroll
+        for (int k = 0; k < NF; ++k) pkL[k] = lane_from_left(pk[k]);
+        S lminx, lmaxx;
+        D::speeds(D::unpack(pkL), fz[i], 0, gamma, lminx, lmaxx);  // face c: cells c-1, c
+        const S lminxR = lane_from_right(lminx);                  // face c+1
+        const double xr = D::side_rate(lmaxx, lminxR, right_ok_x, idx[0], bad);
+        const double yr = D::side_rate(lmaxy, lminy_up, y0 + i + 1 <= sy - 2, idx[1], bad);
+        pend[i] = fmax(xr, yr);
+      }
+      lmaxy = lmaxy_up;
+      fp[i] = fz[i];
+      lmaxz[i] = lmaxz_new[i];
+    }
   }
   return wave_dt(m, bad && out_col);
 }

@@ -255,15 +255,13 @@ void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, h
   }
 }
 
-// wave -> (strip, row chunk, plane), planes slowest
+// wave -> (strip, row chunk, z-chunk), z-chunks slowest
 template <typename S, int SOLVER>
 __global__ void __launch_bounds__(256) dt_march3_kernel(DtMarchArgs a, const S* u0) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (wave >= a.nwaves) return;
-  const int per_plane = a.nstrips * a.nchunks;
-  const int z = 2 + wave / per_plane;
-  const int rest = wave % per_plane;
-  const double m = dt_march3_wave<S, SOLVER>(a, u0, rest % a.nstrips, rest / a.nstrips, z);
+  const int rest = wave / a.nstrips;
+  const double m = dt_march3_wave<S, SOLVER>(a, u0, wave % a.nstrips, rest % a.nchunks, rest / a.nchunks);
   if ((threadIdx.x & 63) == 0) a.wavemin[wave] = m;
 }
 
@@ -280,26 +278,28 @@ void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st)
 // ---------------------------------------------------------------- step head
 // boundary + CFL march in one launch (HeadArgs).  The marching workgroups
 // read every ghost cell through the boundary's map, so the ghost-writing
-// workgroups (dispatched first) run beside them; each marching workgroup
-// leaves its minimum for the dt tail.
+// workgroups run beside them; each marching workgroup leaves its minimum
+// for the dt tail.
 template <typename S, int DIM, int SOLVER>
 __global__ void __launch_bounds__(256) step_head_kernel(HeadArgs h, S* planes, float* lo, S* prim) {
-  if ((int)blockIdx.x < h.bblocks) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // the marching workgroups take the low block indices (dispatched first;
+  // the short ghost-writing workgroups fill the slots they leave)
+  const int bb = (int)blockIdx.x - h.mblocks;
+  if (bb >= 0) {
+    const long long g = (long long)bb * blockDim.x + threadIdx.x;
     if (g >= ghost_count(h.b.dim, h.b.sx, h.b.sy, h.b.sz)) return;
     boundary_cell<S>(h.b, planes, lo, g);
     if (prim) boundary_cell<S>(h.b, prim, nullptr, g);
     return;
   }
-  const int mb = (int)blockIdx.x - h.bblocks;
+  const int mb = (int)blockIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(mb * 4 + (int)(threadIdx.x >> 6));
   double m = INFINITY;
   if (wave < h.a.nwaves) {
     if constexpr (DIM == 3) {
-      const int per_plane = h.a.nstrips * h.a.nchunks;
-      const int z = 2 + wave / per_plane;
-      const int rest = wave % per_plane;
-      m = dt_march3_wave<S, SOLVER, true>(h.a, planes, rest % h.a.nstrips, rest / h.a.nstrips, z, &h.b);
+      const int rest = wave / h.a.nstrips;
+      m = dt_march3_wave<S, SOLVER, true>(h.a, planes, wave % h.a.nstrips, rest % h.a.nchunks, rest / h.a.nchunks,
+                                          &h.b);
     } else if constexpr (SOLVER == 3) {
       m = dt_march_wave<S, DIM, 3, S, true>(h.a, prim, wave % h.a.nstrips, wave / h.a.nstrips, &h.b);
     } else {

@@ -109,7 +109,9 @@ int cfl_path(const CflGrid& g) {
     const long long waves = strips * (g.dim > 1 ? (g.sz[1] - hi - 1 + kDtRows - 1) / kDtRows : 1);
     if (waves <= kDtMaxWaves) return 1;
   }
-  if (g.dim == 3 && d.equation == HYDRO_EQN_EULER && strips * (g.sz[2] - hi - 1) <= kDtMaxWaves) return 2;
+  if (g.dim == 3 && d.equation == HYDRO_EQN_EULER &&
+      strips * ((g.sz[1] - hi - 1 + kDt3RowsF64 - 1) / kDt3RowsF64) <= kDtMaxWaves)
+    return 2;
   return 3;
 }
 
@@ -130,10 +132,15 @@ DtMarchArgs march3_args(const CflGrid& g, double* minima) {
   a.last_z = g.sz[2] - hi;
   a.nstrips = (a.last_x - 2) / 60 + 1;
   const long long ny = a.last_y - 1, nz = a.last_z - 1;
-  a.rows = 8;
-  while ((long long)a.nstrips * ((ny + a.rows - 1) / a.rows) * nz > kDtMaxWaves && a.rows < ny) a.rows *= 2;
+  a.rows = g.mixed ? kDt3Rows : kDt3RowsF64;  // the device function's R
   a.nchunks = (int)((ny + a.rows - 1) / a.rows);
-  a.nwaves = (int)((long long)a.nstrips * a.nchunks * nz);
+  // z-chunks of 8 planes or more: about four waves per SIMD, and no more
+  // waves than minima slots
+  a.zrows = 8;
+  auto waves = [&](int zr) { return (long long)a.nstrips * a.nchunks * ((nz + zr - 1) / zr); };
+  while (a.zrows < nz && (waves(a.zrows) > 4096 || waves(a.zrows) > kDtMaxWaves)) a.zrows *= 2;
+  a.nzchunks = (int)((nz + a.zrows - 1) / a.zrows);
+  a.nwaves = (int)waves(a.zrows);
   a.wavemin = minima;
   return a;
 }

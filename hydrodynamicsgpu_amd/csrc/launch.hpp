@@ -104,6 +104,8 @@ void launch_finalize_dt(const FinalizeArgs& f, hipStream_t st);
 // wave into wavemin[nwaves] (NaN where a speed was not finite), reduced by
 // launch_finalize_dt over nblocks = nwaves
 constexpr int kDtRows = 10;  // rows per wave of the strip-march CFL kernel (at most)
+constexpr int kDt3Rows = 8;     // rows per wave of the 3D CFL z-march (FP32)
+constexpr int kDt3RowsF64 = 4;  // ... (FP64: twice the registers per value)
 struct DtMarchArgs {
   int dim, sx, sy, sz;
   long long plane;
@@ -112,6 +114,7 @@ struct DtMarchArgs {
   int solver;                   // 0 Roe, 1 HLL, 2 HLLC, 3 SRHD Roe (reads the primitives)
   int last_x, last_y, last_z;   // last cell with a dt: S-3 (Roe), S-2 (HLL)
   int nstrips, rows, nchunks, nwaves;
+  int zrows, nzchunks;          // 3D: planes per z-chunk and their count
   double* wavemin;
 };
 void launch_dt_march_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
@@ -120,16 +123,16 @@ void launch_dt_march_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
 void launch_dt_march_srhd(const DtMarchArgs& a, const double* prim, bool fp32, hipStream_t st);
 // SRHD over FP32 primitive planes (the FP32-mixed simulation's storage), FP32 speeds
 void launch_dt_march_srhd(const DtMarchArgs& a, const float* prim, hipStream_t st);
-// 3D (Euler): waves = strips x row chunks x planes (nchunks counts row chunks)
+// 3D (Euler) z-march: waves = strips x row chunks x z-chunks, strips fastest
 void launch_dt_march3_f64(const DtMarchArgs& a, const double* u0, hipStream_t st);
 void launch_dt_march3_f32(const DtMarchArgs& a, const float* u0, hipStream_t st);
 
 // The head of an update in one launch: boundary and CFL strip march (1D/2D,
-// or 3D Euler).  Workgroups 0 .. bblocks-1 write the ghost cells (and the
-// SRHD primitives' ghosts when prim is set); the mblocks others march,
-// reading ghost cells through the boundary's composed map so that they need
-// not wait for it, and leave one minimum per workgroup in a.wavemin for the
-// dt tail (launch_finalize_dt over mblocks minima).
+// or 3D Euler).  Workgroups 0 .. mblocks-1 march, reading ghost cells
+// through the boundary's composed map so that they need not wait for the
+// bblocks others, which write the ghost cells (and the SRHD primitives'
+// ghosts when prim is set); each marching workgroup leaves one minimum in
+// a.wavemin for the dt tail (launch_finalize_dt over mblocks minima).
 struct HeadArgs {
   BoundaryArgs b;
   DtMarchArgs a;
