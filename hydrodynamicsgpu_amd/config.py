@@ -11,7 +11,7 @@ import ctypes
 # include/hydro_capi.h enums -------------------------------------------------
 
 EQUATIONS = {"Euler": 0, "SRHD": 1, "Maxwell": 2}
-SOLVERS = {"Roe": 0, "HLL": 1, "HLLC": 2}
+SOLVERS = {"Roe": 0, "HLL": 1, "HLLC": 2, "Burgers": 3}
 
 # res/SlopeLimiter.cl:5-68, in order
 LIMITERS = [
@@ -44,6 +44,7 @@ SOLVER_NAMES = {
     "EulerRoe": ("Euler", "Roe"),
     "EulerHLL": ("Euler", "HLL"),
     "EulerHLLC": ("Euler", "HLLC"),
+    "EulerBurgers": ("Euler", "Burgers"),
     "SRHDRoe": ("SRHD", "Roe"),
     "MaxwellRoe": ("Maxwell", "Roe"),
 }

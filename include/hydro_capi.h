@@ -39,8 +39,9 @@ extern "C" {
 /* equations (src/Equation/{Euler,SRHD,Maxwell}.cpp) */
 enum { HYDRO_EQN_EULER = 0, HYDRO_EQN_SRHD = 1, HYDRO_EQN_MAXWELL = 2 };
 
-/* flux solvers (src/Solver/{Roe,HLL}.cpp, res/EulerHLLC.cl) */
-enum { HYDRO_SOLVER_ROE = 0, HYDRO_SOLVER_HLL = 1, HYDRO_SOLVER_HLLC = 2 };
+/* flux solvers (src/Solver/{Roe,HLL}.cpp, res/EulerHLLC.cl; EulerBurgers:
+ * src/Solver/EulerBurgers.cpp, res/EulerBurgers.cl -- Euler only) */
+enum { HYDRO_SOLVER_ROE = 0, HYDRO_SOLVER_HLL = 1, HYDRO_SOLVER_HLLC = 2, HYDRO_SOLVER_BURGERS = 3 };
 
 /* slope limiters, in the order of res/SlopeLimiter.cl:5-68 */
 enum {

@@ -47,6 +47,8 @@ struct oracle_sim {
   double* dtbuf;
   double* potential;
   double* prim;         /* SRHD primitives */
+  double* ifvel;        /* EulerBurgers: interface velocities, one per side and cell */
+  double* pressure;     /* EulerBurgers: computePressure's buffer */
   double* rk_state[4];
   double* rk_deriv[4];
   double* fe_deriv;
@@ -706,6 +708,11 @@ oracle_sim* oracle_create(const hydro_sim_desc* d, unsigned opts) {
   for (int i = 0; i < V * D; ++i) s->dtbuf[i] = DBL_MAX; /* Solver.cpp:217-222 */
   s->potential = (double*)calloc((size_t)V, sizeof(double));
   if (d->equation == HYDRO_EQN_SRHD) s->prim = (double*)calloc((size_t)N * V, sizeof(double));
+  if (d->solver == HYDRO_SOLVER_BURGERS) {
+    /* src/Solver/EulerBurgers.cpp:14-22: both zeroed at creation */
+    s->ifvel = (double*)calloc((size_t)V * D, sizeof(double));
+    s->pressure = (double*)calloc((size_t)V, sizeof(double));
+  }
   if (oracle_rk_tableau(d->integrator, &s->order, &s->alpha[0][0], &s->beta[0][0]) != 0) {
     oracle_destroy(s);
     return NULL;
@@ -724,7 +731,7 @@ oracle_sim* oracle_create(const hydro_sim_desc* d, unsigned opts) {
 void oracle_destroy(oracle_sim* s) {
   if (!s) return;
   free(s->state); free(s->eigval); free(s->eigvec); free(s->dqt); free(s->flux);
-  free(s->dtbuf); free(s->potential); free(s->prim); free(s->fe_deriv);
+  free(s->dtbuf); free(s->potential); free(s->prim); free(s->fe_deriv); free(s->ifvel); free(s->pressure);
   for (int i = 0; i < 4; ++i) { free(s->rk_state[i]); free(s->rk_deriv[i]); }
   free(s);
 }
@@ -1087,6 +1094,160 @@ static void hll_calc_flux(oracle_sim* s) {
   }
 }
 
+/* ------------------------------------------------ EulerBurgers */
+
+/* velocity, squared speed and internal energy of one Euler cell
+ * (res/EulerBurgers.cl:48-66, 311-327), with the cell's potential */
+static double burgers_eint(const oracle_sim* s, const double* st, double pot, double* vel) {
+  const int D = s->dim;
+  double density = st[0];
+  double energyTotal = st[D + 1];
+  double velocitySq = 0.;
+  for (int a = 0; a < D; ++a) {
+    vel[a] = st[1 + a] / density;
+    velocitySq += vel[a] * vel[a];
+  }
+  double specificEnergyTotal = energyTotal / density;
+  double specificEnergyKinetic = .5 * velocitySq;
+  return specificEnergyTotal - specificEnergyKinetic - pot;
+}
+
+/* res/EulerBurgers.cl:19-79: dx / (c + |v|) per axis, the smallest */
+static void burgers_calc_cell_timestep(oracle_sim* s) {
+  const int N = s->N, D = s->dim, V = s->vol;
+  const double gamma = s->gamma;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < V; ++c) {
+    int i[3];
+    decode(s, c, i);
+    if (!active(s, i, 2)) { s->dtbuf[c] = INFINITY; continue; }
+    double vel[3];
+    double specificEnergyInternal = burgers_eint(s, s->state + (size_t)N * c, s->potential[c], vel);
+    double speedOfSound = sqrt(gamma * (gamma - 1.) * specificEnergyInternal);
+    double result = s->kdx[0] / (speedOfSound + fabs(vel[0]));
+    for (int a = 1; a < D; ++a) result = fmin(result, s->kdx[a] / (speedOfSound + fabs(vel[a])));
+    s->dtbuf[c] = result;
+  }
+}
+
+/* res/EulerBurgers.cl:81-144: the mean of the two cells' normal velocities
+ * on interfaces 2 .. size-2 of every axis */
+static void burgers_calc_interface_velocity(oracle_sim* s) {
+  const int N = s->N, D = s->dim, V = s->vol;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < V; ++c) {
+    int i[3];
+    decode(s, c, i);
+    if (!active(s, i, 1)) continue;
+    for (int side = 0; side < D; ++side) {
+      int l = c - s->step[side];
+      double densityL = s->state[(size_t)N * l];
+      double velocityL = s->state[side + 1 + (size_t)N * l] / densityL;
+      double densityR = s->state[(size_t)N * c];
+      double velocityR = s->state[side + 1 + (size_t)N * c] / densityR;
+      s->ifvel[side + D * c] = .5 * (velocityL + velocityR);
+    }
+  }
+}
+
+/* res/EulerBurgers.cl:146-281: upwind advective flux of every state with the
+ * limited correction, the ratio from the upwind neighbour's jump */
+static void burgers_calc_flux(oracle_sim* s, double dt) {
+  const int N = s->N, D = s->dim, V = s->vol;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < V; ++c) {
+    int i[3];
+    decode(s, c, i);
+    if (!active(s, i, 1)) continue;
+    for (int side = 0; side < D; ++side) {
+      double dt_dx = dt / s->kdx[side];
+      int iR = c, iL = c - s->step[side], iL2 = iL - s->step[side], iR2 = c + s->step[side];
+      size_t iface = (size_t)side + D * c;
+      double interfaceVelocity = s->ifvel[iface];
+      double* flux = s->flux + (size_t)N * iface;
+      for (int j = 0; j < N; ++j) {
+        double stateR2 = s->state[j + (size_t)N * iR2];
+        double stateR = s->state[j + (size_t)N * iR];
+        double stateL = s->state[j + (size_t)N * iL];
+        double stateL2 = s->state[j + (size_t)N * iL2];
+        double deltaStateL = stateL - stateL2;
+        double deltaState = stateR - stateL;
+        double deltaStateR = stateR2 - stateR;
+        double theta, stateSlopeRatio;
+        if (interfaceVelocity >= 0.) {
+          theta = 1.;
+          stateSlopeRatio = deltaStateL / deltaState;
+        } else {
+          theta = -1.;
+          stateSlopeRatio = deltaStateR / deltaState;
+        }
+        double phi = oracle_slope_limiter(s->d.limiter, stateSlopeRatio);
+        flux[j] = .5 * interfaceVelocity * ((1. + theta) * stateL + (1. - theta) * stateR)
+                  + .5 * deltaState * phi * fabs(interfaceVelocity) * (1. - fabs(interfaceVelocity * dt_dx));
+      }
+    }
+  }
+}
+
+/* res/EulerBurgers.cl:283-348 (no artificial viscosity): cells 1 .. size-2 */
+static void burgers_compute_pressure(oracle_sim* s) {
+  const int N = s->N, D = s->dim, V = s->vol;
+  const double gamma = s->gamma;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < V; ++c) {
+    int i[3];
+    decode(s, c, i);
+    int in = 1;
+    for (int a = 0; a < D; ++a) in &= i[a] >= 1 && i[a] < s->sz[a] - 1;
+    if (!in) continue;
+    double vel[3];
+    const double* st = s->state + (size_t)N * c;
+    double specificEnergyInternal = burgers_eint(s, st, s->potential[c], vel);
+    s->pressure[c] = (gamma - 1.) * st[0] * specificEnergyInternal;
+  }
+}
+
+/* res/EulerBurgers.cl:350-406: deriv[momentum] -= (pR - pL) / (2 dx) */
+static void burgers_diffuse_momentum(oracle_sim* s, double* deriv) {
+  const int N = s->N, D = s->dim, V = s->vol;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < V; ++c) {
+    int i[3];
+    decode(s, c, i);
+    if (!active(s, i, 2)) continue;
+    double* dv = deriv + (size_t)N * c;
+    for (int side = 0; side < D; ++side) {
+      double pressureL = s->pressure[c - s->step[side]];
+      double pressureR = s->pressure[c + s->step[side]];
+      dv[1 + side] -= .5 * (pressureR - pressureL) / s->kdx[side];
+    }
+  }
+}
+
+/* res/EulerBurgers.cl:408-489: deriv[energy] -= (pR vR - pL vL) / (2 dx),
+ * velocities from the current state, pressures from the buffer as the last
+ * computePressure left it */
+static void burgers_diffuse_work(oracle_sim* s, double* deriv) {
+  const int N = s->N, D = s->dim, V = s->vol;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < V; ++c) {
+    int i[3];
+    decode(s, c, i);
+    if (!active(s, i, 2)) continue;
+    double* dv = deriv + (size_t)N * c;
+    double deltaEnergyTotal = 0.;
+    for (int side = 0; side < D; ++side) {
+      int l = c - s->step[side], r = c + s->step[side];
+      double velocityL = s->state[1 + side + (size_t)N * l] / s->state[(size_t)N * l];
+      double velocityR = s->state[1 + side + (size_t)N * r] / s->state[(size_t)N * r];
+      double pressureL = s->pressure[l];
+      double pressureR = s->pressure[r];
+      deltaEnergyTotal -= .5 * (pressureR * velocityR - pressureL * velocityL) / s->kdx[side];
+    }
+    dv[D + 1] += deltaEnergyTotal;
+  }
+}
+
 /* res/SelfGravitation.cl:65-102 */
 static void calc_gravity_deriv(oracle_sim* s, double* deriv) {
   const int N = s->N, D = s->dim, V = s->vol;
@@ -1137,7 +1298,7 @@ static void srhd_update_primitives(oracle_sim* s) {
 
 /* ------------------------------------------------ integrators */
 
-typedef enum { CB_HYDRO = 0, CB_GRAVITY, CB_MAXWELL_SOURCE } callback_t;
+typedef enum { CB_HYDRO = 0, CB_GRAVITY, CB_MAXWELL_SOURCE, CB_BURGERS_MOMENTUM, CB_BURGERS_WORK } callback_t;
 
 static void roe_calc_deriv(oracle_sim* s, double* deriv, double dt) {
   /* src/Solver/Roe.cpp:101-112 */
@@ -1154,8 +1315,23 @@ static void hll_deriv(oracle_sim* s, double* deriv) {
 
 static void run_callback(oracle_sim* s, callback_t cb, double* deriv, double dt) {
   if (cb == CB_HYDRO) {
-    if (s->d.solver == HYDRO_SOLVER_ROE) roe_calc_deriv(s, deriv, dt);
-    else hll_deriv(s, deriv);
+    if (s->d.solver == HYDRO_SOLVER_ROE) {
+      roe_calc_deriv(s, deriv, dt);
+    } else if (s->d.solver == HYDRO_SOLVER_BURGERS) {
+      /* src/Solver/EulerBurgers.cpp:70-78 */
+      burgers_calc_interface_velocity(s);
+      burgers_calc_flux(s, dt);
+      calc_flux_deriv(s, deriv);
+    } else {
+      hll_deriv(s, deriv);
+    }
+  } else if (cb == CB_BURGERS_MOMENTUM) {
+    /* src/Solver/EulerBurgers.cpp:85-91 */
+    burgers_compute_pressure(s);
+    burgers_diffuse_momentum(s, deriv);
+  } else if (cb == CB_BURGERS_WORK) {
+    /* src/Solver/EulerBurgers.cpp:94-98: computePressure not called again */
+    burgers_diffuse_work(s, deriv);
   } else if (cb == CB_GRAVITY) {
     /* src/Solver/SelfGravitation.cpp:111-123 (useGravity = false) */
     calc_gravity_deriv(s, deriv);
@@ -1215,7 +1391,7 @@ static void integrate(oracle_sim* s, double dt, callback_t cb) {
 /* Solver::initStep: HLL freezes its wave speeds (src/Solver/HLL.cpp:225-227);
  * Roe has nothing to do before calcTimestep. */
 void oracle_init_step(oracle_sim* s) {
-  if (s->d.solver != HYDRO_SOLVER_ROE) hll_calc_eigenvalues(s);
+  if (s->d.solver == HYDRO_SOLVER_HLL || s->d.solver == HYDRO_SOLVER_HLLC) hll_calc_eigenvalues(s);
 }
 
 double oracle_calc_timestep(oracle_sim* s) {
@@ -1223,6 +1399,9 @@ double oracle_calc_timestep(oracle_sim* s) {
     /* Roe::calcTimestep (src/Solver/Roe.cpp:88-92) */
     calc_eigen_basis(s);
     roe_calc_cell_timestep(s);
+  } else if (s->d.solver == HYDRO_SOLVER_BURGERS) {
+    /* EulerBurgers::calcTimestep (src/Solver/EulerBurgers.cpp:61-65) */
+    burgers_calc_cell_timestep(s);
   } else {
     /* HLL::calcTimestep (src/Solver/HLL.cpp:229-232); eigenvalues from initStep */
     hll_calc_cell_timestep(s);
@@ -1231,6 +1410,19 @@ double oracle_calc_timestep(oracle_sim* s) {
 }
 
 void oracle_step(oracle_sim* s, double dt) {
+  if (s->d.solver == HYDRO_SOLVER_BURGERS) {
+    /* EulerBurgers::step (src/Solver/EulerBurgers.cpp:67-100): advection,
+       boundary, the zero-potential pass, momentum diffusion, boundary, work
+       diffusion, boundary */
+    integrate(s, dt, CB_HYDRO);
+    oracle_boundary(s);
+    if (s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) integrate(s, dt, CB_GRAVITY);
+    integrate(s, dt, CB_BURGERS_MOMENTUM);
+    oracle_boundary(s);
+    integrate(s, dt, CB_BURGERS_WORK);
+    oracle_boundary(s);
+    return;
+  }
   integrate(s, dt, CB_HYDRO);
   if (s->d.equation == HYDRO_EQN_EULER && (s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS)) {
     /* EulerRoe::step / EulerHLL::step -> SelfGravitation::applyPotential */
