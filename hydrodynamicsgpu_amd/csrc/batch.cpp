@@ -320,7 +320,7 @@ int hydro_batch_create(hydro_ctx* ctx, const hydro_sim_desc* descs, int count, h
   for (int m = 0; m < count; ++m) {
     const hydro_sim_desc& d = descs[m];
     if (int rc = validate(&d)) return rc;
-    if (d.equation != HYDRO_EQN_EULER)
+    if (d.equation != HYDRO_EQN_EULER || d.solver == HYDRO_SOLVER_BURGERS)
       return fail(HYDRO_E_UNSUPPORTED, "batched members are Euler grids (Roe, HLL or HLLC)");
     if (d.dim > 2) return fail(HYDRO_E_UNSUPPORTED, "batched members are 1D or 2D");
     if (d.solver != d0.solver || d.limiter != d0.limiter || d.integrator != d0.integrator ||

@@ -102,6 +102,7 @@ namespace {
 int cfl_path(const CflGrid& g) {
   const hydro_sim_desc& d = *g.d;
   if (d.use_fixed_dt) return 0;
+  if (d.solver == HYDRO_SOLVER_BURGERS) return 3;  // cell-local speeds: block minima
   const int hi = d.solver == HYDRO_SOLVER_ROE ? 3 : 2;
   const long long strips = (g.sz[0] - hi - 2) / 60 + 1;
   const bool srhd = d.equation == HYDRO_EQN_SRHD;
@@ -198,8 +199,15 @@ int enqueue_cfl(const CflGrid& g, const void* U, const void* prim, bool prim_f64
     a.maxwell_lambda = mx[4];
     a.nblocks = g.dt_blocks;
     a.blockmin = minima;
-    if (g.mixed) launch_dt_euler_f32(a, (const float*)U, st);
-    else launch_dt_euler_f64(a, (const double*)U, st);
+    if (d.solver == HYDRO_SOLVER_BURGERS) {
+      // EulerBurgers::calcTimestep (src/Solver/EulerBurgers.cpp:61-65)
+      if (g.mixed) launch_dt_burgers_f32(a, (const float*)U, st);
+      else launch_dt_burgers_f64(a, (const double*)U, st);
+    } else if (g.mixed) {
+      launch_dt_euler_f32(a, (const float*)U, st);
+    } else {
+      launch_dt_euler_f64(a, (const double*)U, st);
+    }
     f.nblocks = g.dt_blocks;
     break;
   }
@@ -234,6 +242,8 @@ struct hydro_sim {
   bool replay = false;
   void* P[kMaxPartials] = {nullptr, nullptr, nullptr};
   void* prim = nullptr;  // SRHD primitive planes (the state's precision: double, or float when mixed)
+  void* pres[2] = {nullptr, nullptr};  // EulerBurgers: pressure planes of the momentum pass's stage states
+  double work_coef = 1.0;              // EulerBurgers: the tableau's sum for a constant derivative
   double* blockmin = nullptr;
   double* partial = nullptr;
   DevScalars* sc = nullptr;
@@ -564,6 +574,75 @@ int enqueue_srhd_tail(hydro_sim* s) {
   return HYDRO_OK;
 }
 
+// EulerBurgers::step after the advection pass (src/Solver/EulerBurgers.cpp:80-99):
+// boundary, the momentum pass (computePressure + diffuseMomentum per stage),
+// boundary, the work pass (diffuseWork with the pressure the last momentum
+// stage computed), boundary.  The zero-potential pass between the advection
+// boundary and the momentum pass is skipped as for EulerRoe (its derivative
+// is +0; its ForwardEuler boundary repeats the one just applied).
+int enqueue_burgers_pressure_passes(hydro_sim* s) {
+  int rc = enqueue_boundary(s);
+  if (rc) return rc;
+  BurgersArgs b{};
+  b.dim = s->dim;
+  b.sx = s->sz[0];
+  b.sy = s->sz[1];
+  b.sz = s->sz[2];
+  b.plane = s->plane;
+  b.gamma = s->d.gamma;
+  for (int i = 0; i < 3; ++i) b.inv_dx[i] = 1.0 / s->kdx[i];
+  b.dt = &s->sc->dt;
+  b.work_coef = s->work_coef;
+  if (s->mixed)
+    launch_burgers_pressure_f32(b, (const float*)s->U[s->cur], (float*)s->pres[0], (float*)s->pres[1], s->stream);
+  else
+    launch_burgers_pressure_f64(b, (const double*)s->U[s->cur], (double*)s->pres[0], (double*)s->pres[1],
+                                s->stream);
+  HIP_TRY(hipGetLastError());
+  // momentum pass: stage k reads pres[k & 1] and leaves pres[(k + 1) & 1]
+  b.u0 = s->U[s->cur];
+  b.u0lo = s->kahan ? s->Ulo[s->cur] : nullptr;
+  b.unew = s->U[s->cur ^ 1];
+  b.unewlo = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
+  const int S = s->tab.order;
+  for (int k = 0; k < S; ++k) {
+    const int mode = s->plan[k].mode;
+    const long long pl = s->plane;
+    auto shift = [pl](auto p) { return p ? p + pl : p; };
+    if (s->mixed) {
+      StageArgs<float, float> a{};
+      fill_stage_args<float>(s, k, a);
+      a.u0 = shift(a.u0); a.u0lo = shift(a.u0lo); a.dk = shift(a.dk); a.dnext = shift(a.dnext);
+      a.unew = shift(a.unew); a.unewlo = shift(a.unewlo); a.pkp = shift(a.pkp);
+      for (int j = 0; j < kMaxPartials; ++j) { a.fp[j] = shift(a.fp[j]); a.rep[j] = shift(a.rep[j]); }
+      launch_burgers_momentum_f32(mode, s->kahan, a, b, (const float*)s->pres[k & 1], (float*)s->pres[(k + 1) & 1],
+                                  s->stream);
+    } else {
+      StageArgs<double, double> a{};
+      fill_stage_args<double>(s, k, a);
+      a.u0 = shift(a.u0); a.dk = shift(a.dk); a.dnext = shift(a.dnext);
+      a.unew = shift(a.unew); a.pkp = shift(a.pkp);
+      for (int j = 0; j < kMaxPartials; ++j) { a.fp[j] = shift(a.fp[j]); a.rep[j] = shift(a.rep[j]); }
+      launch_burgers_momentum_f64(mode, a, b, (const double*)s->pres[k & 1], (double*)s->pres[(k + 1) & 1],
+                                  s->stream);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  s->cur ^= 1;
+  s->ghosts_pending = true;  // interior written; the ghost cells follow from the boundary
+  rc = enqueue_boundary(s);
+  if (rc) return rc;
+  // work pass in place on the current state
+  const int plast = (S - 1) & 1;
+  if (s->mixed)
+    launch_burgers_work_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr,
+                            (const float*)s->pres[plast], s->stream);
+  else
+    launch_burgers_work_f64(b, (double*)s->U[s->cur], (const double*)s->pres[plast], s->stream);
+  HIP_TRY(hipGetLastError());
+  return enqueue_boundary(s);
+}
+
 int enqueue_stages(hydro_sim* s) {
   {
     const int rc = materialize_ghosts(s);  // the stencil reads u0's ghost cells
@@ -600,13 +679,15 @@ int enqueue_stages(hydro_sim* s) {
   s->ghosts_pending = true;  // SRHD: copied inside the con2prim launch below
   // ForwardEuler's zero-gravity pass refreshes the ghosts after the update
   // (src/Solver/SelfGravitation.cpp:122 + src/Integrator/ForwardEuler.cpp:29-31)
+  // (EulerBurgers applies the boundary next in any case)
   if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER &&
-      s->d.equation == HYDRO_EQN_EULER) {
+      s->d.equation == HYDRO_EQN_EULER && s->d.solver != HYDRO_SOLVER_BURGERS) {
     int rc = enqueue_boundary(s);
     if (rc) return rc;
   }
   // SRHDRoe::step tail: constrainState + updatePrimitives (src/Solver/SRHDRoe.cpp:62-73)
   if (s->d.equation == HYDRO_EQN_SRHD) return enqueue_srhd_tail(s);
+  if (s->d.solver == HYDRO_SOLVER_BURGERS) return enqueue_burgers_pressure_passes(s);
   return HYDRO_OK;
 }
 
@@ -617,6 +698,7 @@ constexpr int kRun1dChunk = 256;
 
 bool run1d_eligible(const hydro_sim* s) {
   if (!s->use_run1d || s->profiling || s->dim != 1 || s->d.equation != HYDRO_EQN_EULER) return false;
+  if (s->d.solver == HYDRO_SOLVER_BURGERS) return false;
   if (!boundary_covers_ghosts(s) || s->tab.order > kRun1dMaxStages) return false;
   const int strips = (s->sz[0] - 4 + 59) / 60;  // 60 output columns per strip
   return strips <= (s->mixed ? kRun1dMaxStripsF32 : kRun1dMaxStripsF64);
@@ -665,6 +747,9 @@ bool known_buffer(const hydro_sim* s, const void* p) {
 int generic_supported(const hydro_sim* s) {
   if (s->kahan)
     return fail(HYDRO_E_UNSUPPORTED, "generic stage evaluation needs plain storage (no HYDRO_FLAG_KAHAN)");
+  if (s->d.solver == HYDRO_SOLVER_BURGERS)
+    return fail(HYDRO_E_UNSUPPORTED,
+                "EulerBurgers integrates three passes per step; its step runs fused (hydro_sim_step / update)");
   return HYDRO_OK;
 }
 
@@ -823,6 +908,8 @@ void hydro_sim_destroy(hydro_sim* s) {
   for (void* p : s->user_bufs) hipFree(p);
   if (s->aos_dev) hipFree(s->aos_dev);
   if (s->prim) hipFree(s->prim);
+  for (void* p : s->pres)
+    if (p) hipFree(p);
   if (s->blockmin) hipFree(s->blockmin);
   if (s->partial) hipFree(s->partial);
   if (s->sc) hipFree(s->sc);
@@ -1049,6 +1136,15 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
     if (hipMalloc((void**)&s->prim, prim_bytes(s)) != hipSuccess)
       return cleanup(HYDRO_E_HIP, "out of device memory (primitives)");
     hipMemset(s->prim, 0, prim_bytes(s));
+  }
+  if (d->solver == HYDRO_SOLVER_BURGERS) {
+    // src/Solver/EulerBurgers.cpp:14-22: the pressure buffer (two planes
+    // here: the momentum stages alternate between them)
+    for (auto& p : s->pres) {
+      if (hipMalloc(&p, es * s->plane) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (pressure)");
+      hipMemset(p, 0, es * s->plane);
+    }
+    s->work_coef = constant_derivative_sum(s->tab);
   }
   if (hipMalloc((void**)&s->blockmin, sizeof(double) * std::max(s->dt_blocks, kDtMaxWaves)) != hipSuccess)
     return cleanup(HYDRO_E_HIP, "alloc");

@@ -197,6 +197,19 @@ inline bool plan_stages(const Tableau& t, std::vector<StagePlan>& out, int& nslo
   return true;
 }
 
+// What the tableau's rows make of a constant derivative L: u0 + c dt L after
+// the last row, c = sum over rows of alpha c_k + beta (1 for a consistent
+// tableau; include/HydroGPU/Integrator/RungeKutta.h:84-99 with L fixed)
+inline double constant_derivative_sum(const Tableau& t) {
+  double c[5] = {0, 0, 0, 0, 0};
+  for (int i = 1; i <= t.order; ++i) {
+    double v = 0;
+    for (int k = 0; k < i; ++k) v += t.a[i - 1][k] * c[k] + t.b[i - 1][k];
+    c[i] = v;
+  }
+  return c[t.order];
+}
+
 // Fixed-function combination of a stage, or CM_GENERIC when the stage reads
 // or writes partial sums.
 inline int combine_mode(const StagePlan& p) {
@@ -313,7 +326,7 @@ inline int validate(const hydro_sim_desc* d) {
   if (d->equation < HYDRO_EQN_EULER || d->equation > HYDRO_EQN_MAXWELL) return fail(HYDRO_E_INVALID, "bad equation");
   if (d->equation != HYDRO_EQN_EULER && d->solver != HYDRO_SOLVER_ROE)
     return fail(HYDRO_E_UNSUPPORTED, "SRHD and Maxwell use the Roe solver (SRHDRoe, MaxwellRoe)");
-  if (d->solver < HYDRO_SOLVER_ROE || d->solver > HYDRO_SOLVER_HLLC) return fail(HYDRO_E_INVALID, "bad solver");
+  if (d->solver < HYDRO_SOLVER_ROE || d->solver > HYDRO_SOLVER_BURGERS) return fail(HYDRO_E_INVALID, "bad solver");
   if (d->limiter < 0 || d->limiter >= HYDRO_NUM_LIMITERS) return fail(HYDRO_E_INVALID, "bad limiter");
   if (d->integrator < 0 || d->integrator >= HYDRO_NUM_INTEGRATORS) return fail(HYDRO_E_INVALID, "bad integrator");
   if (d->precision != HYDRO_PRECISION_FP64 && d->precision != HYDRO_PRECISION_FP32_MIXED)
@@ -326,10 +339,12 @@ inline int validate(const hydro_sim_desc* d) {
   return HYDRO_OK;
 }
 
-// kernel family: 0 Euler Roe, 1 Euler HLL, 2 Euler HLLC, 3 SRHD Roe, 4 Maxwell Roe
+// kernel family: 0 Euler Roe, 1 Euler HLL, 2 Euler HLLC, 3 SRHD Roe, 4 Maxwell Roe,
+// 5 EulerBurgers
 inline int kernel_kind(const hydro_sim_desc& d) {
   if (d.equation == HYDRO_EQN_SRHD) return 3;
   if (d.equation == HYDRO_EQN_MAXWELL) return 4;
+  if (d.solver == HYDRO_SOLVER_BURGERS) return 5;
   return d.solver;
 }
 

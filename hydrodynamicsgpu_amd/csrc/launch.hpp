@@ -168,6 +168,40 @@ struct Run1dArgs {
 void launch_run1d_f64(int kind, int lim, const Run1dArgs<double, double>& a, hipStream_t st);
 void launch_run1d_f32(int kind, int lim, bool kahan, const Run1dArgs<float, float>& a, hipStream_t st);
 
+// EulerBurgers (burgers.hip): the CFL pass (per-block minima, as the
+// generic CFL kernel leaves them), and the pressure passes of its split step
+struct BurgersArgs {
+  int dim, sx, sy, sz;
+  long long plane;
+  double gamma;
+  double inv_dx[3];   // 1 / kernel dx per axis
+  const double* dt;   // device-resident step size
+  double work_coef;   // what the tableau's rows sum a constant derivative to
+  // momentum pass: its start state and the new state (unshifted planes;
+  // low words of compensated storage, or null)
+  const void* u0;
+  const float* u0lo;
+  void* unew;
+  float* unewlo;
+};
+void launch_dt_burgers_f64(const DtArgs& a, const double* u, hipStream_t st);
+void launch_dt_burgers_f32(const DtArgs& a, const float* u, hipStream_t st);
+// computePressure of u on cells 1 .. size-2 into p0, and the cells with a
+// coordinate 1 or size-2 into p1 too
+void launch_burgers_pressure_f64(const BurgersArgs& b, const double* u, double* p0, double* p1, hipStream_t st);
+void launch_burgers_pressure_f32(const BurgersArgs& b, const float* u, float* p0, float* p1, hipStream_t st);
+// one RK stage of the momentum pass: a is the stage argument block with every
+// plane pointer advanced by one plane (the x-momentum first); reads the stage
+// state's pressure pin, writes the next stage state's pressure pout
+void launch_burgers_momentum_f64(int mode, const StageArgs<double, double>& a, const BurgersArgs& b,
+                                 const double* pin, double* pout, hipStream_t st);
+void launch_burgers_momentum_f32(int mode, bool kahan, const StageArgs<float, float>& a, const BurgersArgs& b,
+                                 const float* pin, float* pout, hipStream_t st);
+// the work pass on u in place (lo: compensated storage low words, or null),
+// pressures p of the last momentum stage state
+void launch_burgers_work_f64(const BurgersArgs& b, double* u, const double* p, hipStream_t st);
+void launch_burgers_work_f32(const BurgersArgs& b, float* u, float* lo, const float* p, hipStream_t st);
+
 // buffer operations of host-side integrators: a = b + c * k over n
 // elements (multAdd), and a zero fill of n 32-bit words
 void launch_mult_add_f64(double* a, const double* b, const double* c, double k, long long n, hipStream_t st);

@@ -392,4 +392,95 @@ struct HLLPolicy {
   }
 };
 
+// ------------------------------------------------------------------ EulerBurgers
+// res/EulerBurgers.cl:81-281 (advection part of the split Euler step): every
+// state is carried by the interface velocity u = (v_L + v_R) / 2 (the mean of
+// the two cells' normal velocities) in upwind form, plus a limited
+// correction built from the state's own raw jumps -- no eigen-decomposition
+// and no frozen start-of-step data:
+//   F = u q_up + (dq phi(r) / 2) |u| (1 - |u dt/dx|),
+// q_up = q_L for u >= 0 else q_R, r = the upwind neighbour face's jump over
+// this face's jump.  The reference's .5 u ((1+theta) q_L + (1-theta) q_R) is
+// u q_up exactly (theta = +-1: the factors are 2 and 0, every product and the
+// halving exact) for finite states.  Jumps are raw state differences on every
+// face, so no face reads as zero (unlike the Roe family's never-written
+// deltaQTilde slots).
+template <int DIM, typename T, int LIM>
+struct BurgersPolicy {
+  static constexpr int N = DIM + 2;
+  static constexpr bool kNeighbors = true;
+  static constexpr bool kFrozenPrims = false;
+  struct Frozen {};
+  static constexpr int NF = 1;  // one placeholder value (device code has no zero-length arrays)
+  // stage state and its velocities, divided once per cell (the reference
+  // divides per face: the same quotient)
+  struct Stage {
+    T u[N];
+    T v[DIM];
+  };
+  static constexpr int NS = N + DIM;
+  struct FaceA {
+    T dq[N];  // q_R - q_L
+    T q[N];   // the upwind state
+    T ui;     // interface velocity
+  };
+
+  static HYDRO_DEV Stage stage(const T u[N], T) {
+    Stage c;
+#pragma unroll
+    for (int s = 0; s < N; ++s) c.u[s] = u[s];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) c.v[k] = tdiv(u[1 + k], u[0]);
+    return c;
+  }
+  static HYDRO_DEV void pack_stage(const Stage& c, T o[NS]) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) o[s] = c.u[s];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) o[N + k] = c.v[k];
+  }
+  static HYDRO_DEV Stage unpack_stage(const T o[NS]) {
+    Stage c;
+#pragma unroll
+    for (int s = 0; s < N; ++s) c.u[s] = o[s];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) c.v[k] = o[N + k];
+    return c;
+  }
+  template <typename S>
+  static HYDRO_DEV Frozen frozen(const S*, T) { return Frozen{}; }
+  static HYDRO_DEV void pack(const Frozen&, T o[NF]) { o[0] = T(0); }
+  static HYDRO_DEV Frozen unpack(const T*) { return Frozen{}; }
+
+  // res/EulerBurgers.cl:98-133 (interface velocity) and the jumps of
+  // res/EulerBurgers.cl:216-235
+  static HYDRO_DEV FaceA face_a(const Frozen&, const Frozen&, const Stage& sL, const Stage& sR, int side, bool,
+                                const PhysParams<T>&) {
+    FaceA a;
+    a.ui = T(.5) * (sL.v[side] + sR.v[side]);
+    const bool up = a.ui >= T(0);
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      a.dq[s] = sR.u[s] - sL.u[s];
+      a.q[s] = up ? sL.u[s] : sR.u[s];
+    }
+    return a;
+  }
+
+  // res/EulerBurgers.cl:236-263: the ratio from the upwind neighbour face's
+  // jump (dqlo for u >= 0, dqhi otherwise), the correction term in the
+  // reference's order ((dq phi / 2) |u|) (1 - |u dt/dx|)
+  static HYDRO_DEV void face_b(const FaceA& a, const T dqlo[N], const T dqhi[N], T dt_dx, int, int lim, T F[N]) {
+    const bool up = a.ui >= T(0);
+    const T au = tabs(a.ui);
+    const T w = T(1) - tabs(a.ui * dt_dx);
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const T num = up ? dqlo[s] : dqhi[s];
+      const T hd = half_limited_slope<LIM, T>(lim, num, a.dq[s]);
+      F[s] = a.ui * a.q[s] + (hd * au) * w;
+    }
+  }
+};
+
 }  // namespace hydro

@@ -263,11 +263,15 @@ static void launch_pol(int mode, const StageArgs<T, S>& a, int wpb, hipStream_t 
   launch_pol_mode<Pol, DIM, T, S, KAHAN, CM_GENERIC>(a, wpb, st);
 }
 
-// kind: 0 Euler Roe, 1 Euler HLL, 2 Euler HLLC, 3 SRHD Roe, 4 Maxwell Roe
+// kind: 0 Euler Roe, 1 Euler HLL, 2 Euler HLLC, 3 SRHD Roe, 4 Maxwell Roe,
+// 5 EulerBurgers (advection part)
 template <int DIM, typename T, typename S, bool KAHAN>
 static void launch_solver(int kind, int lim, int mode, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
   constexpr bool M = DIM == 2;
-  if (kind == 1) {
+  if (kind == 5) {
+    if (lim == LIM_SUPERBEE) launch_pol<BurgersPolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
+    else launch_pol<BurgersPolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+  } else if (kind == 1) {
     launch_pol<HLLPolicy<DIM, T, 1>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
   } else if (kind == 2) {
     launch_pol<HLLPolicy<DIM, T, 2>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);

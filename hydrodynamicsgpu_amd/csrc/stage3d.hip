@@ -896,6 +896,8 @@ static void launch3_solver(int kind, int lim, int mode, const StageArgs<T, S>& a
     launch3<SRHDRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
   } else if (kind == 4) {
     launch3<MaxwellRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
+  } else if (kind == 5) {
+    launch3<BurgersPolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
   } else {
     switch (lim) {
     case LIM_SUPERBEE: launch3<RoePolicy<3, T, LIM_SUPERBEE>, T, S, KAHAN, W>(mode, a, st); break;

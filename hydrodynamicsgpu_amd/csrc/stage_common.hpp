@@ -199,8 +199,10 @@ struct StageCombine {
     }
   }
 
-  // combine the derivative of a cell into the increment / state
-  HYDRO_DEV void finish(long long e, uint32_t col, const Fin& f, const T deriv[N]) const {
+  // combine the derivative of a cell into the increment / state; a stage
+  // that is not the last also hands the increment it stores to dnext_out
+  // when given
+  HYDRO_DEV void finish(long long e, uint32_t col, const Fin& f, const T deriv[N], T* dnext_out = nullptr) const {
     if constexpr (MODE == CM_DERIV) {
       // calcFluxDeriv accumulates into the caller's derivative buffer
       // (res/CalcFluxDeriv.cl:7-44: deriv -= flux differences)
@@ -208,10 +210,18 @@ struct StageCombine {
       for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, ldp((const T*)a.dnext, pl, s, e, col) + deriv[s]);
     } else if constexpr (MODE == CM_FIRST) {
 #pragma unroll
-      for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, bdt * deriv[s]);
+      for (int s = 0; s < N; ++s) {
+        const T v = bdt * deriv[s];
+        stp(a.dnext, pl, s, e, col, v);
+        if (dnext_out) dnext_out[s] = v;
+      }
     } else if constexpr (MODE == CM_MID) {
 #pragma unroll
-      for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, aself * f.dk[s] + bdt * deriv[s]);
+      for (int s = 0; s < N; ++s) {
+        const T v = aself * f.dk[s] + bdt * deriv[s];
+        stp(a.dnext, pl, s, e, col, v);
+        if (dnext_out) dnext_out[s] = v;
+      }
     } else if constexpr (MODE == CM_SINGLE) {
       T acc[N];
 #pragma unroll
@@ -253,16 +263,19 @@ struct StageCombine {
         store_final(e, col, f, acc);
       } else {
 #pragma unroll
-        for (int s = 0; s < N; ++s) stp(a.dnext, pl, s, e, col, acc[s]);
+        for (int s = 0; s < N; ++s) {
+          stp(a.dnext, pl, s, e, col, acc[s]);
+          if (dnext_out) dnext_out[s] = acc[s];
+        }
       }
     }
   }
 
   // load the operands, then combine (for callers that do not prefetch)
-  HYDRO_DEV void finish(long long e, uint32_t col, const T deriv[N]) const {
+  HYDRO_DEV void finish(long long e, uint32_t col, const T deriv[N], T* dnext_out = nullptr) const {
     Fin f;
     fetch_fin(e, col, f);
-    finish(e, col, f, deriv);
+    finish(e, col, f, deriv, dnext_out);
   }
 };
 

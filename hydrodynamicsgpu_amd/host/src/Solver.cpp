@@ -1,6 +1,6 @@
 // Solver base and the concrete solvers: the reference's virtual interface
 // (src/Solver/Solver.cpp, Roe.cpp, HLL.cpp, EulerRoe.cpp, EulerHLL.cpp,
-// EulerHLLC.cpp, SRHDRoe.cpp, MaxwellRoe.cpp) with bodies that call the C ABI
+// EulerHLLC.cpp, EulerBurgers.cpp, SRHDRoe.cpp, MaxwellRoe.cpp) with bodies that call the C ABI
 // of libhydro_hip.so.
 #include <algorithm>
 #include <cmath>
@@ -18,6 +18,7 @@
 #include "HydroGPU/Shared/Files.h"
 #include "HydroGPU/Solver/EulerHLL.h"
 #include "HydroGPU/Solver/EulerHLLC.h"
+#include "HydroGPU/Solver/EulerBurgers.h"
 #include "HydroGPU/Solver/EulerRoe.h"
 #include "HydroGPU/Solver/MaxwellRoe.h"
 #include "HydroGPU/Solver/SRHDRoe.h"
@@ -360,6 +361,7 @@ void Solver::loadCheckpoint(const std::string& path) {
 
 void EulerRoe::createEquation() { equation = std::make_shared<Equation::Euler>(app); }
 void EulerHLL::createEquation() { equation = std::make_shared<Equation::Euler>(app); }
+void EulerBurgers::createEquation() { equation = std::make_shared<Equation::Euler>(app); }
 void SRHDRoe::createEquation() { equation = std::make_shared<Equation::SRHD>(app); }
 void MaxwellRoe::createEquation() { equation = std::make_shared<Equation::Maxwell>(app); }
 
@@ -374,7 +376,8 @@ std::vector<real> SRHDRoe::getPrimitives() {
 HydroGPUApp::HydroGPUApp() {
 #define MAKE_SOLVER(S) SolverGenPair(#S, [this]() -> SolverPtr { return std::make_shared<Solver::S>(this); })
   solverGensForEqns = {
-      SolverEqnsPair("Euler", {MAKE_SOLVER(EulerHLL), MAKE_SOLVER(EulerHLLC), MAKE_SOLVER(EulerRoe)}),
+      SolverEqnsPair("Euler", {MAKE_SOLVER(EulerBurgers), MAKE_SOLVER(EulerHLL), MAKE_SOLVER(EulerHLLC),
+                               MAKE_SOLVER(EulerRoe)}),
       SolverEqnsPair("SRHD", {MAKE_SOLVER(SRHDRoe)}),
       SolverEqnsPair("Maxwell", {MAKE_SOLVER(MaxwellRoe)}),
   };

@@ -73,8 +73,8 @@ static real relerr(const std::vector<real>& a, const std::vector<real>& b) {
 
 static int run_cpu() {
   HydroGPU::HydroGPUApp app;
-  app.solverName = "EulerBurgers";  // in the reference's registry, not in this build
-  CHECK(throws([&] { app.createSolver(); }, "unknown solver EulerBurgers"), "unknown solver");
+  app.solverName = "MHDBurgers";  // in the reference's registry, not in this build
+  CHECK(throws([&] { app.createSolver(); }, "unknown solver MHDBurgers"), "unknown solver");
   app.solverName = "EulerRoe";
   app.integratorName = "BackwardEulerConjugateGradient";
   app.size = {{64, 1, 1, 1}};
@@ -193,7 +193,7 @@ static int run_gpu() {
     const bool in = std::fabs(y) < .25;
     return euler_cell(in ? 2.0 : 1.0, in ? .5 : -.5, .01 * std::sin(4 * M_PI * x), 2.5, gamma);
   };
-  for (const char* name : {"EulerRoe", "EulerHLL", "EulerHLLC"}) {
+  for (const char* name : {"EulerRoe", "EulerHLL", "EulerHLLC", "EulerBurgers"}) {
     HydroGPU::HydroGPUApp app;
     app.solverName = name;
     app.integratorName = "RungeKutta3TVD";
