@@ -101,6 +101,7 @@ def time_gpu(precision, n, steps, warmup, q0, kahan=True):
     sync_all(sim)
     el = time.perf_counter() - t1
     stage_ms, launches = sim.stage_timing()
+    phases = sim.phase_timing()
     sim.set_profiling(False)
     health = sim.health()
     state = sim.get_state()
@@ -116,7 +117,7 @@ def time_gpu(precision, n, steps, warmup, q0, kahan=True):
         host[k] = time.perf_counter() - t2
     sim.close()
     return dict(elapsed=el_clean, elapsed_profiled=el, stage_ms=stage_ms, launches=launches, health=health,
-                state=state, t=t, nsteps=nsteps, host=host)
+                state=state, t=t, nsteps=nsteps, host=host, phases=phases)
 
 
 def sod_l1(precision, kahan, n=1024):
@@ -679,6 +680,11 @@ def main():
                      "kernel_avg_ms": mixed["stage_ms"], "launches_timed": mixed["launches"],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "health": mixed["health"],
+        # device time per update by phase (HIP events while profiling; the
+        # profiled run launches kernel by kernel, so its sum exceeds
+        # ms_per_step by the launch gaps the graph replay hides)
+        "phases_us_per_update": {k: round(v[0] * 1e3 / a.steps, 2)
+                                 for k, v in mixed["phases"].items() if v[1]},
         "host_inclusive": {
             "note": "not the headline: set_state from a host FP64 AoS array + steps + get_state, wall clock",
             "value_k_steps": cells * a.steps / mixed["host"][a.steps],

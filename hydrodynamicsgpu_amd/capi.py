@@ -12,6 +12,9 @@ from . import config
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhydro_hip.so")
+
+# include/hydro_capi.h HYDRO_PHASE_*, in order
+PHASES = ("boundary", "cfl", "head", "dt_tail", "stages", "con2prim", "pressure", "ghosts", "totals")
 _lib = None
 
 
@@ -53,6 +56,7 @@ def lib():
         L.hydro_sim_health.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         L.hydro_sim_set_profiling.argtypes = [vp, ip]
         L.hydro_sim_stage_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.hydro_sim_phase_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.hydro_sim_stream.argtypes = [vp]
         L.hydro_sim_stream.restype = vp
         L.hydro_sim_element_size.argtypes = [vp]
@@ -251,6 +255,14 @@ class Sim:
         check(lib().hydro_sim_stage_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def phase_timing(self):
+        """{phase: (total ms, intervals)} of the device time spent per phase
+        of the updates since profiling was enabled (PHASES names them)."""
+        ms = (ctypes.c_double * len(PHASES))()
+        n = (ctypes.c_longlong * len(PHASES))()
+        check(lib().hydro_sim_phase_timing(self._h, ms, n))
+        return {name: (ms[i], n[i]) for i, name in enumerate(PHASES)}
+
     def decomposition(self):
         """(rows, waves per workgroup, 3D form, its waves per workgroup) of
         the stage kernels."""
@@ -278,7 +290,7 @@ EXPORTED_SYMBOLS = [
     "hydro_sim_restore", "hydro_sim_get_storage",
     "hydro_sim_boundary", "hydro_sim_calc_timestep", "hydro_sim_step", "hydro_sim_update",
     "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
-    "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing",
+    "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_sim_phase_timing",
     "hydro_sim_element_size", "hydro_integrator_tableau", "hydro_sim_buffer_create", "hydro_sim_buffer_destroy",
     "hydro_sim_state_buffer", "hydro_sim_init_step", "hydro_sim_calc_deriv", "hydro_sim_buffer_copy",
     "hydro_sim_buffer_zero", "hydro_sim_mult_add", "hydro_sim_finish_step",

@@ -9,6 +9,7 @@
 // single-workgroup launch for small 1D grids); the only host sync is the
 // one a caller asks for (update() returns dt, get_state, totals).  The
 // generic entry points at the end serve host-side integrators.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -279,12 +280,74 @@ struct hydro_sim {
   // device staging of the reference's FP64 array-of-structures state, for
   // the layout change on the GPU (allocated on first use)
   double* aos_dev = nullptr;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  // measurement: event pairs bracketing the phases of updates
+  struct PhaseEvent {
+    int phase;
+    hipEvent_t e0, e1;
+  };
+  std::vector<PhaseEvent> ev;
 };
 
 namespace {
 
 size_t esize(const hydro_sim* s) { return s->mixed ? sizeof(float) : sizeof(double); }
+
+// ---- phase timing (hydro_sim_set_profiling)
+// roctx ranges through the ROCm tracer library, loaded on first use
+struct Roctx {
+  int (*push)(const char*) = nullptr;
+  int (*pop)() = nullptr;
+};
+const Roctx& roctx() {
+  static const Roctx r = [] {
+    Roctx x;
+    void* h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);
+    if (!h) h = dlopen("librocprofiler-sdk-roctx.so", RTLD_NOW | RTLD_LOCAL);
+    if (h) {
+      x.push = reinterpret_cast<int (*)(const char*)>(dlsym(h, "roctxRangePushA"));
+      x.pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+      if (!x.push || !x.pop) x.push = nullptr, x.pop = nullptr;
+    }
+    return x;
+  }();
+  return r;
+}
+const char* const kPhaseRange[HYDRO_NUM_PHASES] = {
+    "hydro:boundary", "hydro:cfl", "hydro:head", "hydro:dt_tail", "hydro:stages",
+    "hydro:con2prim", "hydro:pressure", "hydro:ghosts", "hydro:totals"};
+
+// Brackets the work a scope enqueues on the sim's stream with two events
+// (and a roctx range) while profiling is enabled; nothing otherwise.
+class PhaseScope {
+ public:
+  PhaseScope(hydro_sim* s, int phase) : s_(s), phase_(phase) {
+    if (!s_->profiling) return;
+    if (hipEventCreate(&e0_) != hipSuccess || hipEventRecord(e0_, s_->stream) != hipSuccess) {
+      if (e0_) hipEventDestroy(e0_);
+      e0_ = nullptr;
+      return;
+    }
+    if (roctx().push) roctx().push(kPhaseRange[phase_]);
+  }
+  ~PhaseScope() {
+    if (!e0_) return;
+    if (roctx().pop) roctx().pop();
+    hipEvent_t e1 = nullptr;
+    if (hipEventCreate(&e1) == hipSuccess && hipEventRecord(e1, s_->stream) == hipSuccess) {
+      s_->ev.push_back({phase_, e0_, e1});
+    } else {
+      hipEventDestroy(e0_);
+      if (e1) hipEventDestroy(e1);
+    }
+  }
+  PhaseScope(const PhaseScope&) = delete;
+  PhaseScope& operator=(const PhaseScope&) = delete;
+
+ private:
+  hydro_sim* s_;
+  int phase_;
+  hipEvent_t e0_ = nullptr;
+};
 
 // Euler::stateGetBoundaryKernelForBoundaryMethod (src/Equation/Euler.cpp:63-75),
 // SRHD (src/Equation/SRHD.cpp:63-75), Maxwell (src/Equation/Maxwell.cpp:73-85)
@@ -361,6 +424,7 @@ bool boundary_covers_ghosts(const hydro_sim* s) {
 // interior values, so in a run of updates the copy is never needed.
 int materialize_ghosts(hydro_sim* s) {
   if (!s->ghosts_pending) return HYDRO_OK;
+  PhaseScope ph(s, HYDRO_PHASE_GHOSTS);
   const int prev = s->cur ^ 1;
   if (s->mixed) {
     launch_copy_ghosts_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const float*)s->U[prev],
@@ -378,6 +442,7 @@ int materialize_ghosts(hydro_sim* s) {
 }
 
 int enqueue_boundary(hydro_sim* s) {
+  PhaseScope ph(s, HYDRO_PHASE_BOUNDARY);
   if (boundary_covers_ghosts(s)) {
     s->ghosts_pending = false;
   } else {
@@ -447,6 +512,7 @@ int download_prims(hydro_sim* s, std::vector<double>& planes) {
 }
 
 int enqueue_dt(hydro_sim* s, bool advance_time) {
+  PhaseScope ph(s, HYDRO_PHASE_CFL);
   return enqueue_cfl(s->cg, s->U[s->cur], s->prim, !s->mixed, s->blockmin, finalize_args(s, advance_time), s->stream);
 }
 
@@ -559,6 +625,7 @@ int enqueue_srhd_tail(hydro_sim* s) {
   c.tau_min = s->d.srhd_tau_min;
   c.tau_max = s->d.srhd_tau_max;
   c.health = &s->sc->health;
+  PhaseScope ph(s, HYDRO_PHASE_CON2PRIM);
   // constrainState touches every cell, ghosts included: the deferred ghost
   // copy from the start-of-step state happens in the same launch
   if (s->ghosts_pending) {
@@ -580,35 +647,24 @@ int enqueue_srhd_tail(hydro_sim* s) {
 // stage computed), boundary.  The zero-potential pass between the advection
 // boundary and the momentum pass is skipped as for EulerRoe (its derivative
 // is +0; its ForwardEuler boundary repeats the one just applied).
-int enqueue_burgers_pressure_passes(hydro_sim* s) {
-  int rc = enqueue_boundary(s);
-  if (rc) return rc;
-  BurgersArgs b{};
-  b.dim = s->dim;
-  b.sx = s->sz[0];
-  b.sy = s->sz[1];
-  b.sz = s->sz[2];
-  b.plane = s->plane;
-  b.gamma = s->d.gamma;
-  for (int i = 0; i < 3; ++i) b.inv_dx[i] = 1.0 / s->kdx[i];
-  b.dt = &s->sc->dt;
-  b.work_coef = s->work_coef;
+// the momentum pass: the pressure of its start state, then one launch per
+// stage; stage k reads pres[k & 1] and leaves pres[(k + 1) & 1]
+int enqueue_burgers_momentum(hydro_sim* s, BurgersArgs b) {
+  PhaseScope ph(s, HYDRO_PHASE_PRESSURE);
   if (s->mixed)
     launch_burgers_pressure_f32(b, (const float*)s->U[s->cur], (float*)s->pres[0], (float*)s->pres[1], s->stream);
   else
     launch_burgers_pressure_f64(b, (const double*)s->U[s->cur], (double*)s->pres[0], (double*)s->pres[1],
                                 s->stream);
   HIP_TRY(hipGetLastError());
-  // momentum pass: stage k reads pres[k & 1] and leaves pres[(k + 1) & 1]
   b.u0 = s->U[s->cur];
   b.u0lo = s->kahan ? s->Ulo[s->cur] : nullptr;
   b.unew = s->U[s->cur ^ 1];
   b.unewlo = s->kahan ? s->Ulo[s->cur ^ 1] : nullptr;
-  const int S = s->tab.order;
-  for (int k = 0; k < S; ++k) {
+  const long long pl = s->plane;
+  auto shift = [pl](auto p) { return p ? p + pl : p; };
+  for (int k = 0; k < s->tab.order; ++k) {
     const int mode = s->plan[k].mode;
-    const long long pl = s->plane;
-    auto shift = [pl](auto p) { return p ? p + pl : p; };
     if (s->mixed) {
       StageArgs<float, float> a{};
       fill_stage_args<float>(s, k, a);
@@ -630,17 +686,40 @@ int enqueue_burgers_pressure_passes(hydro_sim* s) {
   }
   s->cur ^= 1;
   s->ghosts_pending = true;  // interior written; the ghost cells follow from the boundary
-  rc = enqueue_boundary(s);
-  if (rc) return rc;
-  // work pass in place on the current state
-  const int plast = (S - 1) & 1;
+  return HYDRO_OK;
+}
+
+// the work pass in place on the current state, with the pressure of the
+// momentum pass's last stage state
+int enqueue_burgers_work(hydro_sim* s, const BurgersArgs& b) {
+  PhaseScope ph(s, HYDRO_PHASE_PRESSURE);
+  const int plast = (s->tab.order - 1) & 1;
   if (s->mixed)
     launch_burgers_work_f32(b, (float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr,
                             (const float*)s->pres[plast], s->stream);
   else
     launch_burgers_work_f64(b, (double*)s->U[s->cur], (const double*)s->pres[plast], s->stream);
   HIP_TRY(hipGetLastError());
-  return enqueue_boundary(s);
+  return HYDRO_OK;
+}
+
+int enqueue_burgers_pressure_passes(hydro_sim* s) {
+  BurgersArgs b{};
+  b.dim = s->dim;
+  b.sx = s->sz[0];
+  b.sy = s->sz[1];
+  b.sz = s->sz[2];
+  b.plane = s->plane;
+  b.gamma = s->d.gamma;
+  for (int i = 0; i < 3; ++i) b.inv_dx[i] = 1.0 / s->kdx[i];
+  b.dt = &s->sc->dt;
+  b.work_coef = s->work_coef;
+  int rc = enqueue_boundary(s);
+  if (!rc) rc = enqueue_burgers_momentum(s, b);
+  if (!rc) rc = enqueue_boundary(s);
+  if (!rc) rc = enqueue_burgers_work(s, b);
+  if (!rc) rc = enqueue_boundary(s);
+  return rc;
 }
 
 int enqueue_stages(hydro_sim* s) {
@@ -651,12 +730,7 @@ int enqueue_stages(hydro_sim* s) {
   s->frozen = false;  // U[cur ^ 1] becomes the new state
   const int S = s->tab.order;
   for (int k = 0; k < S; ++k) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (s->profiling) {
-      HIP_TRY(hipEventCreate(&e0));
-      HIP_TRY(hipEventCreate(&e1));
-      HIP_TRY(hipEventRecord(e0, s->stream));
-    }
+    PhaseScope ph(s, HYDRO_PHASE_STAGES);
     const int mode = s->plan[k].mode;
     if (s->mixed) {
       StageArgs<float, float> a{};
@@ -668,10 +742,6 @@ int enqueue_stages(hydro_sim* s) {
       launch_stage(s, mode, a);
     }
     HIP_TRY(hipGetLastError());
-    if (s->profiling) {
-      HIP_TRY(hipEventRecord(e1, s->stream));
-      s->ev.emplace_back(e0, e1);
-    }
   }
   // ghost cells of the new state are those of u0 (the RK combination leaves
   // them unchanged: their derivative is zero); copied when first read
@@ -812,10 +882,16 @@ int enqueue_head(hydro_sim* s) {
   if (s->dim == 3)
     ghosts = 4LL * s->sz[0] * s->sz[1] + 4LL * s->sz[0] * (s->sz[2] - 4) + 4LL * (s->sz[1] - 4) * (s->sz[2] - 4);
   h.bblocks = (int)((ghosts + 255) / 256);
-  launch_step_head(h, s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->mixed, s->stream);
+  {
+    PhaseScope ph(s, HYDRO_PHASE_HEAD);
+    launch_step_head(h, s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr, s->prim, s->mixed, s->stream);
+  }
   FinalizeArgs f = finalize_args(s, true);
   f.nblocks = h.mblocks;
-  launch_finalize_dt(f, s->stream);
+  {
+    PhaseScope ph(s, HYDRO_PHASE_DT_TAIL);
+    launch_finalize_dt(f, s->stream);
+  }
   HIP_TRY(hipGetLastError());
   return HYDRO_OK;
 }
@@ -894,8 +970,8 @@ void hydro_sim_destroy(hydro_sim* s) {
   for (auto& g : s->graphN)
     if (g) hipGraphExecDestroy(g);
   for (auto& p : s->ev) {
-    hipEventDestroy(p.first);
-    hipEventDestroy(p.second);
+    hipEventDestroy(p.e0);
+    hipEventDestroy(p.e1);
   }
   for (int i = 0; i < 2; ++i) {
     if (s->U[i]) hipFree(s->U[i]);
@@ -1382,7 +1458,10 @@ int hydro_sim_step(hydro_sim* s, double dt) {
   f.t = &s->sc->t;
   f.steps = &s->sc->steps;
   f.health = &s->sc->health;
-  launch_finalize_dt(f, s->stream);
+  {
+    PhaseScope ph(s, HYDRO_PHASE_DT_TAIL);
+    launch_finalize_dt(f, s->stream);
+  }
   HIP_TRY(hipGetLastError());
   return enqueue_stages(s);
 }
@@ -1500,8 +1579,8 @@ int hydro_sim_set_profiling(hydro_sim* s, int enable) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
   HIP_TRY(hipStreamSynchronize(s->stream));
   for (auto& p : s->ev) {
-    hipEventDestroy(p.first);
-    hipEventDestroy(p.second);
+    hipEventDestroy(p.e0);
+    hipEventDestroy(p.e1);
   }
   s->ev.clear();
   s->profiling = enable != 0;
@@ -1512,13 +1591,32 @@ int hydro_sim_stage_timing(hydro_sim* s, double* avg_ms, long long* launches) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
   HIP_TRY(hipStreamSynchronize(s->stream));
   double tot = 0.0;
+  long long n = 0;
+  for (auto& p : s->ev) {
+    if (p.phase != HYDRO_PHASE_STAGES) continue;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p.e0, p.e1));
+    tot += ms;
+    ++n;
+  }
+  if (launches) *launches = n;
+  if (avg_ms) *avg_ms = n ? tot / (double)n : 0.0;
+  return HYDRO_OK;
+}
+
+int hydro_sim_phase_timing(hydro_sim* s, double* total_ms, long long* launches) {
+  if (!s || !total_ms) return fail(HYDRO_E_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < HYDRO_NUM_PHASES; ++i) {
+    total_ms[i] = 0.0;
+    if (launches) launches[i] = 0;
+  }
   for (auto& p : s->ev) {
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, p.first, p.second));
-    tot += ms;
+    HIP_TRY(hipEventElapsedTime(&ms, p.e0, p.e1));
+    total_ms[p.phase] += ms;
+    if (launches) ++launches[p.phase];
   }
-  if (launches) *launches = (long long)s->ev.size();
-  if (avg_ms) *avg_ms = s->ev.empty() ? 0.0 : tot / (double)s->ev.size();
   return HYDRO_OK;
 }
 
@@ -1646,12 +1744,15 @@ int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_pe
 
 int hydro_sim_totals(hydro_sim* s, double* totals_out) {
   if (!s || !totals_out) return fail(HYDRO_E_INVALID, "null argument");
-  if (s->mixed)
-    launch_totals_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const float*)s->U[s->cur],
-                      s->kahan ? s->Ulo[s->cur] : nullptr, s->partial, s->tot_blocks, s->stream);
-  else
-    launch_totals_f64(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const double*)s->U[s->cur],
-                      s->partial, s->tot_blocks, s->stream);
+  {
+    PhaseScope ph(s, HYDRO_PHASE_TOTALS);
+    if (s->mixed)
+      launch_totals_f32(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const float*)s->U[s->cur],
+                        s->kahan ? s->Ulo[s->cur] : nullptr, s->partial, s->tot_blocks, s->stream);
+    else
+      launch_totals_f64(s->dim, s->sz[0], s->sz[1], s->sz[2], s->plane, s->N, (const double*)s->U[s->cur],
+                        s->partial, s->tot_blocks, s->stream);
+  }
   HIP_TRY(hipGetLastError());
   std::vector<double> part((size_t)s->tot_blocks * s->N);
   HIP_TRY(hipMemcpyAsync(part.data(), s->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));

@@ -312,11 +312,37 @@ void* hydro_sim_stream(hydro_sim* sim);
 
 /* Measurement hook (no counterpart in the reference, whose event profiling
  * is commented out: src/Solver/Solver.cpp:537-544).  While enabled, every
- * fused RK-stage launch is bracketed by HIP events on the sim's stream. */
+ * phase of an update (below) is bracketed by HIP events on the sim's stream,
+ * and run() launches kernel by kernel instead of replaying graphs. */
 int hydro_sim_set_profiling(hydro_sim* sim, int enable);
 /* Average duration in ms of the fused stage launches timed since profiling
  * was enabled, and their count.  Synchronises the stream. */
 int hydro_sim_stage_timing(hydro_sim* sim, double* avg_ms, long long* launches);
+
+/* Phases of an update timed while profiling is enabled (the reference's
+ * EventProfileEntry per kernel, include/HydroGPU/Solver/Solver.h:24-31, whose
+ * readout src/Solver/Solver.cpp:537-544 keeps commented out):
+ *   BOUNDARY  ghost fill (and the deferred ghost copy it needs)
+ *   CFL       CFL kernel + dt tail of calc_timestep, or of an update whose
+ *             boundary and CFL are separate launches
+ *   HEAD      boundary and CFL march in one launch (an update's head)
+ *   DT_TAIL   the dt tail after HEAD, or the clock of step(dt)
+ *   STAGES    the fused RK stage launches
+ *   CON2PRIM  SRHD constrain + primitive recovery
+ *   PRESSURE  EulerBurgers pressure, momentum-stage and work launches
+ *   GHOSTS    deferred ghost copies outside a boundary
+ *   TOTALS    the conserved-totals reduction
+ * With profiling enabled each phase is also a roctx range
+ * ("hydro:<phase>") when the ROCm tracer library can be loaded, so that
+ * rocprofv3 --marker-trace groups kernels by phase. */
+enum {
+  HYDRO_PHASE_BOUNDARY = 0, HYDRO_PHASE_CFL, HYDRO_PHASE_HEAD, HYDRO_PHASE_DT_TAIL, HYDRO_PHASE_STAGES,
+  HYDRO_PHASE_CON2PRIM, HYDRO_PHASE_PRESSURE, HYDRO_PHASE_GHOSTS, HYDRO_PHASE_TOTALS, HYDRO_NUM_PHASES
+};
+/* Total device time in ms (total_ms[HYDRO_NUM_PHASES]) and the number of
+ * timed intervals (launches[HYDRO_NUM_PHASES], may be NULL) per phase since
+ * profiling was enabled.  Synchronises the stream. */
+int hydro_sim_phase_timing(hydro_sim* sim, double* total_ms, long long* launches);
 
 /* ---- generic stage evaluation (host-side integrators) -------------------
  *

@@ -245,3 +245,50 @@ def test_graph_replays_match_direct_launches_srhd_maxwell(gpu_lib, monkeypatch, 
         graphed.update()
         direct.update()
     assert graphed.health() == direct.health() == 0
+
+
+@pytest.mark.parametrize("solver,integ,phases", [
+    ("EulerRoe", "RungeKutta4", {"head": 1, "dt_tail": 1, "stages": 4}),
+    ("EulerHLLC", "RungeKutta3TVD", {"head": 1, "dt_tail": 1, "stages": 3}),
+    ("EulerBurgers", "RungeKutta2", {"boundary": 4, "cfl": 1, "stages": 2, "pressure": 2}),
+    ("SRHDRoe", "RungeKutta2", {"head": 1, "dt_tail": 1, "stages": 2, "con2prim": 1}),
+    ("MaxwellRoe", "ForwardEuler", {"boundary": 1, "cfl": 1, "stages": 1}),
+])
+def test_phase_timing_counts_every_phase(gpu_lib, solver, integ, phases):
+    """Per-phase device timing: each phase of an update is one interval per
+    launch group, with a positive duration; untimed phases stay at zero."""
+    from hydrodynamicsgpu_amd import capi
+
+    size = [64, 48]
+    kw = {"xmin": [0.0, 0.0], "xmax": [1.0, 1.0]} if solver == "SRHDRoe" else {}
+    d = config.make_desc(size, solverName=solver, integratorName=integ, cfl=0.4,
+                         boundaryMethods=[{"min": "FREEFLOW", "max": "FREEFLOW"}] * 2, **kw)
+    if solver == "SRHDRoe":
+        q = ics.srhd_quadrants(d)
+    elif solver == "MaxwellRoe":
+        q = ics.maxwell_1(d)
+    else:
+        q = _rand_state(d, 3)
+    g = capi.Sim(d)
+    g.set_state(q)
+    g.set_profiling(True)
+    n = 3
+    for _ in range(n):
+        g.update()
+    g.totals()
+    t = g.phase_timing()
+    expect = dict(phases)
+    expect["totals"] = 1 / n
+    for name in capi.PHASES:
+        ms, cnt = t[name]
+        want = round(expect.get(name, 0) * n)
+        if name == "pressure" and solver == "EulerBurgers":
+            want = n * 2  # momentum pass and work pass
+        if name == "boundary" and solver == "EulerBurgers":
+            want = n * 4  # the update's own and one after each pass
+        assert cnt == want, (name, cnt, want, t)
+        assert (ms > 0) == (want > 0), (name, ms)
+    ms, launches = g.stage_timing()
+    assert launches == t["stages"][1] and ms > 0
+    g.set_profiling(False)
+    assert all(v == (0.0, 0) for v in g.phase_timing().values())
