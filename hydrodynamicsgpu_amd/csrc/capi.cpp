@@ -256,6 +256,7 @@ struct hydro_sim {
   // stage decomposition
   int rows = 32, wpb = 4;
   int zwaves = 8;  // first-order 3D kernel: waves per workgroup
+  int rwaves = 8;  // 3D Roe-family kernel: rows (waves) per workgroup, 8 or 16 (FP32)
   int zshare = 0;  // first-order 3D kernel: modes whose rows go through LDS (bit per CombineMode)
   int zmarch = 0;  // first-order 3D kernel: z-march form with this many cells per lane (0: y-march form)
   int form3d = 0;  // first-order 3D kernel: one of the kForm3d* forms
@@ -408,6 +409,18 @@ void set_form3d(hydro_sim* s, int form, int zwaves = 0) {
   const long long lds = (long long)s->zwaves * (3 * s->N + 4) * 64 * cpl * (long long)esize(s);
   const long long per_cu = std::max(1LL, std::min(160LL * 1024 / lds, 32LL / s->zwaves));
   s->rows = pick_planes_zm(zstrips * yblocks, s->sz[2] - 4, s->ncu * per_cu);
+}
+
+// z-planes per chunk of the 3D Roe-family kernel with workgroups of `waves`
+// rows: the longest chunks (64 planes at most) that still give about four
+// workgroups per CU
+int pick_planes_3d_roe(const hydro_sim* s, int waves) {
+  const long long strips = (s->sz[0] - 4 + 59) / 60;
+  const int ry = stage3d_rows_per_block(waves);
+  const long long ych = (s->sz[1] - 4 + ry - 1) / ry;
+  int rows = 64;
+  while (rows > 8 && strips * ych * ((s->sz[2] - 4 + rows - 1) / rows) < 1024) rows >>= 1;
+  return rows;
 }
 
 // true when the boundary rewrites every ghost cell (no side is NONE)
@@ -582,7 +595,8 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
     }
   } else {
     // 3D Roe family: rows = z-planes per workgroup, nchunks = y-chunks, nwaves = workgroups
-    const int ry = stage3d_rows_per_block(kernel_kind(s->d));
+    const int ry = stage3d_rows_per_block(s->rwaves);
+    a.zwaves = s->rwaves;
     a.rows = s->rows;
     a.nchunks = (s->sz[1] - 4 + ry - 1) / ry;
     const int nz = (s->sz[2] - 4 + a.rows - 1) / a.rows;
@@ -1051,9 +1065,11 @@ bool tune_lookup(const std::string& key, TuneParams& p) {
 bool first_order_3d(const hydro_sim* s) {
   return s->dim == 3 && (kernel_kind(s->d) == 1 || kernel_kind(s->d) == 2);
 }
+bool roe_family_3d(const hydro_sim* s) { return s->dim == 3 && !first_order_3d(s); }
 
+// the 4th decomposition parameter: waves per workgroup of the 3D kernel
 TuneParams current_params(const hydro_sim* s) {
-  return TuneParams{s->rows, s->wpb, s->form3d, s->zwaves, 0.0};
+  return TuneParams{s->rows, s->wpb, s->form3d, roe_family_3d(s) ? s->rwaves : s->zwaves, 0.0};
 }
 
 void drop_graphs(hydro_sim* s) {
@@ -1074,6 +1090,7 @@ void apply_params(hydro_sim* s, const TuneParams& p) {
     const int form = (p.form == kForm3dZm2 && !s->mixed) ? kForm3dZm1w8 : p.form;
     set_form3d(s, form, form == kForm3dYMarch ? p.zwaves : 0);
   }
+  if (roe_family_3d(s)) s->rwaves = (s->mixed && p.zwaves == 16) ? 16 : 8;
   s->rows = std::max(1, p.rows);
   s->wpb = std::min(4, std::max(1, p.wpb));
   drop_graphs(s);
@@ -1153,12 +1170,9 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
       s->rows = std::max(1, atoi(e));
     }
   } else if (s->dim == 3) {
-    const long long strips = (s->sz[0] - 4 + 59) / 60;
-    const int ry = stage3d_rows_per_block(kernel_kind(*d));
-    const long long ych = (s->sz[1] - 4 + ry - 1) / ry;
-    int rows = 64;
-    while (rows > 8 && strips * ych * ((s->sz[2] - 4 + rows - 1) / rows) < 1024) rows >>= 1;
-    s->rows = rows;
+    s->rwaves = stage3d_roe_default_waves(s->mixed);
+    if (const char* e = getenv("HYDRO_3D_ROE_WAVES")) s->rwaves = (s->mixed && atoi(e) == 16) ? 16 : 8;
+    s->rows = pick_planes_3d_roe(s, s->rwaves);
   }
   if (const char* e = getenv("HYDRO_ROWS_PER_WAVE")) s->rows = std::max(1, atoi(e));
   if (const char* e = getenv("HYDRO_GRAPHS")) s->use_graphs = atoi(e) != 0;
@@ -1625,7 +1639,7 @@ int hydro_sim_decomposition(hydro_sim* s, int* params_out) {
   params_out[0] = s->rows;
   params_out[1] = s->wpb;
   params_out[2] = first_order_3d(s) ? s->form3d : 0;
-  params_out[3] = first_order_3d(s) ? s->zwaves : 0;
+  params_out[3] = first_order_3d(s) ? s->zwaves : roe_family_3d(s) ? s->rwaves : 0;
   return HYDRO_OK;
 }
 
@@ -1700,7 +1714,9 @@ int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_pe
       }
     }
   } else {
-    for (int r : {base.rows, 8, 16, 32, 64}) add(TuneParams{r, base.wpb, 0, base.zwaves, 0.0});
+    // 3D Roe family: z-planes per chunk x rows per workgroup (16 for FP32 only: LDS)
+    for (int w : s->mixed ? std::vector<int>{8, 16} : std::vector<int>{8})
+      for (int r : {base.rows, pick_planes_3d_roe(s, w), 8, 16, 32, 64}) add(TuneParams{r, base.wpb, 0, w, 0.0});
   }
 
   hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -16,13 +16,18 @@ void launch_stage2d_f32(int dim, int solver, int lim, int mode, const StageArgs<
 void launch_stage2d_f32k(int dim, int solver, int lim, int mode, const StageArgs<float, float>& a, int wpb,
                          hipStream_t st);
 
-// fused RK stage, 3D (stage3d.hip): workgroups of kStage3dWaves rows
+// fused RK stage, 3D (stage3d.hip).  Roe family: workgroups of a.zwaves
+// rows (8, or 16 for FP32, whose LDS rows are half as wide), 2 halo rows on
+// each side
 constexpr int kStage3dWaves = 8;
+// the default workgroup height of the Roe-family kernel
+inline int stage3d_roe_default_waves(bool fp32) { return fp32 ? 16 : 8; }
 // first-order 3D kernel: waves (z-planes) per workgroup (4 or 8), one of
 // them supplies only the top z-face.  Four measured faster in both
 // precisions on the 256^3 blast (more workgroups per CU hide the barrier).
 inline int stage3d_r1_waves(bool) { return 4; }
-int stage3d_rows_per_block(int solver);
+// output rows per workgroup of the Roe-family kernel with `waves` rows
+int stage3d_rows_per_block(int waves);
 void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, double>& a, hipStream_t st);
 void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,
                           hipStream_t st);

@@ -910,7 +910,7 @@ static void launch3_solver(int kind, int lim, int mode, const StageArgs<T, S>& a
 
 // output rows per workgroup of the Roe-family kernel: 2 halo rows per side
 // (limiter neighbours)
-int stage3d_rows_per_block(int kind) { return kStage3dWaves - 2 * 2; }
+int stage3d_rows_per_block(int waves) { return waves - 2 * 2; }
 
 // the Roe-family kernel runs the generic combination (or the derivative
 // alone); the first-order kernel takes every mode
@@ -920,8 +920,16 @@ void launch_stage3d_f64(int solver, int lim, int mode, const StageArgs<double, d
 
 void launch_stage3d_mixed(int solver, int lim, int mode, bool kahan, const StageArgs<float, float>& a,
                           hipStream_t st) {
-  if (kahan) launch3_solver<float, float, true, kStage3dWaves>(solver, lim, mode, a, st);
-  else launch3_solver<float, float, false, kStage3dWaves>(solver, lim, mode, a, st);
+  // the Roe family in 16-row workgroups when a.zwaves says so (the
+  // first-order kernels read zwaves themselves)
+  const bool w16 = solver != 1 && solver != 2 && a.zwaves == 16;
+  if (kahan) {
+    if (w16) launch3_solver<float, float, true, 16>(solver, lim, mode, a, st);
+    else launch3_solver<float, float, true, kStage3dWaves>(solver, lim, mode, a, st);
+  } else {
+    if (w16) launch3_solver<float, float, false, 16>(solver, lim, mode, a, st);
+    else launch3_solver<float, float, false, kStage3dWaves>(solver, lim, mode, a, st);
+  }
 }
 
 }  // namespace hydro
