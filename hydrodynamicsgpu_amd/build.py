@@ -28,7 +28,7 @@ COMMON = [
 
 SOURCES = ["stage2d_f64.hip", "stage2d_f32.hip", "stage2d_f32k.hip", "stage3d.hip", "aux_kernels.hip",
            "batch_f64.hip", "batch_f32.hip", "batch_f32k.hip", "batch_aux.hip", "run1d.hip", "batch.cpp", "capi.cpp",
-           "noalloc.hip", "burgers.hip"]
+           "noalloc.hip", "burgers.hip", "gravity.hip"]
 
 
 def _hipcc():

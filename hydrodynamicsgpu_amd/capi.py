@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhydro_hip.so")
 
 # include/hydro_capi.h HYDRO_PHASE_*, in order
-PHASES = ("boundary", "cfl", "head", "dt_tail", "stages", "con2prim", "pressure", "ghosts", "totals")
+PHASES = ("boundary", "cfl", "head", "dt_tail", "stages", "con2prim", "pressure", "ghosts", "totals", "gravity")
 _lib = None
 
 
@@ -57,6 +57,8 @@ def lib():
         L.hydro_sim_set_profiling.argtypes = [vp, ip]
         L.hydro_sim_stage_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.hydro_sim_phase_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.hydro_sim_set_potential.argtypes = [vp, dp, sz]
+        L.hydro_sim_get_potential.argtypes = [vp, dp, sz]
         L.hydro_sim_stream.argtypes = [vp]
         L.hydro_sim_stream.restype = vp
         L.hydro_sim_element_size.argtypes = [vp]
@@ -142,6 +144,21 @@ class Sim:
     def get_state(self):
         out = np.empty(self.N * self.volume)
         check(lib().hydro_sim_get_state(self._h, _dp(out), out.size))
+        return out
+
+    def set_potential(self, pot=None):
+        """SelfGravitation::resetState after set_state: the initial potential
+        (None: zero; with use_gravity the guess -rho, then relaxed), added
+        into every cell's total energy."""
+        if pot is None:
+            check(lib().hydro_sim_set_potential(self._h, None, 0))
+        else:
+            a = np.ascontiguousarray(pot, dtype=np.float64).ravel()
+            check(lib().hydro_sim_set_potential(self._h, _dp(a), a.size))
+
+    def get_potential(self):
+        out = np.empty(self.volume)
+        check(lib().hydro_sim_get_potential(self._h, _dp(out), out.size))
         return out
 
     def get_primitives(self):
@@ -291,6 +308,7 @@ EXPORTED_SYMBOLS = [
     "hydro_sim_boundary", "hydro_sim_calc_timestep", "hydro_sim_step", "hydro_sim_update",
     "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
     "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_sim_phase_timing",
+    "hydro_sim_set_potential", "hydro_sim_get_potential",
     "hydro_sim_element_size", "hydro_integrator_tableau", "hydro_sim_buffer_create", "hydro_sim_buffer_destroy",
     "hydro_sim_state_buffer", "hydro_sim_init_step", "hydro_sim_calc_deriv", "hydro_sim_buffer_copy",
     "hydro_sim_buffer_zero", "hydro_sim_mult_add", "hydro_sim_finish_step",

@@ -84,6 +84,9 @@ class HydroSimDesc(ctypes.Structure):
         ("maxwell_permeability", ctypes.c_double),
         ("maxwell_conductivity", ctypes.c_double),
         ("flags", ctypes.c_uint),
+        ("use_gravity", ctypes.c_int),
+        ("gauss_seidel_max_iter", ctypes.c_int),
+        ("self_grav_gravitational_constant", ctypes.c_double),
     ]
 
 
@@ -104,6 +107,7 @@ DEFAULT_DEFS = {
     "maxwell_permittivity": 1.0,
     "maxwell_permeability": 1.0,
     "maxwell_conductivity": 1.0,
+    "selfGrav_gravitationalConstant": 1.0,
 }
 
 
@@ -115,7 +119,8 @@ def lua_number(v):
 
 def make_desc(size, xmin=None, xmax=None, solverName="EulerRoe", integratorName="ForwardEuler",
               slopeLimiter="Superbee", boundaryMethods=None, cfl=None, useFixedDT=False,
-              fixedDT=0.001, defs=None, precision="fp64", flags=FLAG_REFERENCE, t_end=0.0):
+              fixedDT=0.001, defs=None, precision="fp64", flags=FLAG_REFERENCE, t_end=0.0,
+              useGravity=False, gaussSeidelMaxIter=20):
     """Build a hydro_sim_desc from reference-style config keys.
 
     `size` includes the 2+2 ghost cells per axis, as in the reference.
@@ -177,6 +182,10 @@ def make_desc(size, xmin=None, xmax=None, solverName="EulerRoe", integratorName=
     d.maxwell_permeability = lua_number(dd["maxwell_permeability"])
     d.maxwell_conductivity = lua_number(dd["maxwell_conductivity"])
     d.flags = int(flags)
+    # src/HydroGPUApp.cpp:77-78, 207-208 (useGravity false, 20 sweeps by default)
+    d.use_gravity = 1 if useGravity else 0
+    d.gauss_seidel_max_iter = int(gaussSeidelMaxIter)
+    d.self_grav_gravitational_constant = lua_number(dd["selfGrav_gravitationalConstant"])
     return d
 
 

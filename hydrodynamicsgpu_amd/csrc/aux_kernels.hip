@@ -129,6 +129,19 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
       } else if (a.eqn == 2) {
         // Maxwell: constant speeds +-1/sqrt(eps mu) (res/MaxwellRoe.cl:139-146)
         d = a.kdx[side] / (fabs(fmax(0.0, a.maxwell_lambda) - fmin(0.0, -a.maxwell_lambda)) + 1e-9);
+      } else if (a.solver == 0 && a.pot) {
+        // with the cells' potentials (res/EulerRoe.cl:56-94)
+        using P = EulerRoe<DIM, S, true>;
+        const S* pt = static_cast<const S*>(a.pot);
+        const long long cm = cell - step[side], cp = right_ok ? cell + step[side] : cell;
+        const auto cc = P::template cell<S>(q, gamma, pt[cell]);
+        const auto fL = P::face(P::template cell<S>(qm, gamma, pt[cm]), cc, side, gamma);
+        const auto fR = P::face(cc, P::template cell<S>(qp, gamma, pt[cp]), side, gamma);
+        const double smax = double(fL.v[0] + fL.c), smin = double(fR.v[0] - fR.c);
+        bad |= !(smax == smax) || !(smin == smin);
+        const double lmax = fmax(0.0, smax);
+        const double lmin = fmin(0.0, smin);
+        d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
       } else if (a.solver == 0) {
         using P = EulerRoe<DIM, S>;
         const auto cc = P::template cell<S>(q, gamma);
@@ -141,7 +154,15 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
         d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
       } else {
         S slL, srL, slR, srR;
-        if (a.solver == 1) {
+        if (a.solver == 1 && a.pot) {
+          // res/EulerHLL.cl:56-93 with the cells' potentials
+          using P = HLLPolicy<DIM, S, 1, true>;
+          const S* pt = static_cast<const S*>(a.pot);
+          const long long cm = cell - step[side], cp = right_ok ? cell + step[side] : cell;
+          const auto cc = P::template frozen_pot<S>(q, pt[cell], gamma);
+          P::speeds(P::template frozen_pot<S>(qm, pt[cm], gamma), cc, side, gamma, slL, srL);
+          P::speeds(cc, P::template frozen_pot<S>(qp, pt[cp], gamma), side, gamma, slR, srR);
+        } else if (a.solver == 1) {
           using P = HLLPolicy<DIM, S, 1>;
           const auto cc = P::template frozen<S>(q, gamma);
           P::speeds(P::template frozen<S>(qm, gamma), cc, side, gamma, slL, srL);

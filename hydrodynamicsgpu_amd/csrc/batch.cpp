@@ -323,6 +323,7 @@ int hydro_batch_create(hydro_ctx* ctx, const hydro_sim_desc* descs, int count, h
     if (d.equation != HYDRO_EQN_EULER || d.solver == HYDRO_SOLVER_BURGERS)
       return fail(HYDRO_E_UNSUPPORTED, "batched members are Euler grids (Roe, HLL or HLLC)");
     if (d.dim > 2) return fail(HYDRO_E_UNSUPPORTED, "batched members are 1D or 2D");
+    if (d.use_gravity) return fail(HYDRO_E_UNSUPPORTED, "batched members have no self-gravity (use_gravity)");
     if (d.solver != d0.solver || d.limiter != d0.limiter || d.integrator != d0.integrator ||
         d.precision != d0.precision || d.flags != d0.flags || d.use_fixed_dt != d0.use_fixed_dt ||
         (d.use_fixed_dt && d.fixed_dt != d0.fixed_dt) || d.gamma != d0.gamma)

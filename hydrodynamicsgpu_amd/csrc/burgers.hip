@@ -23,18 +23,24 @@
 
 namespace hydro {
 
-// computePressure of one cell (res/EulerBurgers.cl:311-329), zero potential:
-// (gamma - 1) rho (E/rho - |v|^2 / 2)
+// computePressure of one cell (res/EulerBurgers.cl:311-329):
+// (gamma - 1) rho (E/rho - |v|^2 / 2 - phi)
 template <int DIM, typename T>
-HYDRO_DEV T burgers_pressure(T rho, const T m[DIM], T E, T gamma) {
+HYDRO_DEV T burgers_pressure(T rho, const T m[DIM], T E, T gamma, T phi) {
   T vsq = T(0);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     const T v = tdiv(m[a], rho);
     vsq += v * v;
   }
-  const T eint = tdiv(E, rho) - T(.5) * vsq;
+  const T eint = tdiv(E, rho) - T(.5) * vsq - phi;
   return (gamma - T(1)) * rho * eint;
+}
+
+// the potential of a cell (zero without one)
+template <typename S>
+HYDRO_DEV S pot_at(const void* pot, long long c) {
+  return pot ? static_cast<const S*>(pot)[c] : S(0);
 }
 
 // ---------------------------------------------------------------- CFL
@@ -66,7 +72,7 @@ __global__ void __launch_bounds__(256) burgers_dt_kernel(DtArgs a, const S* u) {
       v[k] = tdiv(u[(1 + k) * a.plane + cell], rho);
       vsq += v[k] * v[k];
     }
-    const S eint = tdiv(E, rho) - S(.5) * vsq;
+    const S eint = tdiv(E, rho) - S(.5) * vsq - pot_at<S>(a.pot, cell);
     const S cs = tsqrt(gamma * (gamma - S(1)) * eint);
     double r = a.kdx[0] / double(cs + tabs(v[0]));
     bad |= !(r == r);
@@ -147,7 +153,7 @@ __global__ void __launch_bounds__(256) burgers_pressure_kernel(BurgersArgs b, Ce
   S mm[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) mm[a] = u[(1 + a) * b.plane + c];
-  const S p = burgers_pressure<DIM, S>(u[c], mm, u[(DIM + 1) * b.plane + c], S(b.gamma));
+  const S p = burgers_pressure<DIM, S>(u[c], mm, u[(DIM + 1) * b.plane + c], S(b.gamma), pot_at<S>(b.pot, c));
   p0[c] = p;
   bool edge = x == 1 || x == b.sx - 2;
   if (DIM > 1) edge = edge || y == 1 || y == b.sy - 2;
@@ -188,7 +194,8 @@ __global__ void __launch_bounds__(256) burgers_momentum_kernel(StageArgs<T, S> a
     T mm[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) mm[k] = T(u0[(1 + k) * b.plane + c]) + dn[k];
-    pout[c] = burgers_pressure<DIM, T>(T(u0[c]), mm, T(u0[(DIM + 1) * b.plane + c]), T(b.gamma));
+    pout[c] = burgers_pressure<DIM, T>(T(u0[c]), mm, T(u0[(DIM + 1) * b.plane + c]), T(b.gamma),
+                                       T(pot_at<S>(b.pot, c)));
   } else {
     S* un = static_cast<S*>(b.unew);
     un[c] = u0[c];

@@ -104,6 +104,7 @@ int cfl_path(const CflGrid& g) {
   const hydro_sim_desc& d = *g.d;
   if (d.use_fixed_dt) return 0;
   if (d.solver == HYDRO_SOLVER_BURGERS) return 3;  // cell-local speeds: block minima
+  if (g.pot) return 3;  // speeds with the cells' potentials: the generic kernel
   const int hi = d.solver == HYDRO_SOLVER_ROE ? 3 : 2;
   const long long strips = (g.sz[0] - hi - 2) / 60 + 1;
   const bool srhd = d.equation == HYDRO_EQN_SRHD;
@@ -200,6 +201,7 @@ int enqueue_cfl(const CflGrid& g, const void* U, const void* prim, bool prim_f64
     a.maxwell_lambda = mx[4];
     a.nblocks = g.dt_blocks;
     a.blockmin = minima;
+    a.pot = g.pot;
     if (d.solver == HYDRO_SOLVER_BURGERS) {
       // EulerBurgers::calcTimestep (src/Solver/EulerBurgers.cpp:61-65)
       if (g.mixed) launch_dt_burgers_f32(a, (const float*)U, st);
@@ -244,6 +246,12 @@ struct hydro_sim {
   void* P[kMaxPartials] = {nullptr, nullptr, nullptr};
   void* prim = nullptr;  // SRHD primitive planes (the state's precision: double, or float when mixed)
   void* pres[2] = {nullptr, nullptr};  // EulerBurgers: pressure planes of the momentum pass's stage states
+  // Euler: the gravitational potential (pot[0]; pot[1] is the relaxation's
+  // second plane), in the state's precision; pot_on when it is non-zero or
+  // relaxed from the density (use_gravity): then the kernels read it and
+  // every update adds the gravity pass
+  void* pot[2] = {nullptr, nullptr};
+  bool pot_on = false;
   double work_coef = 1.0;              // EulerBurgers: the tableau's sum for a constant derivative
   double* blockmin = nullptr;
   double* partial = nullptr;
@@ -315,7 +323,7 @@ const Roctx& roctx() {
 }
 const char* const kPhaseRange[HYDRO_NUM_PHASES] = {
     "hydro:boundary", "hydro:cfl", "hydro:head", "hydro:dt_tail", "hydro:stages",
-    "hydro:con2prim", "hydro:pressure", "hydro:ghosts", "hydro:totals"};
+    "hydro:con2prim", "hydro:pressure", "hydro:ghosts", "hydro:totals", "hydro:gravity"};
 
 // Brackets the work a scope enqueues on the sim's stream with two events
 // (and a roctx range) while profiling is enabled; nothing otherwise.
@@ -562,6 +570,7 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   for (int i = 0; i < 3; ++i) a.inv_dx[i] = 1.0 / s->kdx[i];
   a.gamma = s->d.gamma;
   a.prim = (const T*)s->prim;
+  a.pot = (s->pot_on && (kernel_kind(s->d) == 0 || kernel_kind(s->d) == 1)) ? (const T*)s->pot[0] : nullptr;
   maxwell_constants(s->d, a.mx);
   a.lim = s->d.limiter;
   const int interior_x = s->sz[0] - 4;
@@ -661,6 +670,114 @@ int enqueue_srhd_tail(hydro_sim* s) {
 // stage computed), boundary.  The zero-potential pass between the advection
 // boundary and the momentum pass is skipped as for EulerRoe (its derivative
 // is +0; its ForwardEuler boundary repeats the one just applied).
+// ---- self-gravity (src/Solver/SelfGravitation.cpp)
+GravityArgs gravity_args(const hydro_sim* s) {
+  GravityArgs g{};
+  g.dim = s->dim;
+  g.sx = s->sz[0];
+  g.sy = s->sz[1];
+  g.sz = s->sz[2];
+  g.plane = s->plane;
+  double diagsum = 0.;
+  for (int a = 0; a < 3; ++a) {
+    g.kdx[a] = s->kdx[a];
+    g.inv_2dx[a] = 1.0 / (2.0 * s->kdx[a]);
+    if (a < s->dim) diagsum += 1. / (s->kdx[a] * s->kdx[a]);
+  }
+  g.diag = -2. * diagsum;
+  // res/SelfGravitation.cl:60 (OpenCL's M_PI for double)
+  g.four_pi_g = 4. * 3.141592653589793115997963468544185161590576171875 * s->d.self_grav_gravitational_constant;
+  // SelfGravitationBehavior::gravityGetBoundaryKernelForBoundaryMethod
+  // (include/HydroGPU/Equation/SelfGravitationBehavior.h:22-34): mirror
+  // sides copy as free-flow
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < 2; ++m) {
+      signed char k = -1;
+      if (a < s->dim) {
+        switch (s->d.boundary[a][m]) {
+        case HYDRO_BOUNDARY_PERIODIC: k = 0; break;
+        case HYDRO_BOUNDARY_MIRROR:
+        case HYDRO_BOUNDARY_FREEFLOW: k = 2; break;
+        default: k = -1;
+        }
+      }
+      g.method[a][m] = k;
+    }
+  g.dt = &s->sc->dt;
+  return g;
+}
+
+// n relaxation sweeps (each with the potential's boundary before it) from
+// the density of the current state; the potential ends in pot[0]
+int enqueue_relax(hydro_sim* s, int n) {
+  if (n <= 0) return HYDRO_OK;
+  const GravityArgs g = gravity_args(s);
+  for (int i = 0; i < n; ++i) {
+    if (s->mixed)
+      launch_potential_relax_f32(g, (const float*)s->U[s->cur], (const float*)s->pot[i & 1], (float*)s->pot[(i + 1) & 1],
+                                 s->stream);
+    else
+      launch_potential_relax_f64(g, (const double*)s->U[s->cur], (const double*)s->pot[i & 1],
+                                 (double*)s->pot[(i + 1) & 1], s->stream);
+  }
+  if (n & 1) {
+    if (s->mixed) launch_copy_plane_f32(s->vol, (const float*)s->pot[1], (float*)s->pot[0], false, s->stream);
+    else launch_copy_plane_f64(s->vol, (const double*)s->pot[1], (double*)s->pot[0], false, s->stream);
+  }
+  HIP_TRY(hipGetLastError());
+  return HYDRO_OK;
+}
+
+// SelfGravitation::applyPotential (src/Solver/SelfGravitation.cpp:99-124):
+// the integrator over the gravity source, with gaussSeidelMaxIter
+// relaxation sweeps per stage when use_gravity.  Density is unchanged by the
+// pass, so every stage relaxes against the current state's density.
+// ForwardEuler keeps the boundary the callback applies (the Runge-Kutta
+// rebuild of the stage state discards it).
+int enqueue_gravity_pass(hydro_sim* s) {
+  PhaseScope ph(s, HYDRO_PHASE_GRAVITY);
+  const bool fe = s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER;
+  if (fe) {
+    const int rc = enqueue_boundary(s);
+    if (rc) return rc;
+  }
+  // the input's ghost cells are valid (after a boundary) or still pending
+  // from the buffer the output is written into, which then holds them
+  const bool in_pending = s->ghosts_pending;
+  const GravityArgs g = gravity_args(s);
+  const long long pl = s->plane;
+  auto shift = [pl](auto p) { return p ? p + pl : p; };
+  for (int k = 0; k < s->tab.order; ++k) {
+    if (s->d.use_gravity) {
+      const int rc = enqueue_relax(s, s->d.gauss_seidel_max_iter);
+      if (rc) return rc;
+    }
+    const int mode = s->plan[k].mode;
+    if (s->mixed) {
+      StageArgs<float, float> a{};
+      fill_stage_args<float>(s, k, a);
+      a.u0 = shift(a.u0); a.u0lo = shift(a.u0lo); a.dk = shift(a.dk); a.dnext = shift(a.dnext);
+      a.unew = shift(a.unew); a.unewlo = shift(a.unewlo); a.pkp = shift(a.pkp);
+      for (int j = 0; j < kMaxPartials; ++j) { a.fp[j] = shift(a.fp[j]); a.rep[j] = shift(a.rep[j]); }
+      launch_gravity_stage_f32(mode, a, g, (const float*)s->U[s->cur], s->kahan ? s->Ulo[s->cur] : nullptr,
+                               (float*)s->U[s->cur ^ 1], s->kahan ? s->Ulo[s->cur ^ 1] : nullptr,
+                               (const float*)s->pot[0], s->stream);
+    } else {
+      StageArgs<double, double> a{};
+      fill_stage_args<double>(s, k, a);
+      a.u0 = shift(a.u0); a.dk = shift(a.dk); a.dnext = shift(a.dnext);
+      a.unew = shift(a.unew); a.pkp = shift(a.pkp);
+      for (int j = 0; j < kMaxPartials; ++j) { a.fp[j] = shift(a.fp[j]); a.rep[j] = shift(a.rep[j]); }
+      launch_gravity_stage_f64(mode, a, g, (const double*)s->U[s->cur], (double*)s->U[s->cur ^ 1],
+                               (const double*)s->pot[0], s->stream);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  s->cur ^= 1;
+  s->ghosts_pending = !in_pending;
+  return HYDRO_OK;
+}
+
 // the momentum pass: the pressure of its start state, then one launch per
 // stage; stage k reads pres[k & 1] and leaves pres[(k + 1) & 1]
 int enqueue_burgers_momentum(hydro_sim* s, BurgersArgs b) {
@@ -728,7 +845,12 @@ int enqueue_burgers_pressure_passes(hydro_sim* s) {
   for (int i = 0; i < 3; ++i) b.inv_dx[i] = 1.0 / s->kdx[i];
   b.dt = &s->sc->dt;
   b.work_coef = s->work_coef;
+  b.pot = s->pot_on ? s->pot[0] : nullptr;
   int rc = enqueue_boundary(s);
+  // the gravity pass between the advection boundary and the momentum pass
+  // (src/Solver/EulerBurgers.cpp:80-82)
+  if (!rc && s->pot_on) rc = enqueue_gravity_pass(s);
+  if (!rc) rc = materialize_ghosts(s);  // computePressure reads the ghost layer
   if (!rc) rc = enqueue_burgers_momentum(s, b);
   if (!rc) rc = enqueue_boundary(s);
   if (!rc) rc = enqueue_burgers_work(s, b);
@@ -763,10 +885,17 @@ int enqueue_stages(hydro_sim* s) {
   s->ghosts_pending = true;  // SRHD: copied inside the con2prim launch below
   // ForwardEuler's zero-gravity pass refreshes the ghosts after the update
   // (src/Solver/SelfGravitation.cpp:122 + src/Integrator/ForwardEuler.cpp:29-31)
-  // (EulerBurgers applies the boundary next in any case)
+  // (EulerBurgers applies the boundary next in any case; with a potential
+  // the gravity pass below runs for real and keeps it itself)
   if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) && s->d.integrator == HYDRO_INTEGRATOR_FORWARDEULER &&
-      s->d.equation == HYDRO_EQN_EULER && s->d.solver != HYDRO_SOLVER_BURGERS) {
+      s->d.equation == HYDRO_EQN_EULER && s->d.solver != HYDRO_SOLVER_BURGERS && !s->pot_on) {
     int rc = enqueue_boundary(s);
+    if (rc) return rc;
+  }
+  // EulerRoe / EulerHLL(C)::step -> SelfGravitation::applyPotential with a
+  // potential (src/Solver/EulerRoe.cpp:31-34, EulerHLL.cpp:25-28)
+  if (s->pot_on && s->d.solver != HYDRO_SOLVER_BURGERS) {
+    int rc = enqueue_gravity_pass(s);
     if (rc) return rc;
   }
   // SRHDRoe::step tail: constrainState + updatePrimitives (src/Solver/SRHDRoe.cpp:62-73)
@@ -782,6 +911,7 @@ constexpr int kRun1dChunk = 256;
 
 bool run1d_eligible(const hydro_sim* s) {
   if (!s->use_run1d || s->profiling || s->dim != 1 || s->d.equation != HYDRO_EQN_EULER) return false;
+  if (s->pot_on) return false;
   if (s->d.solver == HYDRO_SOLVER_BURGERS) return false;
   if (!boundary_covers_ghosts(s) || s->tab.order > kRun1dMaxStages) return false;
   const int strips = (s->sz[0] - 4 + 59) / 60;  // 60 output columns per strip
@@ -957,6 +1087,9 @@ void hydro_sim_desc_default(hydro_sim_desc* d) {
   d->srhd_tau_max = 1e20;
   d->maxwell_permittivity = d->maxwell_permeability = d->maxwell_conductivity = 1.0;
   d->flags = HYDRO_FLAG_REFERENCE;
+  d->use_gravity = 0;
+  d->gauss_seidel_max_iter = 20;
+  d->self_grav_gravitational_constant = 1.0;
 }
 
 const char* hydro_last_error(void) { return g_err.c_str(); }
@@ -999,6 +1132,8 @@ void hydro_sim_destroy(hydro_sim* s) {
   if (s->aos_dev) hipFree(s->aos_dev);
   if (s->prim) hipFree(s->prim);
   for (void* p : s->pres)
+    if (p) hipFree(p);
+  for (void* p : s->pot)
     if (p) hipFree(p);
   if (s->blockmin) hipFree(s->blockmin);
   if (s->partial) hipFree(s->partial);
@@ -1227,6 +1362,14 @@ int hydro_sim_create(hydro_ctx* ctx, const hydro_sim_desc* d, hydro_sim** out) {
       return cleanup(HYDRO_E_HIP, "out of device memory (primitives)");
     hipMemset(s->prim, 0, prim_bytes(s));
   }
+  if (d->equation == HYDRO_EQN_EULER) {
+    // src/Solver/SelfGravitation.cpp:14-18: the potential buffer (two planes
+    // here: the relaxation sweeps alternate between them)
+    for (auto& p : s->pot) {
+      if (hipMalloc(&p, es * s->plane) != hipSuccess) return cleanup(HYDRO_E_HIP, "out of device memory (potential)");
+      hipMemset(p, 0, es * s->plane);
+    }
+  }
   if (d->solver == HYDRO_SOLVER_BURGERS) {
     // src/Solver/EulerBurgers.cpp:14-22: the pressure buffer (two planes
     // here: the momentum stages alternate between them)
@@ -1423,6 +1566,76 @@ int hydro_sim_get_storage(hydro_sim* s, double* hi, float* lo, size_t n) {
       hi[k + (size_t)N * c] = h[(size_t)k * s->plane + c];
       if (lo) lo[k + (size_t)N * c] = l[(size_t)k * s->plane + c];
     }
+  return HYDRO_OK;
+}
+
+int hydro_sim_set_potential(hydro_sim* s, const double* pot, size_t n) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  if (s->d.equation != HYDRO_EQN_EULER) return fail(HYDRO_E_UNSUPPORTED, "a potential belongs to the Euler solvers");
+  if (pot && n != (size_t)s->vol) return fail(HYDRO_E_INVALID, "potential size must be the volume");
+  bool nonzero = s->d.use_gravity != 0;
+  if (pot && !s->d.use_gravity)
+    for (size_t i = 0; i < n && !nonzero; ++i) nonzero = pot[i] != 0.0;
+  if (nonzero && s->dim == 3 && s->d.solver == HYDRO_SOLVER_HLL)
+    return fail(HYDRO_E_UNSUPPORTED, "3D HLL with a gravitational potential is not supported (2D and 1D are; HLLC "
+                                     "and Roe in 3D are)");
+  int rc = materialize_ghosts(s);  // every cell's energy gains the potential
+  if (rc) return rc;
+  PhaseScope ph(s, HYDRO_PHASE_GRAVITY);
+  // phi0: the guess -rho (src/Solver/SelfGravitation.cpp:48-52) or the given potential
+  const size_t bytes = esize(s) * (size_t)s->plane;
+  if (s->d.use_gravity) {
+    if (s->mixed) launch_copy_plane_f32(s->vol, (const float*)s->U[s->cur], (float*)s->pot[0], true, s->stream);
+    else launch_copy_plane_f64(s->vol, (const double*)s->U[s->cur], (double*)s->pot[0], true, s->stream);
+    HIP_TRY(hipGetLastError());
+  } else {
+    std::vector<double> pd((size_t)s->plane, 0.0);
+    if (pot) std::copy(pot, pot + n, pd.begin());
+    if (s->mixed) {
+      std::vector<float> pf(pd.begin(), pd.end());
+      if (copy_sync(s->pot[0], pf.data(), bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return fail(HYDRO_E_HIP, "copy (potential)");
+    } else if (copy_sync(s->pot[0], pd.data(), bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+      return fail(HYDRO_E_HIP, "copy (potential)");
+    }
+  }
+  // :89-94: the initial potential into every cell's total energy
+  const long long E = (long long)(s->dim + 1) * s->plane;
+  if (s->mixed)
+    launch_add_potential_energy_f32(s->vol, (float*)s->U[s->cur] + E, s->kahan ? s->Ulo[s->cur] + E : nullptr,
+                                    (const float*)s->pot[0], s->stream);
+  else
+    launch_add_potential_energy_f64(s->vol, (double*)s->U[s->cur] + E, (const double*)s->pot[0], s->stream);
+  HIP_TRY(hipGetLastError());
+  // :81-88: 100 x gaussSeidelMaxIter sweeps towards the steady potential
+  if (s->d.use_gravity) {
+    for (int tries = 0; tries < 100; ++tries) {
+      rc = enqueue_relax(s, s->d.gauss_seidel_max_iter);
+      if (rc) return rc;
+    }
+  }
+  s->pot_on = nonzero;
+  s->cg.pot = (nonzero && (s->d.solver == HYDRO_SOLVER_ROE || s->d.solver == HYDRO_SOLVER_HLL ||
+                           s->d.solver == HYDRO_SOLVER_BURGERS))
+                  ? s->pot[0]
+                  : nullptr;
+  drop_graphs(s);  // captured updates without (or with) the gravity pass
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return HYDRO_OK;
+}
+
+int hydro_sim_get_potential(hydro_sim* s, double* pot, size_t n) {
+  if (!s || !pot) return fail(HYDRO_E_INVALID, "null argument");
+  if (s->d.equation != HYDRO_EQN_EULER) return fail(HYDRO_E_UNSUPPORTED, "a potential belongs to the Euler solvers");
+  if (n != (size_t)s->vol) return fail(HYDRO_E_INVALID, "potential size must be the volume");
+  if (s->mixed) {
+    std::vector<float> f((size_t)s->vol);
+    if (copy_sync(f.data(), s->pot[0], sizeof(float) * f.size(), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+      return fail(HYDRO_E_HIP, "copy (potential)");
+    std::copy(f.begin(), f.end(), pot);
+  } else if (copy_sync(pot, s->pot[0], sizeof(double) * n, hipMemcpyDeviceToHost, s->stream) != hipSuccess) {
+    return fail(HYDRO_E_HIP, "copy (potential)");
+  }
   return HYDRO_OK;
 }
 
@@ -1670,6 +1883,11 @@ int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_pe
     HIP_TRY(hipMalloc((void**)&savePrim, prim_bytes(s)));
     HIP_TRY(hipMemcpyAsync(savePrim, s->prim, prim_bytes(s), hipMemcpyDeviceToDevice, s->stream));
   }
+  void* savePot = nullptr;  // the relaxation sweeps of the trials move the potential
+  if (s->pot_on && s->d.use_gravity) {
+    HIP_TRY(hipMalloc(&savePot, esize(s) * (size_t)s->plane));
+    HIP_TRY(hipMemcpyAsync(savePot, s->pot[0], esize(s) * (size_t)s->plane, hipMemcpyDeviceToDevice, s->stream));
+  }
   DevScalars sc0[2];
   HIP_TRY(hipMemcpyAsync(sc0, s->sc, sizeof sc0, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1683,6 +1901,8 @@ int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_pe
                              s->stream));
     if (savePrim)
       HIP_TRY(hipMemcpyAsync(s->prim, savePrim, prim_bytes(s), hipMemcpyDeviceToDevice, s->stream));
+    if (savePot)
+      HIP_TRY(hipMemcpyAsync(s->pot[0], savePot, esize(s) * (size_t)s->plane, hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->sc, sc0, sizeof sc0, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return HYDRO_OK;
@@ -1750,6 +1970,7 @@ int hydro_sim_autotune(hydro_sim* s, int updates, int* params_out, double* ms_pe
   hipFree(saveU);
   if (saveLo) hipFree(saveLo);
   if (savePrim) hipFree(savePrim);
+  if (savePot) hipFree(savePot);
   if (rc) return rc;
   if (rc2) return rc2;
   tune_store(tune_key(s), best);

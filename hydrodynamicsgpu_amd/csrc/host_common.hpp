@@ -336,6 +336,13 @@ inline int validate(const hydro_sim_desc* d) {
       if (d->boundary[a][m] < HYDRO_BOUNDARY_NONE || d->boundary[a][m] > HYDRO_BOUNDARY_FREEFLOW)
         return fail(HYDRO_E_INVALID, "bad boundary method");
   if (d->use_fixed_dt && !(d->fixed_dt > 0)) return fail(HYDRO_E_INVALID, "fixed_dt must be > 0");
+  // use_gravity acts on the Euler solvers only (the reference's
+  // SelfGravitationBehavior solvers); the others ignore it, as there
+  if (d->use_gravity && d->equation == HYDRO_EQN_EULER) {
+    if (d->dim == 3 && d->solver == HYDRO_SOLVER_HLL)
+      return fail(HYDRO_E_UNSUPPORTED, "3D HLL with self-gravity is not supported (2D and 1D are; HLLC and Roe in 3D are)");
+    if (d->gauss_seidel_max_iter < 0) return fail(HYDRO_E_INVALID, "gauss_seidel_max_iter must be >= 0");
+  }
   return HYDRO_OK;
 }
 

@@ -65,6 +65,7 @@ struct DtArgs {
   const void* prim;      // SRHD primitive planes
   int prim_f32;          // ... in FP32 (else FP64)
   double maxwell_lambda; // Maxwell: 1/sqrt(eps mu)
+  const void* pot;       // Euler Roe / HLL / EulerBurgers: potential plane (state precision), or null
   int nblocks;
   double* blockmin;      // [nblocks]
 };
@@ -182,6 +183,7 @@ struct BurgersArgs {
   double inv_dx[3];   // 1 / kernel dx per axis
   const double* dt;   // device-resident step size
   double work_coef;   // what the tableau's rows sum a constant derivative to
+  const void* pot;    // the potential plane (state precision), or null
   // momentum pass: its start state and the new state (unshifted planes;
   // low words of compensated storage, or null)
   const void* u0;
@@ -206,6 +208,37 @@ void launch_burgers_momentum_f32(int mode, bool kahan, const StageArgs<float, fl
 // pressures p of the last momentum stage state
 void launch_burgers_work_f64(const BurgersArgs& b, double* u, const double* p, hipStream_t st);
 void launch_burgers_work_f32(const BurgersArgs& b, float* u, float* lo, const float* p, hipStream_t st);
+
+// self-gravity (gravity.hip)
+struct GravityArgs {
+  int dim, sx, sy, sz;
+  long long plane;
+  double kdx[3];            // kernel dx per axis
+  double diag;              // -2 sum_a 1 / kdx_a^2
+  double four_pi_g;         // 4 pi G
+  double inv_2dx[3];        // 1 / (2 kdx_a)
+  signed char method[3][2]; // the potential's boundary per side: 0 periodic, 2 free-flow, -1 none
+  const double* dt;         // device-resident step size
+};
+// one relaxation sweep phi -> out (every cell: interior relaxed, ghosts by
+// the potential's boundary from phi); rho is the density plane
+void launch_potential_relax_f64(const GravityArgs& g, const double* rho, const double* phi, double* out,
+                                hipStream_t st);
+void launch_potential_relax_f32(const GravityArgs& g, const float* rho, const float* phi, float* out,
+                                hipStream_t st);
+// one RK stage of the gravity source pass: a advanced by one plane (momentum
+// and energy), u0 / unew the pass's start and new state (unshifted; low
+// words of compensated storage or null), phi the potential plane
+void launch_gravity_stage_f64(int mode, const StageArgs<double, double>& a, const GravityArgs& g, const double* u0,
+                              double* unew, const double* phi, hipStream_t st);
+void launch_gravity_stage_f32(int mode, const StageArgs<float, float>& a, const GravityArgs& g, const float* u0,
+                              const float* u0lo, float* unew, float* unewlo, const float* phi, hipStream_t st);
+// E += phi on every cell (lo: compensated low words, or null)
+void launch_add_potential_energy_f64(long long vol, double* E, const double* phi, hipStream_t st);
+void launch_add_potential_energy_f32(long long vol, float* E, float* lo, const float* phi, hipStream_t st);
+// dst = src (or -src) over n elements
+void launch_copy_plane_f64(long long n, const double* src, double* dst, bool negate, hipStream_t st);
+void launch_copy_plane_f32(long long n, const float* src, float* dst, bool negate, hipStream_t st);
 
 // buffer operations of host-side integrators: a = b + c * k over n
 // elements (multAdd), and a zero fill of n 32-bit words

@@ -72,6 +72,8 @@ int hydro_calc_timestep_noalloc(const hydro_sim_desc* d, const void* state, size
                                 double t, double* dt_out, unsigned* flags_out, void* stream) {
   if (int rc = validate(d)) return rc;
   if (!state || !dt_out) return fail(HYDRO_E_INVALID, "null argument");
+  if (d->use_gravity && d->equation == HYDRO_EQN_EULER)
+    return fail(HYDRO_E_UNSUPPORTED, "the allocation-free timestep does not read a gravitational potential");
   const long long vol = (long long)d->size[0] * d->size[1] * d->size[2];
   if ((long long)plane < vol) return fail(HYDRO_E_INVALID, "plane stride must be at least the cell count");
   if (d->equation == HYDRO_EQN_SRHD && !prims && !d->use_fixed_dt)

@@ -15,7 +15,10 @@
 
 namespace hydro {
 
-template <int DIM, typename T>
+// POT: the cells carry a gravitational potential phi (res/EulerRoe.cl:56-94):
+// the pressure subtracts it from the specific energy and the face's sound
+// speed subtracts its Roe average; without POT phi is zero and never read.
+template <int DIM, typename T, bool POT = false>
 struct EulerRoe {
   static constexpr int N = DIM + 2;
   static constexpr int E = DIM + 1;
@@ -26,6 +29,7 @@ struct EulerRoe {
     T w;        // sqrt(rho)
     T v[DIM];   // velocity
     T H;        // total specific enthalpy
+    T phi;      // potential (POT only)
   };
 
   // frozen per-face data, normal rotated into component 0 of v
@@ -36,8 +40,9 @@ struct EulerRoe {
   };
 
   template <typename S>
-  static HYDRO_DEV Cell cell(const S q[N], T gamma) {
+  static HYDRO_DEV Cell cell(const S q[N], T gamma, T phi = T(0)) {
     Cell c;
+    c.phi = phi;
     const T rho = T(q[0]);
     T inv, vsq = T(0);
     if constexpr (kFast) {
@@ -58,9 +63,10 @@ struct EulerRoe {
         vsq += c.v[a] * c.v[a];
       }
     }
-    // H = E/rho + p/rho with p = (gamma-1) rho (E/rho - v^2/2)
+    // H = E/rho + p/rho with p = (gamma-1) rho (E/rho - v^2/2 - phi)
     const T eT = T(q[E]) * inv;
-    c.H = eT + (gamma - T(1)) * (eT - T(.5) * vsq);
+    if constexpr (POT) c.H = eT + (gamma - T(1)) * (eT - T(.5) * vsq - phi);
+    else c.H = eT + (gamma - T(1)) * (eT - T(.5) * vsq);
     return c;
   }
 
@@ -76,7 +82,9 @@ struct EulerRoe {
     }
     f.H = (L.w * L.H + R.w * R.H) * norm;
     f.q2 = q2;
-    const T c2 = (f.H - T(.5) * q2) * (gamma - T(1));
+    T c2;
+    if constexpr (POT) c2 = (f.H - T(.5) * q2 - (L.w * L.phi + R.w * R.phi) * norm) * (gamma - T(1));
+    else c2 = (f.H - T(.5) * q2) * (gamma - T(1));
     if constexpr (kFast) {
       const T rs = __builtin_amdgcn_rsqf(c2);
       f.c = c2 * rs;

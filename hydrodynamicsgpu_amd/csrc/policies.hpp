@@ -9,10 +9,39 @@
 //           (HLL/HLLC, first order: res/EulerHLL.cl:317-335 keeps the
 //           limiter disabled).
 #pragma once
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "phys_euler.hpp"
 
 namespace hydro {
+
+// Policies whose faces read a gravitational potential declare
+// kPotential = true and take the cell's potential in frozen_pot and
+// stage_pot; the kernels load one potential plane for them.
+template <class P, class = void>
+struct PolicyPotential : std::false_type {};
+template <class P>
+struct PolicyPotential<P, std::void_t<decltype(P::kPotential)>> : std::bool_constant<P::kPotential> {};
+
+template <class Pol, typename S, typename T>
+HYDRO_DEV typename Pol::Frozen frozen_of(const S* q0, S pot, T gamma) {
+  if constexpr (PolicyPotential<Pol>::value) {
+    return Pol::template frozen_pot<S>(q0, T(pot), gamma);
+  } else {
+    (void)pot;
+    return Pol::template frozen<S>(q0, gamma);
+  }
+}
+template <class Pol, typename T, typename S>
+HYDRO_DEV typename Pol::Stage stage_of(const T* u, S pot, T gamma) {
+  if constexpr (PolicyPotential<Pol>::value) {
+    return Pol::stage_pot(u, T(pot), gamma);
+  } else {
+    (void)pot;
+    return Pol::stage(u, gamma);
+  }
+}
 
 // Per-cell stage data a policy's face_a reads, built once per cell by
 // stage() and moved between lanes as NS values (pack_stage/unpack_stage).
@@ -88,15 +117,16 @@ HYDRO_DEV T wave_flux(int lim, T l, T lq2, T d, T dqlo, T dqhi, T dt_dx) {
 }
 
 // ------------------------------------------------------------------ Roe
-template <int DIM, typename T, int LIM>
+template <int DIM, typename T, int LIM, bool POT = false>
 struct RoePolicy : StageIsState<DIM + 2, T> {
-  using P = EulerRoe<DIM, T>;
+  using P = EulerRoe<DIM, T, POT>;
   static constexpr int N = P::N;
   using Stage = typename StageIsState<N, T>::Stage;
   static constexpr bool kNeighbors = true;
   static constexpr bool kFrozenPrims = false;
+  static constexpr bool kPotential = POT;
   using Frozen = typename P::Cell;
-  static constexpr int NF = DIM + 2;  // w, v[DIM], H
+  static constexpr int NF = DIM + 2 + (POT ? 1 : 0);  // w, v[DIM], H (, phi)
 
   struct FaceA {
     typename P::Face f;
@@ -108,11 +138,17 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
   static HYDRO_DEV Frozen frozen(const S q0[N], T gamma) {
     return P::template cell<S>(q0, gamma);
   }
+  template <typename S>
+  static HYDRO_DEV Frozen frozen_pot(const S q0[N], T phi, T gamma) {
+    return P::template cell<S>(q0, gamma, phi);
+  }
+  static HYDRO_DEV Stage stage_pot(const T u[N], T, T gamma) { return StageIsState<N, T>::stage(u, gamma); }
   static HYDRO_DEV void pack(const Frozen& c, T o[NF]) {
     o[0] = c.w;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) o[1 + d] = c.v[d];
     o[DIM + 1] = c.H;
+    if constexpr (POT) o[DIM + 2] = c.phi;
   }
   static HYDRO_DEV Frozen unpack(const T o[NF]) {
     Frozen c;
@@ -120,6 +156,8 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) c.v[d] = o[1 + d];
     c.H = o[DIM + 1];
+    if constexpr (POT) c.phi = o[DIM + 2];
+    else c.phi = T(0);
     return c;
   }
 
@@ -159,17 +197,23 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
 };
 
 // ------------------------------------------------------------------ HLL / HLLC
-template <int DIM, typename T, int SOLVER>  // SOLVER: 1 HLL, 2 HLLC
+// POT (HLL only): the cells' potentials enter the pressures and the
+// Roe-averaged sound speed (res/EulerHLL.cl:56-93, 226-240); HLLC reads none
+// (res/EulerHLLC.cl).
+template <int DIM, typename T, int SOLVER, bool POT = false>  // SOLVER: 1 HLL, 2 HLLC
 struct HLLPolicy {
+  static_assert(!POT || SOLVER == 1, "only HLL reads the potential");
   static constexpr int N = DIM + 2;
   static constexpr int E = DIM + 1;
   static constexpr bool kNeighbors = false;
   static constexpr bool kFrozenPrims = false;
+  static constexpr bool kPotential = POT;
   struct Frozen {
     T rho, p, cs, w, H;
     T v[DIM];
+    T phi;  // POT only
   };
-  static constexpr int NF = DIM + 5;
+  static constexpr int NF = DIM + 5 + (POT ? 1 : 0);
   struct FaceA {
     T F[N];
     T dq[N];  // unused (no limiter); kept so the kernels stay generic
@@ -183,7 +227,8 @@ struct HLLPolicy {
   };
   static constexpr int NS = N + DIM + 2;
 
-  static HYDRO_DEV Stage stage(const T u[N], T gamma) {
+  static HYDRO_DEV Stage stage(const T u[N], T gamma) { return stage_pot(u, T(0), gamma); }
+  static HYDRO_DEV Stage stage_pot(const T u[N], T phi, T gamma) {
     Stage c;
     const T d = u[0];
     const T inv = trcp(d);
@@ -196,7 +241,8 @@ struct HLLPolicy {
       vsq += c.v[k] * c.v[k];
     }
     const T eT = u[E] * inv;
-    c.p = (gamma - T(1)) * d * (eT - T(.5) * vsq);
+    if constexpr (POT) c.p = (gamma - T(1)) * d * (eT - T(.5) * vsq - phi);
+    else c.p = (gamma - T(1)) * d * (eT - T(.5) * vsq);
     c.h = eT + c.p * inv;
     return c;
   }
@@ -221,7 +267,12 @@ struct HLLPolicy {
 
   template <typename S>
   static HYDRO_DEV Frozen frozen(const S q0[N], T gamma) {
+    return frozen_pot<S>(q0, T(0), gamma);
+  }
+  template <typename S>
+  static HYDRO_DEV Frozen frozen_pot(const S q0[N], T phi, T gamma) {
     Frozen c;
+    c.phi = phi;
     c.rho = T(q0[0]);
     const T inv = trcp(c.rho);
     T vsq = T(0);
@@ -231,7 +282,8 @@ struct HLLPolicy {
       vsq += c.v[d] * c.v[d];
     }
     const T eT = T(q0[E]) * inv;
-    c.p = (gamma - T(1)) * c.rho * (eT - T(.5) * vsq);
+    if constexpr (POT) c.p = (gamma - T(1)) * c.rho * (eT - T(.5) * vsq - phi);
+    else c.p = (gamma - T(1)) * c.rho * (eT - T(.5) * vsq);
     c.H = eT + c.p * inv;
     c.cs = tsqrt((gamma - T(1)) * (c.H - T(.5) * vsq));
     c.w = SOLVER == 1 ? tsqrt(c.rho) : T(0);  // only HLL's Roe averages weigh by sqrt(rho)
@@ -262,12 +314,15 @@ struct HLLPolicy {
     o[0] = c.rho; o[1] = c.p; o[2] = c.cs; o[3] = c.w; o[4] = c.H;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) o[5 + d] = c.v[d];
+    if constexpr (POT) o[5 + DIM] = c.phi;
   }
   static HYDRO_DEV Frozen unpack(const T o[NF]) {
     Frozen c;
     c.rho = o[0]; c.p = o[1]; c.cs = o[2]; c.w = o[3]; c.H = o[4];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) c.v[d] = o[5 + d];
+    if constexpr (POT) c.phi = o[5 + DIM];
+    else c.phi = T(0);
     return c;
   }
 
@@ -285,7 +340,9 @@ struct HLLPolicy {
         if (d == side) vn = v;
       }
       const T H = (L.w * L.H + R.w * R.H) * norm;
-      const T c = tsqrt((gamma - T(1)) * (H - T(.5) * vsq));
+      T c;
+      if constexpr (POT) c = tsqrt((gamma - T(1)) * (H - T(.5) * vsq - (L.w * L.phi + R.w * R.phi) * norm));
+      else c = tsqrt((gamma - T(1)) * (H - T(.5) * vsq));
       sl = tmin(vLn - L.cs, vn - c);
       sr = tmax(vRn + R.cs, vn + c);
     } else {

@@ -67,7 +67,9 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
     S q0[N];
     T dk[N];
     S pr[Pol::kFrozenPrims ? N : 1];
+    S pot;  // the cell's potential (policies that read one), else 0
   };
+  constexpr bool kPot = PolicyPotential<Pol>::value;
 
   auto fetch = [&](int row, Raw& w) {
     const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;  // wave-uniform
@@ -77,13 +79,15 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
 #pragma unroll
       for (int s = 0; s < N; ++s) w.pr[s] = ldp(a.prim, pl, s, e, cl);
     }
+    if constexpr (kPot) w.pot = ldp(a.pot, pl, 0, e, cl);
+    else w.pot = S(0);
   };
   auto build = [&](const Raw& w, Row& R) {
     T u[N];
     comb.stage_state(w.q0, w.dk, u);
-    R.st = Pol::stage(u, gamma);
+    R.st = stage_of<Pol>(u, w.pot, gamma);
     if constexpr (Pol::kFrozenPrims) R.fz = Pol::template frozen<S>(w.pr, gamma);
-    else R.fz = Pol::template frozen<S>(w.q0, gamma);
+    else R.fz = frozen_of<Pol>(w.q0, w.pot, gamma);
   };
   auto load_row = [&](int row, Row& R) {
     Raw w;
@@ -268,7 +272,11 @@ static void launch_pol(int mode, const StageArgs<T, S>& a, int wpb, hipStream_t 
 template <int DIM, typename T, typename S, bool KAHAN>
 static void launch_solver(int kind, int lim, int mode, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
   constexpr bool M = DIM == 2;
-  if (kind == 5) {
+  if (a.pot && (kind == 0 || kind == 1)) {
+    // a non-zero gravitational potential: the runtime-limiter Roe form, or HLL
+    if (kind == 0) launch_pol<RoePolicy<DIM, T, LIM_RUNTIME, true>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+    else launch_pol<HLLPolicy<DIM, T, 1, true>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+  } else if (kind == 5) {
     if (lim == LIM_SUPERBEE) launch_pol<BurgersPolicy<DIM, T, LIM_SUPERBEE>, DIM, T, S, KAHAN, M>(mode, a, wpb, st);
     else launch_pol<BurgersPolicy<DIM, T, LIM_RUNTIME>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
   } else if (kind == 1) {

@@ -72,14 +72,16 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
     comb.load_u0(e, cl, q0);
     comb.load_dk(e, cl, dk);
     comb.stage_state(q0, dk, u);
-    st = Pol::stage(u, gamma);
+    S pot = S(0);
+    if constexpr (PolicyPotential<Pol>::value) pot = ldp(a.pot, pl, 0, e, cl);
+    st = stage_of<Pol>(u, pot, gamma);
     if constexpr (Pol::kFrozenPrims) {
       S pr[N];
 #pragma unroll
       for (int s = 0; s < N; ++s) pr[s] = ldp(a.prim, pl, s, e, cl);
       fz = Pol::template frozen<S>(pr, gamma);
     } else {
-      fz = Pol::template frozen<S>(q0, gamma);
+      fz = frozen_of<Pol>(q0, pot, gamma);
     }
   };
 
@@ -898,6 +900,8 @@ static void launch3_solver(int kind, int lim, int mode, const StageArgs<T, S>& a
     launch3<MaxwellRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
   } else if (kind == 5) {
     launch3<BurgersPolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
+  } else if (kind == 0 && a.pot) {
+    launch3<RoePolicy<3, T, LIM_RUNTIME, true>, T, S, KAHAN, W>(mode, a, st);
   } else {
     switch (lim) {
     case LIM_SUPERBEE: launch3<RoePolicy<3, T, LIM_SUPERBEE>, T, S, KAHAN, W>(mode, a, st); break;

@@ -50,7 +50,8 @@ struct HydroGPUApp {
   int boundaryMethods[3][2] = {{BOUNDARY_METHOD_FREEFLOW, BOUNDARY_METHOD_FREEFLOW},
                                {BOUNDARY_METHOD_FREEFLOW, BOUNDARY_METHOD_FREEFLOW},
                                {BOUNDARY_METHOD_FREEFLOW, BOUNDARY_METHOD_FREEFLOW}};
-  bool useGravity = false;  // self-gravity Poisson relaxation is not part of this build
+  bool useGravity = false;      // relax the potential from the density (src/HydroGPUApp.cpp:77, 207)
+  int gaussSeidelMaxIter = 20;  // relaxation sweeps per potential solve (src/HydroGPUApp.cpp:78, 208)
   // the `defs` table (res/config.lua:70-93): idealGas_heatCapacityRatio,
   // srhd_*, maxwell_*; missing keys take the reference defaults
   std::map<std::string, real> defs;

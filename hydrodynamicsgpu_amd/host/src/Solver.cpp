@@ -137,7 +137,6 @@ Solver::~Solver() {
 }
 
 void Solver::init() {
-  if (app->useGravity) throw Exception() << "self-gravity is not part of this build (useGravity must be false)";
   createEquation();
   if (!equation) throw Exception() << name() << " has no equation";
 
@@ -195,6 +194,10 @@ void Solver::init() {
   d.maxwell_permeability = def(app, "maxwell_permeability", d.maxwell_permeability);
   d.maxwell_conductivity = def(app, "maxwell_conductivity", d.maxwell_conductivity);
   d.flags = app->flags;
+  d.use_gravity = app->useGravity ? 1 : 0;
+  d.gauss_seidel_max_iter = app->gaussSeidelMaxIter;
+  d.self_grav_gravitational_constant =
+      def(app, "selfGrav_gravitationalConstant", d.self_grav_gravitational_constant);
 
   if (!ctx) checkStatus(hydro_ctx_create(app->device, &ctx), "hydro_ctx_create");
   if (sim) {
@@ -237,6 +240,8 @@ void Solver::resetState() {
   if (!sim) throw Exception() << "resetState before init";
   const int n = numStates();
   std::vector<real> stateVec((size_t)getVolume() * n);
+  std::vector<real> potentialVec((size_t)getVolume(), 0.0);
+  bool anyPotential = false;
   std::vector<real> cellResults;
   const int nch = equation->numReadStateChannels();
   int flattenedIndex = 0;
@@ -251,10 +256,21 @@ void Solver::resetState() {
         if ((int)cellResults.size() < nch)
           throw Exception() << "initState returned " << cellResults.size() << " values, expected " << nch;
         equation->readStateCell(stateVec.data() + (size_t)flattenedIndex * n, cellResults.data());
+        // SelfGravitationBehavior::Converter::setValues: the potential is the
+        // next-to-last channel (include/HydroGPU/Solver/SelfGravitationBehavior.h:64-68)
+        if ((int)cellResults.size() >= nch + 2) {
+          potentialVec[flattenedIndex] = cellResults[cellResults.size() - 2];
+          anyPotential = anyPotential || potentialVec[flattenedIndex] != 0.0;
+        }
       }
     }
   }
   setState(stateVec);
+  // the Euler solvers' SelfGravitation::resetState: the potential (or the
+  // relaxed guess with useGravity) and its energy
+  if (desc_.equation == HYDRO_EQN_EULER && (anyPotential || desc_.use_gravity))
+    checkStatus(hydro_sim_set_potential(sim, anyPotential ? potentialVec.data() : nullptr, potentialVec.size()),
+                "resetState");
 }
 
 std::vector<real> Solver::getState() {

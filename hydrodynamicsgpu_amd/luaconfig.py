@@ -697,7 +697,8 @@ def load(path_or_text, precision="fp64", flags=_config.FLAG_REFERENCE, t_end=0.0
     (src/HydroGPUApp.cpp:164-230; the limiter key is `slopeLimiter`,
     src/Solver/Solver.cpp:285-287, default Superbee) and info holds
     initCondName, the initial-state function (initconds) and the keys this
-    build does not act on (maxFrames, useGravity, ...)."""
+    build does not act on (maxFrames, ...); useGravity and
+    gaussSeidelMaxIter go into the descriptor."""
     from . import initconds
 
     text = path_or_text
@@ -731,7 +732,9 @@ def load(path_or_text, precision="fp64", flags=_config.FLAG_REFERENCE, t_end=0.0
     desc = _config.make_desc(size, xmin=xmin, xmax=xmax, solverName=solver, integratorName=integ,
                              slopeLimiter=lim, boundaryMethods=bms, cfl=cfl,
                              useFixedDT=truthy(g.get("useFixedDT")), fixedDT=float(_get(g, "fixedDT", 0.001)),
-                             defs=defs, precision=precision, flags=flags, t_end=t_end)
+                             defs=defs, precision=precision, flags=flags, t_end=t_end,
+                             useGravity=truthy(g.get("useGravity")),
+                             gaussSeidelMaxIter=int(_get(g, "gaussSeidelMaxIter", 20)))
     names = to_py(g.get("__initCondNames")) or []
     name = names[-1] if names else None
     info = {

@@ -143,6 +143,12 @@ typedef struct hydro_sim_desc {
   double srhd_D_min, srhd_D_max, srhd_tau_min, srhd_tau_max;
   double maxwell_permittivity, maxwell_permeability, maxwell_conductivity;
   unsigned flags;        /* HYDRO_FLAG_* */
+  /* self-gravity (Euler solvers: src/Solver/SelfGravitation.cpp,
+   * res/SelfGravitation.cl; useGravity, gaussSeidelMaxIter and the def
+   * selfGrav_gravitationalConstant of res/config.lua:65-74) */
+  int use_gravity;                         /* relax the potential from the density */
+  int gauss_seidel_max_iter;               /* relaxation sweeps per potential solve */
+  double self_grav_gravitational_constant; /* G */
 } hydro_sim_desc;
 
 typedef struct hydro_ctx hydro_ctx;
@@ -185,6 +191,26 @@ int hydro_sim_set_state(hydro_sim* sim, const double* aos, size_t n);
 
 /* Replaces Solver::Converter::fromGPU (src/Solver/Solver.cpp:339-343). */
 int hydro_sim_get_state(hydro_sim* sim, double* aos, size_t n);
+
+/* The self-gravity half of the Euler solvers' resetState
+ * (SelfGravitationBehavior::Converter::toGPU -> SelfGravitation::resetState,
+ * src/Solver/SelfGravitation.cpp:35-97), after hydro_sim_set_state: the
+ * potential phi0 = `pot` (one value per cell, NULL: zero), or the guess
+ * -rho when use_gravity is set, in which case 100 x gauss_seidel_max_iter
+ * relaxation sweeps of the Poisson equation follow; then every cell's total
+ * energy gains phi0 (the initial potential, as the reference adds it).  With
+ * a non-zero potential or use_gravity, each update adds the reference's
+ * gravity pass (SelfGravitation::applyPotential, :99-124: per stage
+ * gauss_seidel_max_iter sweeps when use_gravity, then the gravity source
+ * res/SelfGravitation.cl:65-102), and the Roe, HLL and EulerBurgers kernels
+ * read the potential (HLLC's do not, as in res/EulerHLLC.cl).  The
+ * reference relaxes in place in whatever order work-items run; here each
+ * sweep reads the previous sweep everywhere (Jacobi order), the
+ * order-independent form of the same update.  Euler solvers only; not for
+ * 3D HLL, batches or the allocation-free entry point. */
+int hydro_sim_set_potential(hydro_sim* sim, const double* pot, size_t n);
+/* The potential (one value per cell, ghost cells included). */
+int hydro_sim_get_potential(hydro_sim* sim, double* pot, size_t n);
 
 /* SRHD primitives (rho, v, eInt) in the reference layout (src/Solver/SRHDRoe.cpp:9-13).
  * They are stored in the state's precision: FP64, or FP32 for the FP32-mixed
@@ -332,12 +358,14 @@ int hydro_sim_stage_timing(hydro_sim* sim, double* avg_ms, long long* launches);
  *   PRESSURE  EulerBurgers pressure, momentum-stage and work launches
  *   GHOSTS    deferred ghost copies outside a boundary
  *   TOTALS    the conserved-totals reduction
+ *   GRAVITY   the gravity source pass and the potential's relaxation sweeps
  * With profiling enabled each phase is also a roctx range
  * ("hydro:<phase>") when the ROCm tracer library can be loaded, so that
  * rocprofv3 --marker-trace groups kernels by phase. */
 enum {
   HYDRO_PHASE_BOUNDARY = 0, HYDRO_PHASE_CFL, HYDRO_PHASE_HEAD, HYDRO_PHASE_DT_TAIL, HYDRO_PHASE_STAGES,
-  HYDRO_PHASE_CON2PRIM, HYDRO_PHASE_PRESSURE, HYDRO_PHASE_GHOSTS, HYDRO_PHASE_TOTALS, HYDRO_NUM_PHASES
+  HYDRO_PHASE_CON2PRIM, HYDRO_PHASE_PRESSURE, HYDRO_PHASE_GHOSTS, HYDRO_PHASE_TOTALS, HYDRO_PHASE_GRAVITY,
+  HYDRO_NUM_PHASES
 };
 /* Total device time in ms (total_ms[HYDRO_NUM_PHASES]) and the number of
  * timed intervals (launches[HYDRO_NUM_PHASES], may be NULL) per phase since

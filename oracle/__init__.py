@@ -39,7 +39,8 @@ def lib():
         L.oracle_num_states.argtypes = [vp]
         L.oracle_volume.argtypes = [vp]
         L.oracle_volume.restype = sz
-        for n in ("oracle_set_state", "oracle_get_state", "oracle_get_primitives", "oracle_set_potential"):
+        for n in ("oracle_set_state", "oracle_get_state", "oracle_get_primitives", "oracle_set_potential",
+                  "oracle_get_potential"):
             getattr(L, n).argtypes = [vp, dp, sz]
             getattr(L, n).restype = ip
         L.oracle_boundary.argtypes = [vp]
@@ -63,6 +64,11 @@ def lib():
         L.oracle_euler_hll_speeds.argtypes = [ip, ip, ip, ctypes.c_double, dp, dp, dp]
         L.oracle_euler_hll_flux.argtypes = [ip, ip, ip, ctypes.c_double, dp, dp, dp, dp]
         L.oracle_euler_hll_flux.restype = ip
+        L.oracle_euler_hll_speeds_pot.argtypes = [ip, ip, ip, ctypes.c_double, dp, dp, ctypes.c_double,
+                                                  ctypes.c_double, dp]
+        L.oracle_euler_hll_flux_pot.argtypes = [ip, ip, ip, ctypes.c_double, dp, dp, ctypes.c_double,
+                                                ctypes.c_double, dp, dp]
+        L.oracle_euler_hll_flux_pot.restype = ip
         L.oracle_srhd_con2prim.argtypes = [vp, ip, dp, dp]
         L.oracle_srhd_prim2cons.argtypes = [ip, ctypes.c_double, dp, dp, dp]
         _lib = L
@@ -104,10 +110,22 @@ class OracleSim:
         if lib().oracle_set_state(self._h, _dp(a), a.size) != 0:
             raise ValueError("bad state size")
 
-    def set_potential(self, pot):
-        a = f64(pot)
-        if lib().oracle_set_potential(self._h, _dp(a), a.size) != 0:
+    def set_potential(self, pot=None):
+        """SelfGravitation::resetState after set_state: the initial potential
+        (None: zero; ignored with useGravity, whose guess is -rho and which
+        relaxes it), then that initial potential added into the energy."""
+        if pot is None:
+            rc = lib().oracle_set_potential(self._h, None, 0)
+        else:
+            a = f64(pot)
+            rc = lib().oracle_set_potential(self._h, _dp(a), a.size)
+        if rc != 0:
             raise ValueError("bad potential")
+
+    def get_potential(self):
+        out = np.empty(self.volume)
+        lib().oracle_get_potential(self._h, _dp(out), out.size)
+        return out
 
     def get_state(self):
         out = np.empty(self.N * self.volume)
@@ -183,16 +201,16 @@ def srhd_eigenbasis(dim, side, gamma, primL, primR):
     return ev, L.reshape(N, N).T.copy(), R.reshape(N, N).T.copy()
 
 
-def hll_speeds(solver, dim, side, gamma, qL, qR):
+def hll_speeds(solver, dim, side, gamma, qL, qR, potL=0.0, potR=0.0):
     ev = np.zeros(dim + 2)
-    lib().oracle_euler_hll_speeds(solver, dim, side, gamma, _dp(f64(qL)), _dp(f64(qR)), _dp(ev))
+    lib().oracle_euler_hll_speeds_pot(solver, dim, side, gamma, _dp(f64(qL)), _dp(f64(qR)), potL, potR, _dp(ev))
     return ev
 
 
-def hll_flux(solver, dim, side, gamma, qL, qR, ev):
+def hll_flux(solver, dim, side, gamma, qL, qR, ev, potL=0.0, potR=0.0):
     f = np.full(dim + 2, np.nan)
-    ok = lib().oracle_euler_hll_flux(solver, dim, side, gamma, _dp(f64(qL)), _dp(f64(qR)),
-                                     _dp(f64(ev)), _dp(f))
+    ok = lib().oracle_euler_hll_flux_pot(solver, dim, side, gamma, _dp(f64(qL)), _dp(f64(qR)), potL, potR,
+                                         _dp(f64(ev)), _dp(f))
     return f if ok else None
 
 

@@ -46,6 +46,8 @@ struct oracle_sim {
   double* flux;
   double* dtbuf;
   double* potential;
+  double* pot_next;     /* the relaxation's second buffer */
+  int pot_on;           /* a non-zero potential or use_gravity: the gravity pass runs */
   double* prim;         /* SRHD primitives */
   double* ifvel;        /* EulerBurgers: interface velocities, one per side and cell */
   double* pressure;     /* EulerBurgers: computePressure's buffer */
@@ -490,16 +492,18 @@ static void rotate_in(int dim, int side, const double* src, double* dst) {
   double t = dst[1]; dst[1] = dst[1 + side]; dst[1 + side] = t;
 }
 
-/* res/EulerHLL.cl:17-115 and res/EulerHLLC.cl:15-108 (potential zero here) */
-void oracle_euler_hll_speeds(int solver, int dim, int side, double gamma, const double* srcL,
-                             const double* srcR, double* eigval) {
+/* res/EulerHLL.cl:17-115 (the cells' potentials potL, potR) and
+ * res/EulerHLLC.cl:15-108 (no potential: pass zeros) */
+void oracle_euler_hll_speeds_pot(int solver, int dim, int side, double gamma, const double* srcL,
+                                 const double* srcR, double potL, double potR, double* eigval) {
   const int E = dim + 1;
+  if (solver != HYDRO_SOLVER_HLL) potL = potR = 0.; /* res/EulerHLLC.cl reads no potential */
   double qL[5] = {0}, qR[5] = {0};
   rotate_in(dim, side, srcL, qL);
   rotate_in(dim, side, srcR, qR);
   double vL[4], vR[4], vSqL, vSqR, eTL, eTR, pL, pR, hL, hR;
-  hll_prims(dim, gamma, qL, 0., vL, &vSqL, &eTL, &pL, &hL);
-  hll_prims(dim, gamma, qR, 0., vR, &vSqR, &eTR, &pR, &hR);
+  hll_prims(dim, gamma, qL, potL, vL, &vSqL, &eTL, &pL, &hL);
+  hll_prims(dim, gamma, qR, potR, vR, &vSqR, &eTR, &pR, &hR);
   double speedOfSoundL = sqrt((gamma - 1.) * (hL - .5 * vSqL));
   double speedOfSoundR = sqrt((gamma - 1.) * (hR - .5 * vSqR));
   double roeWeightL = sqrt(qL[0]);
@@ -511,7 +515,7 @@ void oracle_euler_hll_speeds(int solver, int dim, int side, double gamma, const 
   double sl, sr;
   if (solver == HYDRO_SOLVER_HLL) {
     double enthalpyTotal = (roeWeightL * hL + roeWeightR * hR) * roeWeightNormalization;
-    double energyPotential = 0.;
+    double energyPotential = (roeWeightL * potL + roeWeightR * potR) * roeWeightNormalization;
     double velocitySq = dot4(velocity, velocity);
     double speedOfSound = sqrt((gamma - 1.) * (enthalpyTotal - .5 * velocitySq - energyPotential));
     double eigenvaluesMinL = vL[0] - speedOfSoundL;
@@ -532,16 +536,23 @@ void oracle_euler_hll_speeds(int solver, int dim, int side, double gamma, const 
   eigval[E] = sr;
 }
 
-/* res/EulerHLL.cl:185-343 and res/EulerHLLC.cl:178-419 (HLLC_METHOD 2) */
-int oracle_euler_hll_flux(int solver, int dim, int side, double gamma, const double* srcL,
-                          const double* srcR, const double* eigval, double* flux) {
+void oracle_euler_hll_speeds(int solver, int dim, int side, double gamma, const double* srcL,
+                             const double* srcR, double* eigval) {
+  oracle_euler_hll_speeds_pot(solver, dim, side, gamma, srcL, srcR, 0., 0., eigval);
+}
+
+/* res/EulerHLL.cl:185-343 (the cells' potentials in the pressures,
+ * :226-240) and res/EulerHLLC.cl:178-419 (HLLC_METHOD 2, no potential) */
+int oracle_euler_hll_flux_pot(int solver, int dim, int side, double gamma, const double* srcL,
+                              const double* srcR, double potL, double potR, const double* eigval, double* flux) {
   const int N = dim + 2, E = dim + 1;
+  if (solver != HYDRO_SOLVER_HLL) potL = potR = 0.; /* res/EulerHLLC.cl reads no potential */
   double qL[5] = {0}, qR[5] = {0};
   rotate_in(dim, side, srcL, qL);
   rotate_in(dim, side, srcR, qR);
   double vL[4], vR[4], vSqL, vSqR, eTL, eTR, pL, pR, hL, hR;
-  hll_prims(dim, gamma, qL, 0., vL, &vSqL, &eTL, &pL, &hL);
-  hll_prims(dim, gamma, qR, 0., vR, &vSqR, &eTR, &pR, &hR);
+  hll_prims(dim, gamma, qL, potL, vL, &vSqL, &eTL, &pL, &hL);
+  hll_prims(dim, gamma, qR, potR, vR, &vSqR, &eTR, &pR, &hR);
   double dL = qL[0], dR = qR[0];
   double sl = eigval[0];
   double sr = (solver == HYDRO_SOLVER_HLL) ? eigval[N - 1] : eigval[E];
@@ -594,6 +605,11 @@ int oracle_euler_hll_flux(int solver, int dim, int side, double gamma, const dou
   double t = f[1]; f[1] = f[1 + side]; f[1 + side] = t;
   for (int i = 0; i < N; ++i) flux[i] = f[i];
   return 1;
+}
+
+int oracle_euler_hll_flux(int solver, int dim, int side, double gamma, const double* srcL,
+                          const double* srcR, const double* eigval, double* flux) {
+  return oracle_euler_hll_flux_pot(solver, dim, side, gamma, srcL, srcR, 0., 0., eigval, flux);
 }
 
 /* ------------------------------------------------ SRHD cell kernels */
@@ -707,6 +723,7 @@ oracle_sim* oracle_create(const hydro_sim_desc* d, unsigned opts) {
   s->dtbuf = (double*)malloc(sizeof(double) * V * D);
   for (int i = 0; i < V * D; ++i) s->dtbuf[i] = DBL_MAX; /* Solver.cpp:217-222 */
   s->potential = (double*)calloc((size_t)V, sizeof(double));
+  s->pot_next = (double*)calloc((size_t)V, sizeof(double));
   if (d->equation == HYDRO_EQN_SRHD) s->prim = (double*)calloc((size_t)N * V, sizeof(double));
   if (d->solver == HYDRO_SOLVER_BURGERS) {
     /* src/Solver/EulerBurgers.cpp:14-22: both zeroed at creation */
@@ -731,7 +748,7 @@ oracle_sim* oracle_create(const hydro_sim_desc* d, unsigned opts) {
 void oracle_destroy(oracle_sim* s) {
   if (!s) return;
   free(s->state); free(s->eigval); free(s->eigvec); free(s->dqt); free(s->flux);
-  free(s->dtbuf); free(s->potential); free(s->prim); free(s->fe_deriv); free(s->ifvel); free(s->pressure);
+  free(s->dtbuf); free(s->potential); free(s->pot_next); free(s->prim); free(s->fe_deriv); free(s->ifvel); free(s->pressure);
   for (int i = 0; i < 4; ++i) { free(s->rk_state[i]); free(s->rk_deriv[i]); }
   free(s);
 }
@@ -760,11 +777,32 @@ int oracle_set_state(oracle_sim* s, const double* aos, size_t n) {
   return 0;
 }
 
+static void gravity_relax(oracle_sim* s, int sweeps);
+
+/* SelfGravitation::resetState (src/Solver/SelfGravitation.cpp:35-97) after
+ * the state upload: the initial potential phi0 (`pot`, NULL: zero; the
+ * guess -rho when useGravity), 100 x gaussSeidelMaxIter relaxation sweeps
+ * when useGravity, then phi0 added into every cell's total energy. */
 int oracle_set_potential(oracle_sim* s, const double* pot, size_t n) {
-  /* the HLL/HLLC restatement assumes a zero potential */
-  if (n != (size_t)s->vol || s->d.equation != HYDRO_EQN_EULER || s->d.solver != HYDRO_SOLVER_ROE) return -1;
-  memcpy(s->potential, pot, n * sizeof(double));
-  for (int c = 0; c < s->vol; ++c) s->state[(size_t)s->N * c + s->dim + 1] += pot[c];
+  if ((pot && n != (size_t)s->vol) || s->d.equation != HYDRO_EQN_EULER) return -1;
+  const int V = s->vol;
+  double* phi0 = (double*)malloc((size_t)V * sizeof(double));
+  for (int c = 0; c < V; ++c)
+    phi0[c] = s->d.use_gravity ? -s->state[(size_t)s->N * c] : (pot ? pot[c] : 0.);
+  memcpy(s->potential, phi0, (size_t)V * sizeof(double));
+  int nonzero = s->d.use_gravity;
+  for (int c = 0; c < V; ++c) nonzero |= phi0[c] != 0.;
+  s->pot_on = nonzero;
+  if (s->d.use_gravity)
+    for (int tries = 0; tries < 100; ++tries) gravity_relax(s, s->d.gauss_seidel_max_iter);
+  for (int c = 0; c < V; ++c) s->state[(size_t)s->N * c + s->dim + 1] += phi0[c];
+  free(phi0);
+  return 0;
+}
+
+int oracle_get_potential(const oracle_sim* s, double* pot, size_t n) {
+  if (n != (size_t)s->vol) return -1;
+  memcpy(pot, s->potential, n * sizeof(double));
   return 0;
 }
 
@@ -798,10 +836,24 @@ static int boundary_kernel(const oracle_sim* s, int axis, int state, int minmax)
   return -1;
 }
 
+/* SelfGravitationBehavior::gravityGetBoundaryKernelForBoundaryMethod
+ * (include/HydroGPU/Equation/SelfGravitationBehavior.h:22-34): the
+ * potential's ghosts copy (mirror becomes free-flow, nothing is negated) */
+static int potential_boundary_kernel(const oracle_sim* s, int axis, int state, int minmax) {
+  (void)state;
+  switch (s->d.boundary[axis][minmax]) {
+  case HYDRO_BOUNDARY_PERIODIC: return BK_PERIODIC;
+  case HYDRO_BOUNDARY_MIRROR: return BK_FREEFLOW;
+  case HYDRO_BOUNDARY_FREEFLOW: return BK_FREEFLOW;
+  }
+  return -1;
+}
+
 /* res/Common.cl:49-194 applied over the boundary NDRange of
- * src/Solver/Solver.cpp:412-444; in 1D only work-item 0 is in bounds. */
-static void boundary_buffer(oracle_sim* s, double* buf) {
-  const int N = s->N;
+ * src/Solver/Solver.cpp:412-444; in 1D only work-item 0 is in bounds.
+ * N interleaved states per cell, each state's kernel from `kernel`. */
+static void boundary_buffer_n(oracle_sim* s, double* buf, int N,
+                              int (*kernel)(const oracle_sim*, int, int, int)) {
   for (int axis = 0; axis < s->dim; ++axis) {
     int o1 = (axis == 0) ? 1 : 0;
     int o2 = (axis == 2) ? 1 : 2;
@@ -810,7 +862,7 @@ static void boundary_buffer(oracle_sim* s, double* buf) {
     int st = s->step[axis];
     for (int j = 0; j < N; ++j) {
       for (int minmax = 0; minmax < 2; ++minmax) {
-        int k = boundary_kernel(s, axis, j, minmax);
+        int k = kernel(s, axis, j, minmax);
         if (k < 0) continue;
         for (int b = 0; b < n2; ++b) {
           for (int a = 0; a < n1; ++a) {
@@ -841,9 +893,58 @@ static void boundary_buffer(oracle_sim* s, double* buf) {
   }
 }
 
+static void boundary_buffer(oracle_sim* s, double* buf) { boundary_buffer_n(s, buf, s->N, boundary_kernel); }
+
 void oracle_boundary(oracle_sim* s) {
   boundary_buffer(s, s->state);
   if (s->prim) boundary_buffer(s, s->prim); /* SRHDRoe.cpp:79-96 */
+}
+
+/* SelfGravitation::potentialBoundary (src/Solver/SelfGravitation.cpp:126-141) */
+static void potential_boundary(oracle_sim* s) { boundary_buffer_n(s, s->potential, 1, potential_boundary_kernel); }
+
+/* OpenCL's M_PI for double */
+#define ORACLE_PI 3.141592653589793115997963468544185161590576171875
+
+/* res/SelfGravitation.cl:20-63: one Jacobi-style relaxation sweep of
+ * nabla^2 phi = 4 pi G rho on the interior cells (neighbours outside the
+ * interior count as zero).  The reference updates the potential in place,
+ * so its sweep reads neighbours old or new depending on the order its
+ * work-items run; this restatement reads the previous sweep everywhere. */
+static void gravity_relax_sweep(oracle_sim* s) {
+  const int N = s->N, D = s->dim, V = s->vol;
+  const double* phi = s->potential;
+  double* next = s->pot_next;
+  memcpy(next, phi, (size_t)V * sizeof(double));
+  double diagsum = 0.;
+  for (int a = 0; a < D; ++a) diagsum += 1. / (s->kdx[a] * s->kdx[a]);
+  const double diag = -2. * diagsum;
+  const double G = s->d.self_grav_gravitational_constant;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < V; ++c) {
+    int i[3];
+    decode(s, c, i);
+    if (!active(s, i, 2)) continue;
+    double skewSum = 0.;
+    for (int a = 0; a < D; ++a) {
+      const double h2 = s->kdx[a] * s->kdx[a];
+      if (i[a] < s->sz[a] - 3) skewSum += phi[c + s->step[a]] / h2;
+      if (i[a] > 2) skewSum += phi[c - s->step[a]] / h2;
+    }
+    const double density = s->state[(size_t)N * c];
+    next[c] = (4. * ORACLE_PI * G * density - skewSum) / diag;
+  }
+  s->potential = next;
+  s->pot_next = (double*)phi;
+}
+
+/* gaussSeidelMaxIter x (potential boundary, relaxation sweep)
+ * (src/Solver/SelfGravitation.cpp:112-117, 83-88) */
+static void gravity_relax(oracle_sim* s, int sweeps) {
+  for (int k = 0; k < sweeps; ++k) {
+    potential_boundary(s);
+    gravity_relax_sweep(s);
+  }
 }
 
 /* ------------------------------------------------ Roe kernels */
@@ -1071,8 +1172,10 @@ static void hll_calc_eigenvalues(oracle_sim* s) {
     if (!active(s, i, 1)) continue;
     for (int side = 0; side < D; ++side) {
       int prev = c - s->step[side];
-      oracle_euler_hll_speeds(s->d.solver, D, side, s->gamma, s->state + (size_t)N * prev,
-                              s->state + (size_t)N * c, s->eigval + (size_t)N * (side + D * c));
+      const int pot = s->d.solver == HYDRO_SOLVER_HLL;
+      oracle_euler_hll_speeds_pot(s->d.solver, D, side, s->gamma, s->state + (size_t)N * prev,
+                                  s->state + (size_t)N * c, pot ? s->potential[prev] : 0.,
+                                  pot ? s->potential[c] : 0., s->eigval + (size_t)N * (side + D * c));
     }
   }
 }
@@ -1088,8 +1191,11 @@ static void hll_calc_flux(oracle_sim* s) {
     for (int side = 0; side < D; ++side) {
       int prev = c - s->step[side];
       size_t iface = (size_t)side + D * c;
-      oracle_euler_hll_flux(s->d.solver, D, side, s->gamma, s->state + (size_t)N * prev,
-                            s->state + (size_t)N * c, s->eigval + N * iface, s->flux + N * iface);
+      /* EulerHLL reads the potential, EulerHLLC does not */
+      const int pot = s->d.solver == HYDRO_SOLVER_HLL;
+      oracle_euler_hll_flux_pot(s->d.solver, D, side, s->gamma, s->state + (size_t)N * prev,
+                                s->state + (size_t)N * c, pot ? s->potential[prev] : 0., pot ? s->potential[c] : 0.,
+                                s->eigval + N * iface, s->flux + N * iface);
     }
   }
 }
@@ -1333,7 +1439,8 @@ static void run_callback(oracle_sim* s, callback_t cb, double* deriv, double dt)
     /* src/Solver/EulerBurgers.cpp:94-98: computePressure not called again */
     burgers_diffuse_work(s, deriv);
   } else if (cb == CB_GRAVITY) {
-    /* src/Solver/SelfGravitation.cpp:111-123 (useGravity = false) */
+    /* src/Solver/SelfGravitation.cpp:111-123 */
+    if (s->d.use_gravity) gravity_relax(s, s->d.gauss_seidel_max_iter);
     calc_gravity_deriv(s, deriv);
     oracle_boundary(s);
   } else {
@@ -1416,7 +1523,7 @@ void oracle_step(oracle_sim* s, double dt) {
        diffusion, boundary */
     integrate(s, dt, CB_HYDRO);
     oracle_boundary(s);
-    if (s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) integrate(s, dt, CB_GRAVITY);
+    if ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) || s->pot_on) integrate(s, dt, CB_GRAVITY);
     integrate(s, dt, CB_BURGERS_MOMENTUM);
     oracle_boundary(s);
     integrate(s, dt, CB_BURGERS_WORK);
@@ -1424,7 +1531,7 @@ void oracle_step(oracle_sim* s, double dt) {
     return;
   }
   integrate(s, dt, CB_HYDRO);
-  if (s->d.equation == HYDRO_EQN_EULER && (s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS)) {
+  if (s->d.equation == HYDRO_EQN_EULER && ((s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS) || s->pot_on)) {
     /* EulerRoe::step / EulerHLL::step -> SelfGravitation::applyPotential */
     integrate(s, dt, CB_GRAVITY);
   } else if (s->d.equation == HYDRO_EQN_MAXWELL && (s->d.flags & HYDRO_FLAG_GRAVITY_IDENTITY_PASS)) {

@@ -31,6 +31,7 @@ int oracle_set_state(oracle_sim* s, const double* aos, size_t n);
 int oracle_get_state(const oracle_sim* s, double* aos, size_t n);
 int oracle_get_primitives(const oracle_sim* s, double* aos, size_t n);
 int oracle_set_potential(oracle_sim* s, const double* pot, size_t n);
+int oracle_get_potential(const oracle_sim* s, double* pot, size_t n);
 void oracle_boundary(oracle_sim* s);
 void oracle_init_step(oracle_sim* s);
 double oracle_calc_timestep(oracle_sim* s);
@@ -57,6 +58,11 @@ void oracle_euler_hll_speeds(int solver, int dim, int side, double gamma, const 
 /* returns 0 when no branch of the kernel is taken (NaN speeds): flux untouched */
 int oracle_euler_hll_flux(int solver, int dim, int side, double gamma, const double* qL,
                           const double* qR, const double* eigval, double* flux);
+/* the same with the cells' potentials (EulerHLL; HLLC takes zeros) */
+void oracle_euler_hll_speeds_pot(int solver, int dim, int side, double gamma, const double* qL,
+                                 const double* qR, double potL, double potR, double* eigval);
+int oracle_euler_hll_flux_pot(int solver, int dim, int side, double gamma, const double* qL,
+                              const double* qR, double potL, double potR, const double* eigval, double* flux);
 /* SRHD con2prim of one cell; returns iterations used or -1 if not converged
  * (prim keeps the initial guess then) */
 int oracle_srhd_con2prim(const hydro_sim_desc* d, int dim, const double* cons, double* prim);
