@@ -237,3 +237,28 @@ def test_hll_speeds_with_potential_closed_form(phiL, phiR):
     e1 = oracle.hll_speeds(2, 1, 0, g, cons(rL, uL, pL, 0.0), cons(rR, uR, pR, 0.0))
     e2 = oracle.hll_speeds(2, 1, 0, g, cons(rL, uL, pL, 0.0), cons(rR, uR, pR, 0.0), phiL, phiR)
     np.testing.assert_array_equal(e1, e2)
+
+
+@pytest.mark.parametrize("dim,side", [(1, 0), (2, 1), (3, 2)])
+def test_roe_eigenvalues_see_the_potential_only_through_the_pressure(dim, side):
+    """res/EulerRoe.cl:56-94: the pressure subtracts phi from the specific
+    energy and the Roe-averaged sound speed subtracts the averaged phi, which
+    the averaged enthalpy carries: a state whose energy holds rho phi besides
+    its internal and kinetic parts has the eigenvalues of the same state
+    without potential (a slip in either subtraction shifts c)."""
+    g = GAMMA
+    rng = np.random.default_rng(10 + dim)
+    rL, rR = 1.0 + rng.random(), 0.3 + rng.random()
+    vL, vR = rng.uniform(-0.5, 0.5, dim), rng.uniform(-0.5, 0.5, dim)
+    pL, pR = 0.5 + rng.random(), 0.2 + rng.random()
+    phiL, phiR = rng.uniform(-0.8, 0.8, 2)
+
+    def cons(r, v, p, phi):
+        return np.concatenate([[r], r * v, [p / (g - 1) + 0.5 * r * v @ v + r * phi]])
+
+    ev_pot, _, _ = oracle.euler_eigenbasis(dim, side, g, cons(rL, vL, pL, phiL), cons(rR, vR, pR, phiR), phiL, phiR)
+    ev0, _, _ = oracle.euler_eigenbasis(dim, side, g, cons(rL, vL, pL, 0.0), cons(rR, vR, pR, 0.0))
+    np.testing.assert_allclose(ev_pot, ev0, rtol=1e-12, atol=1e-14)
+    # and the potential does change them when the energy does not carry it
+    ev1, _, _ = oracle.euler_eigenbasis(dim, side, g, cons(rL, vL, pL, 0.0), cons(rR, vR, pR, 0.0), phiL, phiR)
+    assert np.max(np.abs(ev1 - ev0)) > 1e-3
