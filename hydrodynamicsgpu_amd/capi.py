@@ -59,6 +59,7 @@ def lib():
         L.hydro_sim_phase_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.hydro_sim_set_potential.argtypes = [vp, dp, sz]
         L.hydro_sim_get_potential.argtypes = [vp, dp, sz]
+        L.hydro_sim_set_solid.argtypes = [vp, ctypes.c_void_p, sz]
         L.hydro_sim_stream.argtypes = [vp]
         L.hydro_sim_stream.restype = vp
         L.hydro_sim_element_size.argtypes = [vp]
@@ -155,6 +156,15 @@ class Sim:
         else:
             a = np.ascontiguousarray(pot, dtype=np.float64).ravel()
             check(lib().hydro_sim_set_potential(self._h, _dp(a), a.size))
+
+    def set_solid(self, solid=None):
+        """The solid flag per cell (nonzero = solid; None clears them), read
+        by EulerRoe and EulerBurgers."""
+        if solid is None:
+            check(lib().hydro_sim_set_solid(self._h, None, 0))
+        else:
+            a = np.ascontiguousarray(np.asarray(solid).ravel() != 0, dtype=np.uint8)
+            check(lib().hydro_sim_set_solid(self._h, a.ctypes.data, a.size))
 
     def get_potential(self):
         out = np.empty(self.volume)
@@ -308,7 +318,7 @@ EXPORTED_SYMBOLS = [
     "hydro_sim_boundary", "hydro_sim_calc_timestep", "hydro_sim_step", "hydro_sim_update",
     "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
     "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_sim_phase_timing",
-    "hydro_sim_set_potential", "hydro_sim_get_potential",
+    "hydro_sim_set_potential", "hydro_sim_get_potential", "hydro_sim_set_solid",
     "hydro_sim_element_size", "hydro_integrator_tableau", "hydro_sim_buffer_create", "hydro_sim_buffer_destroy",
     "hydro_sim_state_buffer", "hydro_sim_init_step", "hydro_sim_calc_deriv", "hydro_sim_buffer_copy",
     "hydro_sim_buffer_zero", "hydro_sim_mult_add", "hydro_sim_finish_step",

@@ -129,6 +129,39 @@ __global__ void __launch_bounds__(256) dt_kernel(DtArgs a, const S* u0) {
       } else if (a.eqn == 2) {
         // Maxwell: constant speeds +-1/sqrt(eps mu) (res/MaxwellRoe.cl:139-146)
         d = a.kdx[side] / (fabs(fmax(0.0, a.maxwell_lambda) - fmin(0.0, -a.maxwell_lambda)) + 1e-9);
+      } else if (a.solver == 0 && a.solid) {
+        // solid cells (res/Roe.cl:38-44): no limit across a side touching a
+        // solid cell; a solid lower neighbour enters the lower face mirrored
+        // (res/EulerRoe.cl:48-65): this cell's density and velocity, velocity
+        // component side+1 negated, its own energy and potential
+        using P = EulerRoe<DIM, S, true>;
+        const S* pt = static_cast<const S*>(a.pot);
+        const int ms = int(static_cast<const S*>(a.solid)[cell]);
+        if ((ms & 1) || ((ms >> (2 + 2 * side)) & 1)) {
+          d = INFINITY;
+        } else {
+          const long long cm = cell - step[side], cp = cell + step[side];
+          const auto cc = P::template cell<S>(q, gamma, pt[cell]);
+          typename P::Cell cl;
+          if ((ms >> (1 + 2 * side)) & 1) {
+            S v[DIM];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) {
+              v[k] = q[1 + k] / q[0];
+              if (k == side + 1) v[k] = -v[k];
+            }
+            cl = P::cell_rv(q[0], v, qm[DIM + 1], pt[cm], gamma);
+          } else {
+            cl = P::template cell<S>(qm, gamma, pt[cm]);
+          }
+          const auto fL = P::face(cl, cc, side, gamma);
+          const auto fR = P::face(cc, P::template cell<S>(qp, gamma, pt[cp]), side, gamma);
+          const double smax = double(fL.v[0] + fL.c), smin = double(fR.v[0] - fR.c);
+          bad |= !(smax == smax) || !(smin == smin);
+          const double lmax = fmax(0.0, smax);
+          const double lmin = fmin(0.0, smin);
+          d = a.kdx[side] / (fabs(lmax - lmin) + 1e-9);
+        }
       } else if (a.solver == 0 && a.pot) {
         // with the cells' potentials (res/EulerRoe.cl:56-94)
         using P = EulerRoe<DIM, S, true>;

@@ -43,6 +43,13 @@ HYDRO_DEV S pot_at(const void* pot, long long c) {
   return pot ? static_cast<const S*>(pot)[c] : S(0);
 }
 
+// the solid mask of a cell (0 without one): bit 0 the cell itself, bits
+// 1+2a / 2+2a its lower / upper neighbour along axis a
+template <typename S>
+HYDRO_DEV int mask_at(const void* solid, long long c) {
+  return solid ? int(static_cast<const S*>(solid)[c]) : 0;
+}
+
 // ---------------------------------------------------------------- CFL
 // res/EulerBurgers.cl:19-79: cells 2 .. size-3, dx / (c + |v|) with
 // c = sqrt(gamma (gamma - 1) e_int), the smallest over the axes (fmin).
@@ -63,6 +70,7 @@ __global__ void __launch_bounds__(256) burgers_dt_kernel(DtArgs a, const S* u) {
     const int y = DIM > 1 ? 2 + (int)((i / nx) % ny) : 0;
     const int z = DIM > 2 ? 2 + (int)(i / ((long long)nx * ny)) : 0;
     const long long cell = x + (long long)sx * (y + (long long)sy * z);
+    if (mask_at<S>(a.solid, cell) & 1) continue;  // res/EulerBurgers.cl:42-47: INFINITY
     const S rho = u[cell];
     const S E = u[(DIM + 1) * a.plane + cell];
     S v[DIM];
@@ -181,9 +189,15 @@ __global__ void __launch_bounds__(256) burgers_momentum_kernel(StageArgs<T, S> a
   const double dtd = *a.dt;
   const StageCombine<T, S, KAHAN, DIM, MODE> comb(a, dtd);
   T deriv[DIM];
+  // solid cells (res/EulerBurgers.cl:372-400): none for a solid cell; a solid
+  // neighbour reads as this cell's pressure
+  const int ms = mask_at<S>(b.solid, c);
 #pragma unroll
-  for (int k = 0; k < DIM; ++k)
-    deriv[k] = -(T(.5) * (pin[c + step[k]] - pin[c - step[k]])) * T(b.inv_dx[k]);
+  for (int k = 0; k < DIM; ++k) {
+    const T pL = (ms >> (1 + 2 * k)) & 1 ? pin[c] : pin[c - step[k]];
+    const T pR = (ms >> (2 + 2 * k)) & 1 ? pin[c] : pin[c + step[k]];
+    deriv[k] = (ms & 1) ? T(0) : -(T(.5) * (pR - pL)) * T(b.inv_dx[k]);
+  }
   T dn[DIM];  // d_{k+1} of this cell, as stored
   comb.finish(e, (uint32_t)x, deriv, dn);
   constexpr bool kFinal = MODE == CM_FINAL || MODE == CM_SINGLE;
@@ -224,13 +238,22 @@ __global__ void __launch_bounds__(256) burgers_work_kernel(BurgersArgs b, CellMa
   if (x > b.sx - 3) return;
   const long long c = e + x;
   const long long step[3] = {1, (long long)b.sx, (long long)b.sx * b.sy};
+  // solid cells (res/EulerBurgers.cl:425-447): none for a solid cell; a solid
+  // neighbour's own velocity negated, and this cell's pressure
+  const int ms = mask_at<S>(b.solid, c);
+  if (ms & 1) return;
   T W = T(0);
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
     const long long l = c - step[k], r = c + step[k];
-    const T vL = tdiv(T(u[(1 + k) * b.plane + l]), T(u[l]));
-    const T vR = tdiv(T(u[(1 + k) * b.plane + r]), T(u[r]));
-    W -= (T(.5) * (p[r] * vR - p[l] * vL)) * T(b.inv_dx[k]);
+    const bool sl = (ms >> (1 + 2 * k)) & 1, sr = (ms >> (2 + 2 * k)) & 1;
+    T vL = tdiv(T(u[(1 + k) * b.plane + l]), T(u[l]));
+    T vR = tdiv(T(u[(1 + k) * b.plane + r]), T(u[r]));
+    if (sl) vL = -vL;
+    if (sr) vR = -vR;
+    const T pL = sl ? p[c] : p[l];
+    const T pR = sr ? p[c] : p[r];
+    W -= (T(.5) * (pR * vR - pL * vL)) * T(b.inv_dx[k]);
   }
   const T acc = T(b.work_coef * *b.dt) * W;
   const long long ie = (DIM + 1) * b.plane + c;

@@ -104,7 +104,7 @@ int cfl_path(const CflGrid& g) {
   const hydro_sim_desc& d = *g.d;
   if (d.use_fixed_dt) return 0;
   if (d.solver == HYDRO_SOLVER_BURGERS) return 3;  // cell-local speeds: block minima
-  if (g.pot) return 3;  // speeds with the cells' potentials: the generic kernel
+  if (g.pot || g.solid) return 3;  // speeds with the cells' potentials or solid cells: the generic kernel
   const int hi = d.solver == HYDRO_SOLVER_ROE ? 3 : 2;
   const long long strips = (g.sz[0] - hi - 2) / 60 + 1;
   const bool srhd = d.equation == HYDRO_EQN_SRHD;
@@ -202,6 +202,7 @@ int enqueue_cfl(const CflGrid& g, const void* U, const void* prim, bool prim_f64
     a.nblocks = g.dt_blocks;
     a.blockmin = minima;
     a.pot = g.pot;
+    a.solid = g.solid;
     if (d.solver == HYDRO_SOLVER_BURGERS) {
       // EulerBurgers::calcTimestep (src/Solver/EulerBurgers.cpp:61-65)
       if (g.mixed) launch_dt_burgers_f32(a, (const float*)U, st);
@@ -252,6 +253,11 @@ struct hydro_sim {
   // every update adds the gravity pass
   void* pot[2] = {nullptr, nullptr};
   bool pot_on = false;
+  // Euler: the solid mask plane (state precision; per cell bit 0 the cell's
+  // own solid flag, bits 1+2a / 2+2a its lower / upper neighbour's along
+  // axis a), read by the Roe and EulerBurgers kernels when solid_on
+  void* solid = nullptr;
+  bool solid_on = false;
   double work_coef = 1.0;              // EulerBurgers: the tableau's sum for a constant derivative
   double* blockmin = nullptr;
   double* partial = nullptr;
@@ -570,7 +576,10 @@ void fill_stage_args(hydro_sim* s, int k, StageArgs<T, T>& a) {
   for (int i = 0; i < 3; ++i) a.inv_dx[i] = 1.0 / s->kdx[i];
   a.gamma = s->d.gamma;
   a.prim = (const T*)s->prim;
-  a.pot = (s->pot_on && (kernel_kind(s->d) == 0 || kernel_kind(s->d) == 1)) ? (const T*)s->pot[0] : nullptr;
+  const int kind = kernel_kind(s->d);
+  // the solid Roe form always reads the potential (a zero plane without one)
+  a.pot = ((s->pot_on && (kind == 0 || kind == 1)) || (s->solid_on && kind == 0)) ? (const T*)s->pot[0] : nullptr;
+  a.solid = (s->solid_on && (kind == 0 || kind == 5)) ? (const T*)s->solid : nullptr;
   maxwell_constants(s->d, a.mx);
   a.lim = s->d.limiter;
   const int interior_x = s->sz[0] - 4;
@@ -846,6 +855,7 @@ int enqueue_burgers_pressure_passes(hydro_sim* s) {
   b.dt = &s->sc->dt;
   b.work_coef = s->work_coef;
   b.pot = s->pot_on ? s->pot[0] : nullptr;
+  b.solid = s->solid_on ? s->solid : nullptr;
   int rc = enqueue_boundary(s);
   // the gravity pass between the advection boundary and the momentum pass
   // (src/Solver/EulerBurgers.cpp:80-82)
@@ -911,7 +921,7 @@ constexpr int kRun1dChunk = 256;
 
 bool run1d_eligible(const hydro_sim* s) {
   if (!s->use_run1d || s->profiling || s->dim != 1 || s->d.equation != HYDRO_EQN_EULER) return false;
-  if (s->pot_on) return false;
+  if (s->pot_on || s->solid_on) return false;
   if (s->d.solver == HYDRO_SOLVER_BURGERS) return false;
   if (!boundary_covers_ghosts(s) || s->tab.order > kRun1dMaxStages) return false;
   const int strips = (s->sz[0] - 4 + 59) / 60;  // 60 output columns per strip
@@ -1135,6 +1145,7 @@ void hydro_sim_destroy(hydro_sim* s) {
     if (p) hipFree(p);
   for (void* p : s->pot)
     if (p) hipFree(p);
+  if (s->solid) hipFree(s->solid);
   if (s->blockmin) hipFree(s->blockmin);
   if (s->partial) hipFree(s->partial);
   if (s->sc) hipFree(s->sc);
@@ -1205,6 +1216,18 @@ bool roe_family_3d(const hydro_sim* s) { return s->dim == 3 && !first_order_3d(s
 // the 4th decomposition parameter: waves per workgroup of the 3D kernel
 TuneParams current_params(const hydro_sim* s) {
   return TuneParams{s->rows, s->wpb, s->form3d, roe_family_3d(s) ? s->rwaves : s->zwaves, 0.0};
+}
+
+// the planes the CFL pass reads: the potential (Roe, HLL, EulerBurgers with
+// one; the solid Roe form always) and the solid mask (Roe, EulerBurgers)
+void set_cfl_planes(hydro_sim* s) {
+  const int sv = s->d.solver;
+  const bool roe = sv == HYDRO_SOLVER_ROE && s->d.equation == HYDRO_EQN_EULER;
+  const bool solid = s->solid_on && (roe || sv == HYDRO_SOLVER_BURGERS);
+  s->cg.solid = solid ? s->solid : nullptr;
+  const bool pot = (s->pot_on && (sv == HYDRO_SOLVER_ROE || sv == HYDRO_SOLVER_HLL || sv == HYDRO_SOLVER_BURGERS)) ||
+                   (solid && roe);
+  s->cg.pot = pot ? s->pot[0] : nullptr;
 }
 
 void drop_graphs(hydro_sim* s) {
@@ -1569,6 +1592,51 @@ int hydro_sim_get_storage(hydro_sim* s, double* hi, float* lo, size_t n) {
   return HYDRO_OK;
 }
 
+// SelfGravitation::resetState's solidBuffer upload
+// (src/Solver/SelfGravitation.cpp:56-75): one flag per cell, kept as a mask
+// plane with the neighbours' flags folded in.
+int hydro_sim_set_solid(hydro_sim* s, const unsigned char* solid, size_t n) {
+  if (!s) return fail(HYDRO_E_INVALID, "null argument");
+  if (s->d.equation != HYDRO_EQN_EULER) return fail(HYDRO_E_UNSUPPORTED, "solid cells belong to the Euler solvers");
+  if (solid && n != (size_t)s->vol) return fail(HYDRO_E_INVALID, "solid mask size must be the volume");
+  bool any = false;
+  if (solid)
+    for (size_t i = 0; i < n && !any; ++i) any = solid[i] != 0;
+  if (any && !s->solid) {
+    if (hipMalloc(&s->solid, esize(s) * (size_t)s->plane) != hipSuccess)
+      return fail(HYDRO_E_HIP, "out of device memory (solid mask)");
+  }
+  if (any) {
+    const long long step[3] = {1, s->sz[0], (long long)s->sz[0] * s->sz[1]};
+    std::vector<double> md((size_t)s->plane, 0.0);
+    for (int z = 0; z < s->sz[2]; ++z)
+      for (int y = 0; y < s->sz[1]; ++y)
+        for (int x = 0; x < s->sz[0]; ++x) {
+          const int ci[3] = {x, y, z};
+          const long long c = x + step[1] * y + step[2] * z;
+          int m = solid[c] ? 1 : 0;
+          for (int a = 0; a < s->dim; ++a) {
+            if (ci[a] > 0 && solid[c - step[a]]) m |= 1 << (1 + 2 * a);
+            if (ci[a] < s->sz[a] - 1 && solid[c + step[a]]) m |= 1 << (2 + 2 * a);
+          }
+          md[c] = m;
+        }
+    const size_t bytes = esize(s) * (size_t)s->plane;
+    hipError_t e;
+    if (s->mixed) {
+      std::vector<float> mf(md.begin(), md.end());
+      e = copy_sync(s->solid, mf.data(), bytes, hipMemcpyHostToDevice, s->stream);
+    } else {
+      e = copy_sync(s->solid, md.data(), bytes, hipMemcpyHostToDevice, s->stream);
+    }
+    if (e != hipSuccess) return fail(HYDRO_E_HIP, "copy (solid mask)");
+  }
+  s->solid_on = any;
+  set_cfl_planes(s);
+  drop_graphs(s);  // captured updates with the other kernel forms
+  return HYDRO_OK;
+}
+
 int hydro_sim_set_potential(hydro_sim* s, const double* pot, size_t n) {
   if (!s) return fail(HYDRO_E_INVALID, "null argument");
   if (s->d.equation != HYDRO_EQN_EULER) return fail(HYDRO_E_UNSUPPORTED, "a potential belongs to the Euler solvers");
@@ -1615,10 +1683,7 @@ int hydro_sim_set_potential(hydro_sim* s, const double* pot, size_t n) {
     }
   }
   s->pot_on = nonzero;
-  s->cg.pot = (nonzero && (s->d.solver == HYDRO_SOLVER_ROE || s->d.solver == HYDRO_SOLVER_HLL ||
-                           s->d.solver == HYDRO_SOLVER_BURGERS))
-                  ? s->pot[0]
-                  : nullptr;
+  set_cfl_planes(s);
   drop_graphs(s);  // captured updates without (or with) the gravity pass
   HIP_TRY(hipStreamSynchronize(s->stream));
   return HYDRO_OK;

@@ -23,6 +23,7 @@ struct CflGrid {
   int dt_rows = 0;       // 2D strip march: rows per wave (0: chosen from the grid size)
   int dt_blocks = 1024;  // blocks of the generic CFL kernel
   const void* pot = nullptr;  // potential plane the speeds read (Euler Roe, HLL, EulerBurgers), or null
+  const void* solid = nullptr;  // solid mask plane (Euler Roe, EulerBurgers), or null; Roe then reads pot too
 };
 
 // sizes, kernel dx (src/HydroGPUApp.cpp:224-226 and the 6-digit rendering of

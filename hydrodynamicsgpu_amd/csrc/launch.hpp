@@ -66,6 +66,8 @@ struct DtArgs {
   int prim_f32;          // ... in FP32 (else FP64)
   double maxwell_lambda; // Maxwell: 1/sqrt(eps mu)
   const void* pot;       // Euler Roe / HLL / EulerBurgers: potential plane (state precision), or null
+  const void* solid;     // Euler Roe / EulerBurgers: solid mask plane (state precision), or null;
+                         // Roe with a mask reads pot too
   int nblocks;
   double* blockmin;      // [nblocks]
 };
@@ -184,6 +186,7 @@ struct BurgersArgs {
   const double* dt;   // device-resident step size
   double work_coef;   // what the tableau's rows sum a constant derivative to
   const void* pot;    // the potential plane (state precision), or null
+  const void* solid;  // the solid mask plane (state precision), or null
   // momentum pass: its start state and the new state (unshifted planes;
   // low words of compensated storage, or null)
   const void* u0;

@@ -70,6 +70,31 @@ struct EulerRoe {
     return c;
   }
 
+  // the same frozen data from density, velocity, total energy and potential
+  // given separately (a solid cell's mirrored density and velocity with its
+  // own energy and potential)
+  static HYDRO_DEV Cell cell_rv(T rho, const T v[DIM], T E_, T phi, T gamma) {
+    Cell c;
+    c.phi = phi;
+    T inv, vsq = T(0);
+    if constexpr (kFast) {
+      const T rs = __builtin_amdgcn_rsqf(rho);
+      c.w = rho * rs;
+      inv = rs * rs;
+    } else {
+      c.w = tsqrt(rho);
+      inv = T(1) / rho;
+    }
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      c.v[a] = v[a];
+      vsq += v[a] * v[a];
+    }
+    const T eT = E_ * inv;
+    c.H = eT + (gamma - T(1)) * (eT - T(.5) * vsq - phi);
+    return c;
+  }
+
   static HYDRO_DEV Face face(const Cell& L, const Cell& R, int side, T gamma) {
     Face f;
     const T norm = trcp(L.w + R.w);

@@ -24,6 +24,14 @@ struct PolicyPotential : std::false_type {};
 template <class P>
 struct PolicyPotential<P, std::void_t<decltype(P::kPotential)>> : std::bool_constant<P::kPotential> {};
 
+// Policies that read the solid mask (kSolid = true) take it in frozen_solid:
+// one value per cell whose bit 0 is the cell's own solid flag and bits
+// 1+2a / 2+2a those of its lower / upper neighbour along axis a.
+template <class P, class = void>
+struct PolicySolid : std::false_type {};
+template <class P>
+struct PolicySolid<P, std::void_t<decltype(P::kSolid)>> : std::bool_constant<P::kSolid> {};
+
 template <class Pol, typename S, typename T>
 HYDRO_DEV typename Pol::Frozen frozen_of(const S* q0, S pot, T gamma) {
   if constexpr (PolicyPotential<Pol>::value) {
@@ -32,6 +40,16 @@ HYDRO_DEV typename Pol::Frozen frozen_of(const S* q0, S pot, T gamma) {
     (void)pot;
     return Pol::template frozen<S>(q0, gamma);
   }
+}
+template <class Pol, typename S, typename T>
+HYDRO_DEV typename Pol::Frozen frozen_of(const S* q0, S pot, S sol, T gamma) {
+  if constexpr (PolicySolid<Pol>::value) return Pol::template frozen_solid<S>(q0, T(pot), T(sol), gamma);
+  else return frozen_of<Pol>(q0, pot, gamma);
+}
+template <class Pol, typename S>
+HYDRO_DEV bool solid_cell_of(S sol) {
+  if constexpr (PolicySolid<Pol>::value) return (int(sol) & 1) != 0;
+  else return false;
 }
 template <class Pol, typename T, typename S>
 HYDRO_DEV typename Pol::Stage stage_of(const T* u, S pot, T gamma) {
@@ -191,6 +209,137 @@ struct RoePolicy : StageIsState<DIM + 2, T> {
     P::eig(a.f, lam);
 #pragma unroll
     for (int k = 0; k < N; ++k) ft[k] = wave_flux<LIM, T>(lim, lam[k], a.lq2[k], a.dq[k], dqlo[k], dqhi[k], dt_dx);
+    P::right(a.f, ft, F);
+    P::rotate(F, side);
+  }
+};
+
+// ------------------------------------------------------------------ Roe, solid cells
+// The SOLID build of EulerRoe (src/Solver/EulerRoe.cpp:11-26).  Across a face
+// between a solid and a fluid cell the solid side is replaced by the fluid
+// cell:
+//  * eigenbasis (res/EulerRoe.cl:48-65): the fluid cell's density and
+//    velocity, with component side+1 of the velocity 4-vector (vx, vy, vz, 0)
+//    negated (y on x-faces, z on y-faces, the padding on z-faces), and the
+//    solid cell's own total energy and potential;
+//  * deltaQTilde and the flux average (res/Roe.cl:110-124, 235-248): the
+//    fluid cell's stage state with the normal momentum negated;
+//  * the limiter ratio is 1 when the cell two over on the upwind side is
+//    solid (res/Roe.cl:249-250, 278-290);
+// and solid cells take no flux divergence (res/CalcFluxDeriv.cl:28-30; the
+// kernels zero their derivative).  The frozen data are the raw (rho, v, E,
+// phi) and the mask, the Roe averages formed per face after the mirror.
+// The potential is always read (a zero plane gives the plain Roe values:
+// x - 0 is x); the limiter is the runtime one.
+template <int DIM, typename T>
+struct SolidRoePolicy : StageIsState<DIM + 2, T> {
+  using P = EulerRoe<DIM, T, true>;
+  static constexpr int N = P::N;
+  static constexpr int E = DIM + 1;
+  using Stage = typename StageIsState<N, T>::Stage;
+  static constexpr bool kNeighbors = true;
+  static constexpr bool kFrozenPrims = false;
+  static constexpr bool kPotential = true;
+  static constexpr bool kSolid = true;
+  struct Frozen {
+    T rho;
+    T v[DIM];
+    T en, phi, mask;
+  };
+  static constexpr int NF = DIM + 4;
+  struct FaceA {
+    typename P::Face f;
+    T dq[N];
+    T lq2[N];
+    bool one_lo, one_hi;  // the ratio is 1 for waves moving right / left
+  };
+
+  template <typename S>
+  static HYDRO_DEV Frozen frozen_solid(const S q0[N], T phi, T mask, T) {
+    Frozen c;
+    c.rho = T(q0[0]);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) c.v[a] = tdiv(T(q0[1 + a]), c.rho);
+    c.en = T(q0[E]);
+    c.phi = phi;
+    c.mask = mask;
+    return c;
+  }
+  static HYDRO_DEV Stage stage_pot(const T u[N], T, T gamma) { return StageIsState<N, T>::stage(u, gamma); }
+  static HYDRO_DEV void pack(const Frozen& c, T o[NF]) {
+    o[0] = c.rho;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) o[1 + a] = c.v[a];
+    o[DIM + 1] = c.en;
+    o[DIM + 2] = c.phi;
+    o[DIM + 3] = c.mask;
+  }
+  static HYDRO_DEV Frozen unpack(const T o[NF]) {
+    Frozen c;
+    c.rho = o[0];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) c.v[a] = o[1 + a];
+    c.en = o[DIM + 1];
+    c.phi = o[DIM + 2];
+    c.mask = o[DIM + 3];
+    return c;
+  }
+
+  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const Stage& sL, const Stage& sR, int side,
+                                bool valid, const PhysParams<T>& pp) {
+    const T gamma = pp.gamma;
+    const int mL = int(cL.mask), mR = int(cR.mask);
+    const bool sl = (mL & 1) != 0, sr = (mR & 1) != 0;
+    const bool mirL = sl && !sr, mirR = sr && !sl;
+    T rL = cL.rho, rR = cR.rho, vL[DIM], vR[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      const bool flip = a == side + 1;
+      vL[a] = mirL ? (flip ? -cR.v[a] : cR.v[a]) : cL.v[a];
+      vR[a] = mirR ? (flip ? -cL.v[a] : cL.v[a]) : cR.v[a];
+    }
+    if (mirL) rL = rR;
+    if (mirR) rR = rL;
+    FaceA a;
+    a.f = P::face(P::cell_rv(rL, vL, cL.en, cL.phi, gamma), P::cell_rv(rR, vR, cR.en, cR.phi, gamma), side,
+                  gamma);
+    T dlt[N], sum[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const bool normal = s == side + 1;
+      T uL = sL.u[s], uR = sR.u[s];
+      if (mirL) uL = normal ? -sR.u[s] : sR.u[s];
+      if (mirR) uR = normal ? -sL.u[s] : sL.u[s];
+      dlt[s] = uR - uL;
+      sum[s] = uR + uL;
+    }
+    P::rotate(dlt, side);
+    P::rotate(sum, side);
+    P::left(a.f, dlt, a.dq, gamma);
+    P::left(a.f, sum, a.lq2, gamma);
+    if (!valid) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) a.dq[s] = T(0);
+    }
+    a.one_lo = ((mL >> (1 + 2 * side)) & 1) != 0;
+    a.one_hi = ((mR >> (2 + 2 * side)) & 1) != 0;
+    return a;
+  }
+
+  static HYDRO_DEV void face_b(const FaceA& a, const T dqlo[N], const T dqhi[N], T dt_dx, int side, int lim,
+                               T F[N]) {
+    T lam[N], ft[N];
+    P::eig(a.f, lam);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const T l = lam[k], d = a.dq[k];
+      const bool up = l >= T(0);
+      const T r = (up ? a.one_lo : a.one_hi) ? T(1) : tdiv(up ? dqlo[k] : dqhi[k], d);
+      const T hl = T(.5) * l;
+      const T hd = (T(.5) * d) * limiter<LIM_RUNTIME, T>(lim, r);
+      const T w = l * (l * dt_dx) - tabs(l);
+      ft[k] = hl * a.lq2[k] - (tabs(hl) * d + hd * w);
+    }
     P::right(a.f, ft, F);
     P::rotate(F, side);
   }
@@ -536,6 +685,89 @@ struct BurgersPolicy {
       const T num = up ? dqlo[s] : dqhi[s];
       const T hd = half_limited_slope<LIM, T>(lim, num, a.dq[s]);
       F[s] = a.ui * a.q[s] + (hd * au) * w;
+    }
+  }
+};
+
+// The SOLID build of EulerBurgers (src/Solver/EulerBurgers.cpp:29-58),
+// advection part:
+//  * interface velocity (res/EulerBurgers.cl:119-127): a solid cell facing a
+//    fluid one takes the negated normal velocity of the fluid cell, so the
+//    face velocity is exactly 0 and the face carries no flux (unless the
+//    limited term is not finite, as in the reference);
+//  * flux states (res/EulerBurgers.cl:209-236): the solid side takes the
+//    fluid state, normal momentum kept and every other state negated; a
+//    solid cell two over repeats the near state with the normal momentum
+//    negated, so that jump is 2 q (normal momentum) or 0.
+// The upwind neighbour's jump is the raw jump of the neighbour face plus
+// this face's own mirror correction (zero unless this face is a wall, where
+// the face velocity is 0).  The frozen data are the solid mask alone.
+template <int DIM, typename T>
+struct SolidBurgersPolicy : BurgersPolicy<DIM, T, LIM_RUNTIME> {
+  using Base = BurgersPolicy<DIM, T, LIM_RUNTIME>;
+  using Stage = typename Base::Stage;
+  static constexpr int N = Base::N;
+  static constexpr bool kSolid = true;
+  struct Frozen {
+    T mask;
+  };
+  static constexpr int NF = 1;
+  struct FaceA {
+    T dq[N];  // raw q_R - q_L (the neighbours' ratios)
+    T dm[N];  // this face's jump after the mirror
+    T ql[N], qr[N];  // this face's states after the mirror
+    T cl[N], cr[N];  // q_L' - q_L and q_R - q_R'
+    T ui;
+    bool one_lo, one_hi;
+  };
+
+  template <typename S>
+  static HYDRO_DEV Frozen frozen_solid(const S*, T, T mask, T) {
+    return Frozen{mask};
+  }
+  static HYDRO_DEV void pack(const Frozen& c, T o[NF]) { o[0] = c.mask; }
+  static HYDRO_DEV Frozen unpack(const T o[NF]) { return Frozen{o[0]}; }
+
+  static HYDRO_DEV FaceA face_a(const Frozen& cL, const Frozen& cR, const Stage& sL, const Stage& sR, int side,
+                                bool, const PhysParams<T>&) {
+    const int mL = int(cL.mask), mR = int(cR.mask);
+    const bool sl = (mL & 1) != 0, sr = (mR & 1) != 0;
+    const bool mirL = sl && !sr, mirR = sr && !sl;
+    FaceA a;
+    T vL = sL.v[side], vR = sR.v[side];
+    if (mirL) vL = -vR;
+    if (mirR) vR = -vL;
+    a.ui = T(.5) * (vL + vR);
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const bool normal = s == side + 1;
+      T qL = sL.u[s], qR = sR.u[s];
+      if (mirL) qL = normal ? sR.u[s] : -sR.u[s];
+      if (mirR) qR = normal ? sL.u[s] : -sL.u[s];
+      a.dq[s] = sR.u[s] - sL.u[s];
+      a.dm[s] = qR - qL;
+      a.ql[s] = qL;
+      a.qr[s] = qR;
+      a.cl[s] = qL - sL.u[s];
+      a.cr[s] = sR.u[s] - qR;
+    }
+    a.one_lo = ((mL >> (1 + 2 * side)) & 1) != 0;
+    a.one_hi = ((mR >> (2 + 2 * side)) & 1) != 0;
+    return a;
+  }
+
+  static HYDRO_DEV void face_b(const FaceA& a, const T dqlo[N], const T dqhi[N], T dt_dx, int side, int lim, T F[N]) {
+    const bool up = a.ui >= T(0);
+    const T au = tabs(a.ui);
+    const T w = T(1) - tabs(a.ui * dt_dx);
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const bool normal = s == side + 1;
+      T num;
+      if (up) num = a.one_lo ? a.ql[s] - (normal ? -a.ql[s] : a.ql[s]) : a.cl[s] + dqlo[s];
+      else num = a.one_hi ? (normal ? -a.qr[s] : a.qr[s]) - a.qr[s] : dqhi[s] + a.cr[s];
+      const T hd = half_limited_slope<LIM_RUNTIME, T>(lim, num, a.dm[s]);
+      F[s] = a.ui * (up ? a.ql[s] : a.qr[s]) + (hd * au) * w;
     }
   }
 };

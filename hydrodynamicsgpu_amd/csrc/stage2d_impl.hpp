@@ -22,7 +22,9 @@
 //  * the Roe limiter ratio uses the neighbour interface's own basis
 //    (res/Roe.cl:218-220, 278-290) and the full-step dt (res/Roe.cl:206, 292);
 //  * interfaces the reference never writes (index 1 and size-1 along an
-//    axis, or any coordinate outside 2..size-2) contribute deltaQTilde = 0.
+//    axis, or any coordinate outside 2..size-2) contribute deltaQTilde = 0;
+//  * solid cells (policies with kSolid) take a zero flux divergence
+//    (res/CalcFluxDeriv.cl:28-30).
 #pragma once
 #include "device_common.hpp"
 #include "launch.hpp"
@@ -68,8 +70,10 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
     T dk[N];
     S pr[Pol::kFrozenPrims ? N : 1];
     S pot;  // the cell's potential (policies that read one), else 0
+    S sol;  // the cell's solid mask (policies that read one), else 0
   };
   constexpr bool kPot = PolicyPotential<Pol>::value;
+  constexpr bool kSol = PolicySolid<Pol>::value;
 
   auto fetch = [&](int row, Raw& w) {
     const long long e = (long long)__builtin_amdgcn_readfirstlane(row) * sx;  // wave-uniform
@@ -81,13 +85,15 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
     }
     if constexpr (kPot) w.pot = ldp(a.pot, pl, 0, e, cl);
     else w.pot = S(0);
+    if constexpr (kSol) w.sol = ldp(a.solid, pl, 0, e, cl);
+    else w.sol = S(0);
   };
   auto build = [&](const Raw& w, Row& R) {
     T u[N];
     comb.stage_state(w.q0, w.dk, u);
     R.st = stage_of<Pol>(u, w.pot, gamma);
     if constexpr (Pol::kFrozenPrims) R.fz = Pol::template frozen<S>(w.pr, gamma);
-    else R.fz = frozen_of<Pol>(w.q0, w.pot, gamma);
+    else R.fz = frozen_of<Pol>(w.q0, w.pot, w.sol, gamma);
   };
   auto load_row = [&](int row, Row& R) {
     Raw w;
@@ -129,6 +135,10 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
     build(w, R);
     T div[N];
     xdiv(R, div);
+    if (solid_cell_of<Pol>(w.sol)) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) div[s] = T(0);
+    }
     if (out_lane) comb.finish(0, c, fin, div);
   } else {
     const T dt_dy = T(dtd * a.inv_dx[1]);
@@ -195,6 +205,10 @@ HYDRO_DEV void stage2d_wave(const StageArgs<T, S>& a, int strip, int chunk, doub
         T deriv[N];
 #pragma unroll
         for (int s = 0; s < N; ++s) deriv[s] = acc[s] - G[s] * idy;
+        if (solid_cell_of<Pol>(w2.sol)) {
+#pragma unroll
+          for (int s = 0; s < N; ++s) deriv[s] = T(0);
+        }
         comb.finish(eo, c, fin, deriv);
       }
       if (more) {
@@ -272,7 +286,11 @@ static void launch_pol(int mode, const StageArgs<T, S>& a, int wpb, hipStream_t 
 template <int DIM, typename T, typename S, bool KAHAN>
 static void launch_solver(int kind, int lim, int mode, const StageArgs<T, S>& a, int wpb, hipStream_t st) {
   constexpr bool M = DIM == 2;
-  if (a.pot && (kind == 0 || kind == 1)) {
+  if (a.solid && kind == 0) {
+    launch_pol<SolidRoePolicy<DIM, T>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+  } else if (a.solid && kind == 5) {
+    launch_pol<SolidBurgersPolicy<DIM, T>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
+  } else if (a.pot && (kind == 0 || kind == 1)) {
     // a non-zero gravitational potential: the runtime-limiter Roe form, or HLL
     if (kind == 0) launch_pol<RoePolicy<DIM, T, LIM_RUNTIME, true>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);
     else launch_pol<HLLPolicy<DIM, T, 1, true>, DIM, T, S, KAHAN, false>(mode, a, wpb, st);

@@ -65,15 +65,17 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
   const StageCombine<T, S, KAHAN, N, MODE> comb(a, dtd);
   const long long psz = (long long)sx * sy;
 
-  auto load = [&](int z, Stage& st, Frozen& fz) {
+  auto load = [&](int z, Stage& st, Frozen& fz, bool& solid) {
     const long long e = (long long)z * psz + (long long)yl * sx;  // wave-uniform
     S q0[N];
     T dk[N], u[N];
     comb.load_u0(e, cl, q0);
     comb.load_dk(e, cl, dk);
     comb.stage_state(q0, dk, u);
-    S pot = S(0);
+    S pot = S(0), sol = S(0);
     if constexpr (PolicyPotential<Pol>::value) pot = ldp(a.pot, pl, 0, e, cl);
+    if constexpr (PolicySolid<Pol>::value) sol = ldp(a.solid, pl, 0, e, cl);
+    solid = solid_cell_of<Pol>(sol);
     st = stage_of<Pol>(u, pot, gamma);
     if constexpr (Pol::kFrozenPrims) {
       S pr[N];
@@ -81,7 +83,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
       for (int s = 0; s < N; ++s) pr[s] = ldp(a.prim, pl, s, e, cl);
       fz = Pol::template frozen<S>(pr, gamma);
     } else {
-      fz = frozen_of<Pol>(q0, pot, gamma);
+      fz = frozen_of<Pol>(q0, pot, sol, gamma);
     }
   };
 
@@ -98,13 +100,15 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
     Fz_prev[s] = div_m1[s] = div_m2[s] = T(0);
     fa_m1.dq[s] = fa_m2.dq[s] = T(0);
   }
-  load(z0 - 2, u_prev, f_prev);
+  bool solid_m1, solid_m2 = false;  // solid flags of this row's cells in planes zn-1, zn-2
+  load(z0 - 2, u_prev, f_prev, solid_m1);
 
 #pragma unroll 1
   for (int zn = z0 - 1; zn <= z1 + 1; ++zn) {
     Stage u;
     Frozen fz;
-    load(zn, u, fz);
+    bool solid;
+    load(zn, u, fz, solid);
     const bool out_plane = zn >= z0 && zn < z1;
     const int buf = zn & 1;
 
@@ -192,7 +196,7 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
       if (zn >= z0 + 2 && out_row && out_lane) {
         T deriv[N];
 #pragma unroll
-        for (int s = 0; s < N; ++s) deriv[s] = div_m2[s] - (Fz[s] - Fz_prev[s]) * idz;
+        for (int s = 0; s < N; ++s) deriv[s] = solid_m2 ? T(0) : div_m2[s] - (Fz[s] - Fz_prev[s]) * idz;
         comb.finish((long long)(zn - 2) * psz + (long long)y * sx, c, deriv);
       }
     } else {
@@ -209,6 +213,8 @@ __global__ void __launch_bounds__(64 * W) stage3d_kernel(StageArgs<T, S> a) {
     fa_m2 = fa_m1;
     fa_m1 = fa;
     f_prev = fz;
+    solid_m2 = solid_m1;
+    solid_m1 = solid;
   }
 }
 
@@ -898,8 +904,12 @@ static void launch3_solver(int kind, int lim, int mode, const StageArgs<T, S>& a
     launch3<SRHDRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
   } else if (kind == 4) {
     launch3<MaxwellRoePolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
+  } else if (kind == 5 && a.solid) {
+    launch3<SolidBurgersPolicy<3, T>, T, S, KAHAN, W>(mode, a, st);
   } else if (kind == 5) {
     launch3<BurgersPolicy<3, T, LIM_RUNTIME>, T, S, KAHAN, W>(mode, a, st);
+  } else if (kind == 0 && a.solid) {
+    launch3<SolidRoePolicy<3, T>, T, S, KAHAN, W>(mode, a, st);
   } else if (kind == 0 && a.pot) {
     launch3<RoePolicy<3, T, LIM_RUNTIME, true>, T, S, KAHAN, W>(mode, a, st);
   } else {

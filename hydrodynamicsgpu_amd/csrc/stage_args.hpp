@@ -29,6 +29,7 @@ struct StageArgs {
   const S* u0;          // start-of-step state planes (FP64, or FP32 high words)
   const S* prim;        // SRHD: start-of-step primitive planes (storage precision), else null
   const S* pot;         // gravitational potential plane (policies with kPotential), else null
+  const S* solid;       // solid mask plane (policies with kSolid), else null
   const float* u0lo;    // FP32 low words (compensated storage), or null
   const T* dk;          // d_k planes, null for k == 0
   T* dnext;             // d_{k+1} planes (non-final stage)

@@ -241,7 +241,8 @@ void Solver::resetState() {
   const int n = numStates();
   std::vector<real> stateVec((size_t)getVolume() * n);
   std::vector<real> potentialVec((size_t)getVolume(), 0.0);
-  bool anyPotential = false;
+  std::vector<unsigned char> solidVec((size_t)getVolume(), 0);
+  bool anyPotential = false, anySolid = false;
   std::vector<real> cellResults;
   const int nch = equation->numReadStateChannels();
   int flattenedIndex = 0;
@@ -261,6 +262,9 @@ void Solver::resetState() {
         if ((int)cellResults.size() >= nch + 2) {
           potentialVec[flattenedIndex] = cellResults[cellResults.size() - 2];
           anyPotential = anyPotential || potentialVec[flattenedIndex] != 0.0;
+          // ... and the solid flag the last, converted to char (:69-73)
+          solidVec[flattenedIndex] = static_cast<long long>(cellResults.back()) != 0;
+          anySolid = anySolid || solidVec[flattenedIndex];
         }
       }
     }
@@ -271,6 +275,9 @@ void Solver::resetState() {
   if (desc_.equation == HYDRO_EQN_EULER && (anyPotential || desc_.use_gravity))
     checkStatus(hydro_sim_set_potential(sim, anyPotential ? potentialVec.data() : nullptr, potentialVec.size()),
                 "resetState");
+  // ... and the solid flags (src/Solver/SelfGravitation.cpp:75)
+  if (desc_.equation == HYDRO_EQN_EULER && anySolid)
+    checkStatus(hydro_sim_set_solid(sim, solidVec.data(), solidVec.size()), "resetState");
 }
 
 std::vector<real> Solver::getState() {

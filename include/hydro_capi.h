@@ -211,6 +211,16 @@ int hydro_sim_get_state(hydro_sim* sim, double* aos, size_t n);
 int hydro_sim_set_potential(hydro_sim* sim, const double* pot, size_t n);
 /* The potential (one value per cell, ghost cells included). */
 int hydro_sim_get_potential(hydro_sim* sim, double* pot, size_t n);
+/* Solid cells: the solidBuffer upload of SelfGravitation::resetState
+ * (src/Solver/SelfGravitation.cpp:56-75; one flag per cell, ghost cells
+ * included, nonzero = solid; NULL or all zero clears them).  Read by the
+ * solvers the reference builds with SOLID: EulerRoe (res/EulerRoe.cl:48-65
+ * mirror in the eigenbasis, res/Roe.cl:38-44 CFL, 110-124 and 235-290 mirror
+ * and limiter ratio, res/CalcFluxDeriv.cl:28-30 no derivative in solid cells)
+ * and EulerBurgers (res/EulerBurgers.cl:42-47, 119-127, 209-236, 307-309,
+ * 372-400, 425-447).  EulerHLL/HLLC accept the mask and ignore it, as their
+ * kernels do.  Euler solvers only; the mask stays until replaced. */
+int hydro_sim_set_solid(hydro_sim* sim, const unsigned char* solid, size_t n);
 
 /* SRHD primitives (rho, v, eInt) in the reference layout (src/Solver/SRHDRoe.cpp:9-13).
  * They are stored in the state's precision: FP64, or FP32 for the FP32-mixed

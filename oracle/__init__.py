@@ -60,6 +60,9 @@ def lib():
         L.oracle_rk_tableau.argtypes = [ip, ctypes.POINTER(ip), dp, dp]
         L.oracle_euler_eigenbasis.argtypes = [ip, ip, ctypes.c_double, dp, dp, ctypes.c_double,
                                               ctypes.c_double, dp, dp, dp]
+        L.oracle_euler_eigenbasis_solid.argtypes = [ip, ip, ctypes.c_double, dp, dp, ctypes.c_double,
+                                                    ctypes.c_double, ip, ip, dp, dp, dp]
+        L.oracle_set_solid.argtypes = [vp, ctypes.c_void_p, sz]
         L.oracle_srhd_eigenbasis.argtypes = [ip, ip, ctypes.c_double, dp, dp, dp, dp, dp]
         L.oracle_euler_hll_speeds.argtypes = [ip, ip, ip, ctypes.c_double, dp, dp, dp]
         L.oracle_euler_hll_flux.argtypes = [ip, ip, ip, ctypes.c_double, dp, dp, dp, dp]
@@ -121,6 +124,16 @@ class OracleSim:
             rc = lib().oracle_set_potential(self._h, _dp(a), a.size)
         if rc != 0:
             raise ValueError("bad potential")
+
+    def set_solid(self, solid=None):
+        """solidBuffer: one flag per cell (nonzero = solid), None clears it."""
+        if solid is None:
+            rc = lib().oracle_set_solid(self._h, None, 0)
+        else:
+            a = np.ascontiguousarray(np.asarray(solid).ravel() != 0, dtype=np.uint8)
+            rc = lib().oracle_set_solid(self._h, a.ctypes.data, a.size)
+        if rc != 0:
+            raise ValueError("bad solid mask")
 
     def get_potential(self):
         out = np.empty(self.volume)
@@ -188,6 +201,16 @@ def euler_eigenbasis(dim, side, gamma, qL, qR, potL=0.0, potR=0.0):
     lib().oracle_euler_eigenbasis(dim, side, gamma, _dp(f64(qL)), _dp(f64(qR)), potL, potR,
                                   _dp(ev), _dp(L), _dp(R))
     # column-major m[i + N*j] -> numpy [i, j]
+    return ev, L.reshape(N, N).T.copy(), R.reshape(N, N).T.copy()
+
+
+def euler_eigenbasis_solid(dim, side, gamma, qL, qR, potL=0.0, potR=0.0, solidL=0, solidR=0):
+    N = dim + 2
+    ev = np.zeros(N)
+    L = np.zeros(N * N)
+    R = np.zeros(N * N)
+    lib().oracle_euler_eigenbasis_solid(dim, side, gamma, _dp(f64(qL)), _dp(f64(qR)), potL, potR, int(solidL),
+                                        int(solidR), _dp(ev), _dp(L), _dp(R))
     return ev, L.reshape(N, N).T.copy(), R.reshape(N, N).T.copy()
 
 

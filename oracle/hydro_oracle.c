@@ -14,10 +14,11 @@
  * OpenCL source writes it.  OpenCL min/max on doubles are taken as fmin/fmax
  * (NaN-ignoring), which is what the limiter relies on for r = 0/0.
  *
- * Solid cells (solidBuffer) are not modelled: the buffer is all zero unless a
- * solidFilename image is loaded (src/Solver/SelfGravitation.cpp:56-74), which no
- * supported configuration does.  Self-gravity Poisson relaxation is not
- * modelled; the potential is a fixed array (zero by default).
+ * Solid cells (solidBuffer, src/Solver/SelfGravitation.cpp:17, filled from
+ * the initState's last channel, include/HydroGPU/Solver/SelfGravitationBehavior.h:
+ * 70-74) enter only the two solvers compiled with SOLID: EulerRoe
+ * (src/Solver/EulerRoe.cpp:11-26) and EulerBurgers (src/Solver/EulerBurgers.cpp:
+ * 29-58).  The HLL family and the other equations never read the buffer.
  */
 #include "hydro_oracle.h"
 
@@ -51,6 +52,7 @@ struct oracle_sim {
   double* prim;         /* SRHD primitives */
   double* ifvel;        /* EulerBurgers: interface velocities, one per side and cell */
   double* pressure;     /* EulerBurgers: computePressure's buffer */
+  unsigned char* solid; /* solidBuffer: 1 for a solid cell (NULL: none) */
   double* rk_state[4];
   double* rk_deriv[4];
   double* fe_deriv;
@@ -196,6 +198,17 @@ static double dot4(const double* a, const double* b) {
  * res/include/HydroGPU/Euler.h:6-12 (momentum / density per component). */
 void oracle_euler_eigenbasis(int dim, int side, double gamma, const double* qL, const double* qR,
                              double potL, double potR, double* eigval, double* evl, double* evr) {
+  oracle_euler_eigenbasis_solid(dim, side, gamma, qL, qR, potL, potR, 0, 0, eigval, evl, evr);
+}
+
+/* The same with the cells' solid flags (res/EulerRoe.cl:48-65): a solid cell
+ * facing a fluid one takes the fluid cell's density and velocity with
+ * velocity component side+1 of the 4-vector (vx, vy, vz, 0) negated -- the
+ * tangential y for x-faces, z for y-faces, the padding for z-faces -- while
+ * its total energy and potential stay its own. */
+void oracle_euler_eigenbasis_solid(int dim, int side, double gamma, const double* qL, const double* qR,
+                                   double potL, double potR, int solidL, int solidR, double* eigval,
+                                   double* evl, double* evr) {
   const int N = dim + 2;
   const int E = dim + 1;
   double densityL = qL[0], densityR = qR[0];
@@ -203,6 +216,16 @@ void oracle_euler_eigenbasis(int dim, int side, double gamma, const double* qL, 
   for (int a = 0; a < dim; ++a) {
     velocityL[a] = qL[1 + a] / qL[0];
     velocityR[a] = qR[1 + a] / qR[0];
+  }
+  if (solidL && !solidR) {
+    densityL = densityR;
+    for (int a = 0; a < 4; ++a) velocityL[a] = velocityR[a];
+    velocityL[side + 1] = -velocityR[side + 1];
+  }
+  if (solidR && !solidL) {
+    densityR = densityL;
+    for (int a = 0; a < 4; ++a) velocityR[a] = velocityL[a];
+    velocityR[side + 1] = -velocityL[side + 1];
   }
   double invDensityL = 1. / densityL;
   double energyTotalL = qL[E] * invDensityL;
@@ -749,6 +772,7 @@ void oracle_destroy(oracle_sim* s) {
   if (!s) return;
   free(s->state); free(s->eigval); free(s->eigvec); free(s->dqt); free(s->flux);
   free(s->dtbuf); free(s->potential); free(s->pot_next); free(s->prim); free(s->fe_deriv); free(s->ifvel); free(s->pressure);
+  free(s->solid);
   for (int i = 0; i < 4; ++i) { free(s->rk_state[i]); free(s->rk_deriv[i]); }
   free(s);
 }
@@ -798,6 +822,24 @@ int oracle_set_potential(oracle_sim* s, const double* pot, size_t n) {
   for (int c = 0; c < V; ++c) s->state[(size_t)s->N * c + s->dim + 1] += phi0[c];
   free(phi0);
   return 0;
+}
+
+/* SelfGravitation::resetState writes solidBuffer (src/Solver/SelfGravitation.cpp:75) */
+int oracle_set_solid(oracle_sim* s, const unsigned char* solid, size_t n) {
+  if (solid && n != (size_t)s->vol) return -1;
+  free(s->solid);
+  s->solid = NULL;
+  if (!solid) return 0;
+  s->solid = (unsigned char*)malloc((size_t)s->vol);
+  for (int c = 0; c < s->vol; ++c) s->solid[c] = solid[c] != 0;
+  return 0;
+}
+
+/* the solid flag of a cell as the running solver's kernels see it: only the
+ * SOLID builds (EulerRoe, EulerBurgers) read solidBuffer */
+static inline int solid_at(const oracle_sim* s, int c) {
+  return s->solid && s->d.equation == HYDRO_EQN_EULER &&
+         (s->d.solver == HYDRO_SOLVER_ROE || s->d.solver == HYDRO_SOLVER_BURGERS) && s->solid[c];
 }
 
 int oracle_get_potential(const oracle_sim* s, double* pot, size_t n) {
@@ -963,8 +1005,9 @@ static void calc_eigen_basis(oracle_sim* s) {
       double* ev = s->eigval + N * iface;
       if (s->d.equation == HYDRO_EQN_EULER) {
         double* evl = s->eigvec + s->ev_size * iface;
-        oracle_euler_eigenbasis(D, side, s->gamma, s->state + N * prev, s->state + N * c,
-                                s->potential[prev], s->potential[c], ev, evl, evl + N * N);
+        oracle_euler_eigenbasis_solid(D, side, s->gamma, s->state + N * prev, s->state + N * c,
+                                      s->potential[prev], s->potential[c], solid_at(s, prev), solid_at(s, c), ev,
+                                      evl, evl + N * N);
       } else if (s->d.equation == HYDRO_EQN_SRHD) {
         double* evl = s->eigvec + s->ev_size * iface;
         oracle_srhd_eigenbasis(D, side, s->gamma, s->prim + N * prev, s->prim + N * c, ev, evl, evl + N * N);
@@ -990,6 +1033,11 @@ static void roe_calc_cell_timestep(oracle_sim* s) {
       continue;
     }
     for (int side = 0; side < D; ++side) {
+      /* res/Roe.cl:38-44: no limit across a side that touches a solid cell */
+      if (solid_at(s, c) || solid_at(s, c + s->step[side])) {
+        s->dtbuf[side + D * c] = INFINITY;
+        continue;
+      }
       const double* evL = s->eigval + (size_t)N * (side + D * c);
       const double* evR = s->eigval + (size_t)N * (side + D * (c + s->step[side]));
       double maxLambda = fmax(0., evL[N - 1]);
@@ -1010,6 +1058,11 @@ static void hll_calc_cell_timestep(oracle_sim* s) {
     if (!active(s, i, 1)) { s->dtbuf[c] = INFINITY; continue; }
     double result = INFINITY;
     for (int side = 0; side < D; ++side) {
+      /* res/Roe.cl:38-44: no limit across a side that touches a solid cell */
+      if (solid_at(s, c) || solid_at(s, c + s->step[side])) {
+        s->dtbuf[side + D * c] = INFINITY;
+        continue;
+      }
       const double* evL = s->eigval + (size_t)N * (side + D * c);
       const double* evR = s->eigval + (size_t)N * (side + D * (c + s->step[side]));
       double minLambda = fmin(0., evR[0]);
@@ -1060,6 +1113,24 @@ static const double* eigvec_at(oracle_sim* s, size_t iface) {
   return s->eigvec + s->ev_size * iface;
 }
 
+/* res/Roe.cl:110-124 and 235-248: across a face between a solid and a fluid
+ * cell the solid side takes the fluid cell's state with the normal momentum
+ * negated.  qL/qR are redirected to the copies mL/mR when mirrored. */
+static void mirror_wall(const oracle_sim* s, int l, int r, int side, const double** qL, const double** qR,
+                        double* mL, double* mR) {
+  const int N = s->N;
+  const int sl = solid_at(s, l), sr = solid_at(s, r);
+  if (sl && !sr) {
+    for (int k = 0; k < N; ++k) mL[k] = (*qR)[k];
+    mL[side + 1] = -mL[side + 1];
+    *qL = mL;
+  } else if (sr && !sl) {
+    for (int k = 0; k < N; ++k) mR[k] = (*qL)[k];
+    mR[side + 1] = -mR[side + 1];
+    *qR = mR;
+  }
+}
+
 /* res/Roe.cl:77-168 */
 static void calc_delta_q_tilde(oracle_sim* s) {
   const int N = s->N, D = s->dim, V = s->vol;
@@ -1074,8 +1145,10 @@ static void calc_delta_q_tilde(oracle_sim* s) {
       size_t iface = (size_t)side + D * c;
       const double* ev = eigvec_at(s, iface);
       double* dq = s->dqt + N * iface;
+      double mL[6], mR[6];
       const double* qL = s->state + (size_t)N * prev;
       const double* qR = s->state + (size_t)N * c;
+      mirror_wall(s, prev, c, side, &qL, &qR, mL, mR);
       if (separate) {
         double tl[6], tr[6];
         left_transform(s, tl, ev, qL, side);
@@ -1113,6 +1186,11 @@ static void roe_calc_flux(oracle_sim* s, double dt) {
       const double* ev = eigvec_at(s, iface);
       const double* qL = s->state + (size_t)N * indexL;
       const double* qR = s->state + (size_t)N * c;
+      double mL[6], mR[6];
+      mirror_wall(s, indexL, c, side, &qL, &qR, mL, mR);
+      /* res/Roe.cl:249-250, 278-290: a solid cell two over sets the ratio to 1 */
+      const int solidL2 = solid_at(s, indexL - s->step[side]);
+      const int solidR2 = solid_at(s, indexR2);
       double fluxTilde[6];
       if (separate) {
         double tl[6], tr[6];
@@ -1128,8 +1206,15 @@ static void roe_calc_flux(oracle_sim* s, double dt) {
         double eigenvalue = eigenvalues[k];
         fluxTilde[k] *= eigenvalue;
         double rTilde, theta;
-        if (eigenvalue >= 0.) { rTilde = dqL[k] / dq[k]; theta = 1.; }
-        else { rTilde = dqR[k] / dq[k]; theta = -1.; }
+        if (eigenvalue >= 0.) {
+          rTilde = dqL[k] / dq[k];
+          theta = 1.;
+          if (solidL2) rTilde = 1.;
+        } else {
+          rTilde = dqR[k] / dq[k];
+          theta = -1.;
+          if (solidR2) rTilde = 1.;
+        }
         double phi = oracle_slope_limiter(lim, rTilde);
         double epsilon = eigenvalue * dt_dx;
         double deltaFluxTilde = eigenvalue * dq[k];
@@ -1148,6 +1233,7 @@ static void calc_flux_deriv(oracle_sim* s, double* deriv) {
     int i[3];
     decode(s, c, i);
     if (!active(s, i, 2)) continue;
+    if (solid_at(s, c)) continue; /* res/CalcFluxDeriv.cl:28-30 */
     double* dv = deriv + (size_t)N * c;
     for (int side = 0; side < D; ++side) {
       size_t iface = (size_t)side + D * c;
@@ -1226,7 +1312,7 @@ static void burgers_calc_cell_timestep(oracle_sim* s) {
   for (int c = 0; c < V; ++c) {
     int i[3];
     decode(s, c, i);
-    if (!active(s, i, 2)) { s->dtbuf[c] = INFINITY; continue; }
+    if (!active(s, i, 2) || solid_at(s, c)) { s->dtbuf[c] = INFINITY; continue; } /* EulerBurgers.cl:42-47 */
     double vel[3];
     double specificEnergyInternal = burgers_eint(s, s->state + (size_t)N * c, s->potential[c], vel);
     double speedOfSound = sqrt(gamma * (gamma - 1.) * specificEnergyInternal);
@@ -1251,6 +1337,10 @@ static void burgers_calc_interface_velocity(oracle_sim* s) {
       double velocityL = s->state[side + 1 + (size_t)N * l] / densityL;
       double densityR = s->state[(size_t)N * c];
       double velocityR = s->state[side + 1 + (size_t)N * c] / densityR;
+      /* res/EulerBurgers.cl:119-127 */
+      const int solidL = solid_at(s, l), solidR = solid_at(s, c);
+      if (solidL && !solidR) velocityL = -velocityR;
+      else if (solidR && !solidL) velocityR = -velocityL;
       s->ifvel[side + D * c] = .5 * (velocityL + velocityR);
     }
   }
@@ -1271,11 +1361,31 @@ static void burgers_calc_flux(oracle_sim* s, double dt) {
       size_t iface = (size_t)side + D * c;
       double interfaceVelocity = s->ifvel[iface];
       double* flux = s->flux + (size_t)N * iface;
+      const int solidL2 = solid_at(s, iL2), solidL = solid_at(s, iL);
+      const int solidR = solid_at(s, iR), solidR2 = solid_at(s, iR2);
       for (int j = 0; j < N; ++j) {
         double stateR2 = s->state[j + (size_t)N * iR2];
         double stateR = s->state[j + (size_t)N * iR];
         double stateL = s->state[j + (size_t)N * iL];
         double stateL2 = s->state[j + (size_t)N * iL2];
+        /* res/EulerBurgers.cl:209-236: a solid neighbour takes the fluid
+         * state, the normal momentum kept and every other state negated; a
+         * solid cell two over repeats the near state, normal momentum negated */
+        const int normal = j == 1 + side;
+        if (solidL && solidR) {
+        } else if (solidL) {
+          stateL = normal ? stateR : -stateR;
+        } else if (solidR) {
+          stateR = normal ? stateL : -stateL;
+        }
+        if (solidL2) {
+          stateL2 = stateL;
+          if (normal) stateL2 = -stateL2;
+        }
+        if (solidR2) {
+          stateR2 = stateR;
+          if (normal) stateR2 = -stateR2;
+        }
         double deltaStateL = stateL - stateL2;
         double deltaState = stateR - stateL;
         double deltaStateR = stateR2 - stateR;
@@ -1305,7 +1415,7 @@ static void burgers_compute_pressure(oracle_sim* s) {
     decode(s, c, i);
     int in = 1;
     for (int a = 0; a < D; ++a) in &= i[a] >= 1 && i[a] < s->sz[a] - 1;
-    if (!in) continue;
+    if (!in || solid_at(s, c)) continue; /* EulerBurgers.cl:307-309: the buffer keeps its old value */
     double vel[3];
     const double* st = s->state + (size_t)N * c;
     double specificEnergyInternal = burgers_eint(s, st, s->potential[c], vel);
@@ -1320,11 +1430,14 @@ static void burgers_diffuse_momentum(oracle_sim* s, double* deriv) {
   for (int c = 0; c < V; ++c) {
     int i[3];
     decode(s, c, i);
-    if (!active(s, i, 2)) continue;
+    if (!active(s, i, 2) || solid_at(s, c)) continue;
     double* dv = deriv + (size_t)N * c;
     for (int side = 0; side < D; ++side) {
       double pressureL = s->pressure[c - s->step[side]];
       double pressureR = s->pressure[c + s->step[side]];
+      /* EulerBurgers.cl:372-375: a solid neighbour reads as this cell's pressure */
+      if (solid_at(s, c - s->step[side])) pressureL = s->pressure[c];
+      if (solid_at(s, c + s->step[side])) pressureR = s->pressure[c];
       dv[1 + side] -= .5 * (pressureR - pressureL) / s->kdx[side];
     }
   }
@@ -1339,7 +1452,7 @@ static void burgers_diffuse_work(oracle_sim* s, double* deriv) {
   for (int c = 0; c < V; ++c) {
     int i[3];
     decode(s, c, i);
-    if (!active(s, i, 2)) continue;
+    if (!active(s, i, 2) || solid_at(s, c)) continue;
     double* dv = deriv + (size_t)N * c;
     double deltaEnergyTotal = 0.;
     for (int side = 0; side < D; ++side) {
@@ -1348,6 +1461,12 @@ static void burgers_diffuse_work(oracle_sim* s, double* deriv) {
       double velocityR = s->state[1 + side + (size_t)N * r] / s->state[(size_t)N * r];
       double pressureL = s->pressure[l];
       double pressureR = s->pressure[r];
+      /* EulerBurgers.cl:442-447: a solid neighbour's own velocity negated,
+       * and this cell's pressure */
+      if (solid_at(s, l)) velocityL = -velocityL;
+      if (solid_at(s, r)) velocityR = -velocityR;
+      if (solid_at(s, l)) pressureL = s->pressure[c];
+      if (solid_at(s, r)) pressureR = s->pressure[c];
       deltaEnergyTotal -= .5 * (pressureR * velocityR - pressureL * velocityL) / s->kdx[side];
     }
     dv[D + 1] += deltaEnergyTotal;

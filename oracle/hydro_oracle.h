@@ -32,6 +32,9 @@ int oracle_get_state(const oracle_sim* s, double* aos, size_t n);
 int oracle_get_primitives(const oracle_sim* s, double* aos, size_t n);
 int oracle_set_potential(oracle_sim* s, const double* pot, size_t n);
 int oracle_get_potential(const oracle_sim* s, double* pot, size_t n);
+/* solidBuffer (n = volume, nonzero = solid); NULL clears it.  Read by the
+ * EulerRoe and EulerBurgers updates only. */
+int oracle_set_solid(oracle_sim* s, const unsigned char* solid, size_t n);
 void oracle_boundary(oracle_sim* s);
 void oracle_init_step(oracle_sim* s);
 double oracle_calc_timestep(oracle_sim* s);
@@ -48,6 +51,10 @@ int oracle_rk_tableau(int integrator, int* order, double* alphas /*4x4*/, double
  * outputs eigval[N], evl[N*N], evr[N*N] (column-major m[i + N*j]) */
 void oracle_euler_eigenbasis(int dim, int side, double gamma, const double* qL, const double* qR,
                              double potL, double potR, double* eigval, double* evl, double* evr);
+/* the same with the two cells' solid flags (the wall mirror) */
+void oracle_euler_eigenbasis_solid(int dim, int side, double gamma, const double* qL, const double* qR,
+                                   double potL, double potR, int solidL, int solidR, double* eigval,
+                                   double* evl, double* evr);
 /* SRHD eigenbasis of one interface from primitives (rho, v[dim], eInt) */
 void oracle_srhd_eigenbasis(int dim, int side, double gamma, const double* primL, const double* primR,
                             double* eigval, double* evl, double* evr);

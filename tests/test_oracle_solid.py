@@ -1,0 +1,274 @@
+"""Pins of the oracle's solid cells (the SOLID builds of EulerRoe and
+EulerBurgers: res/EulerRoe.cl:48-65, res/Roe.cl:38-44, 110-124, 235-290,
+res/CalcFluxDeriv.cl:28-30, res/EulerBurgers.cl:42-47, 119-127, 209-236,
+307-309, 372-400, 425-447).
+
+Expected values come from independent statements of what the reference's
+code fixes:
+
+  * the mirrored eigenbasis equals the plain eigenbasis of the substituted
+    left state (the fluid cell's density and velocity, velocity component
+    side+1 of (vx, vy, vz, 0) negated, the solid cell's own total energy),
+    and with equal energies it has closed forms: the flipped tangential
+    component averages to exactly 0, the eigenvalues are v_n, v_n -+ c with
+    c^2 = (gamma-1)(H - |v_avg|^2/2) from the fluid cell's enthalpy (its own
+    sound speed where nothing is flipped), and in 1D nothing is flipped at
+    all (the face speed is v, not 0 as a physical mirror would give);
+  * a uniform state at rest stays exactly at rest with any solid mask;
+  * solid cells take no derivative: without gravity their states never
+    change;
+  * the limiter ratio of a face whose cell two over (upwind) is solid is 1:
+    after one fixed-dt ForwardEuler update (with the eigenbasis of the
+    step's state, which the reference skips under a fixed dt and which the
+    oracle then computes on request), the cell two beyond a solid cell
+    does not depend on the solid cell's state at all, although the face
+    next to the solid cell (whose deltaQTilde the ratio would otherwise read)
+    depends on the solid cell's energy;
+  * the fluid cells do not see the solid cells' density and momentum (Roe:
+    the mirror takes them from the fluid side, the CFL skips solid sides)
+    or, for EulerBurgers with solid cells at rest, their density and energy
+    (zero face velocity at walls, the pressure of the fluid cell, the
+    solid neighbour's zero velocity in the work pass).
+Each pin fails for a plausible slip: a physical mirror (vx negated) or the
+fluid cell's energy in the basis, a missing derivative skip, no ratio
+override, a solid side left in the CFL minimum.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hydrodynamicsgpu_amd import config, ics
+
+GAMMA = 1.4
+
+
+def _desc(size, solver="EulerRoe", bm="FREEFLOW", **kw):
+    kw.setdefault("xmin", [0.0, 0.0, 0.0])
+    kw.setdefault("xmax", [1.0, 1.0, 1.0])
+    return config.make_desc(size, solverName=solver, boundaryMethods=[{"min": bm, "max": bm}] * 3, **kw)
+
+
+def _state(d, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z, _ = ics.grid(d)
+    rho = 1.0 + 0.2 * rng.random(x.shape)
+    vx = 0.3 * (rng.random(x.shape) - 0.5)
+    vy = 0.3 * (rng.random(x.shape) - 0.5) if d.dim > 1 else None
+    vz = 0.3 * (rng.random(x.shape) - 0.5) if d.dim > 2 else None
+    p = 1.0 + 0.2 * rng.random(x.shape)
+    return ics.build_state_euler(d, rho, vx, vy, vz, p)
+
+
+def _mask(d, seed, frac=0.15):
+    rng = np.random.default_rng(seed)
+    return (rng.random(config.volume(d)) < frac).astype(np.uint8)
+
+
+def _cells(d, q):
+    return q.reshape(-1, d.dim + 2)
+
+
+def _interior(d, g=2):
+    """cells whose every coordinate lies in g .. size-1-g (g = 2: the cells
+    the boundary does not rewrite)"""
+    ok = np.ones(config.volume(d), bool)
+    idx = np.indices([d.size[a] for a in range(d.dim)][::-1])
+    for a in range(d.dim):
+        i = idx[d.dim - 1 - a].ravel()
+        ok &= (i >= g) & (i <= d.size[a] - 1 - g)
+    return ok
+
+
+@pytest.mark.parametrize("dim,side", [(1, 0), (2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])
+def test_mirrored_basis_is_the_substituted_plain_basis(dim, side):
+    rng = np.random.default_rng(10 * dim + side)
+    N = dim + 2
+    rhoL, rhoR = 1.3, 0.8
+    vL, vR = rng.random(dim) - 0.5, rng.random(dim) - 0.5
+    qL = np.r_[rhoL, rhoL * vL, 2.9]
+    qR = np.r_[rhoR, rhoR * vR, 2.1]
+    potL, potR = 0.05, -0.02
+    ev, L, R = oracle.euler_eigenbasis_solid(dim, side, GAMMA, qL, qR, potL, potR, 1, 0)
+    v = qR[1:1 + dim] / qR[0]
+    if side + 1 < dim:
+        v[side + 1] = -v[side + 1]
+    qsub = np.r_[qR[0], qR[0] * v, qL[N - 1]]
+    ev2, L2, R2 = oracle.euler_eigenbasis(dim, side, GAMMA, qsub, qR, potL, potR)
+    np.testing.assert_allclose(ev, ev2, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(L, L2, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(R, R2, rtol=1e-13, atol=1e-14)
+    # and symmetrically for a solid right cell
+    evr, _, _ = oracle.euler_eigenbasis_solid(dim, side, GAMMA, qL, qR, potL, potR, 0, 1)
+    v = qL[1:1 + dim] / qL[0]
+    if side + 1 < dim:
+        v[side + 1] = -v[side + 1]
+    evr2, _, _ = oracle.euler_eigenbasis(dim, side, GAMMA, qL, np.r_[qL[0], qL[0] * v, qR[N - 1]], potL, potR)
+    np.testing.assert_allclose(evr, evr2, rtol=1e-14, atol=1e-15)
+    # two solid cells (or two fluid ones): the plain basis
+    ev3, _, _ = oracle.euler_eigenbasis_solid(dim, side, GAMMA, qL, qR, potL, potR, 1, 1)
+    ev4, _, _ = oracle.euler_eigenbasis(dim, side, GAMMA, qL, qR, potL, potR)
+    np.testing.assert_array_equal(ev3, ev4)
+
+
+@pytest.mark.parametrize("dim,side", [(1, 0), (2, 0), (2, 1), (3, 1), (3, 2)])
+def test_mirrored_basis_closed_form_with_equal_energies(dim, side):
+    rho, E = 1.1, 2.5
+    v = np.array([0.31, -0.22, 0.17][:dim])
+    qR = np.r_[rho, rho * v, E]
+    qL = np.r_[7.0, 0.4 * np.ones(dim), E]  # the solid cell: only its energy counts
+    ev, L, R = oracle.euler_eigenbasis_solid(dim, side, GAMMA, qL, qR, 0.0, 0.0, 1, 0)
+    # the averaged velocity: component side+1 (if any) exactly 0, the rest v;
+    # both cells' enthalpy H = (E + p) / rho, so c^2 = (gamma-1)(H - |vavg|^2/2)
+    vavg = v.copy()
+    if side + 1 < dim:
+        vavg[side + 1] = 0.0
+    p = (GAMMA - 1) * (E - 0.5 * rho * v @ v)
+    H = (E + p) / rho
+    c = np.sqrt((GAMMA - 1) * (H - 0.5 * vavg @ vavg))
+    if side + 1 >= dim:
+        assert c == pytest.approx(np.sqrt(GAMMA * p / rho), rel=1e-14)  # the fluid cell's own sound speed
+    vn = v[side]
+    np.testing.assert_allclose(ev, np.r_[vn - c, [vn] * dim, vn + c], rtol=1e-13)
+    # R column 1 (the entropy wave, rotated back) is (1, vavg, |vavg|^2 / 2)
+    np.testing.assert_allclose(R[:, 1], np.r_[1.0, vavg, 0.5 * vavg @ vavg], rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("solver", ["EulerRoe", "EulerBurgers"])
+@pytest.mark.parametrize("size", [[40], [24, 20], [12, 10, 14]])
+def test_rest_state_stays_at_rest(solver, size):
+    d = _desc(size, solver, integratorName="RungeKutta3TVD", cfl=0.4)
+    x, y, z, _ = ics.grid(d)
+    q = ics.build_state_euler(d, 1.2 + 0 * x, 0 * x, 0 * x if d.dim > 1 else None, 0 * x if d.dim > 2 else None,
+                              0.9 + 0 * x)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    o.set_solid(_mask(d, 1, 0.25))
+    for _ in range(4):
+        o.update()
+    np.testing.assert_array_equal(o.get_state(), q)
+
+
+@pytest.mark.parametrize("solver", ["EulerRoe", "EulerBurgers"])
+@pytest.mark.parametrize("size", [[40], [24, 20], [12, 10, 14]])
+def test_solid_cells_never_change(solver, size):
+    d = _desc(size, solver, "PERIODIC", integratorName="RungeKutta2", cfl=0.4)
+    q = _state(d, 2)
+    m = _mask(d, 3)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    o.set_solid(m)
+    for _ in range(3):
+        o.update()
+    out = _cells(d, o.get_state())
+    inner = _interior(d)
+    sel = (m == 1) & inner
+    assert sel.sum() > 0
+    np.testing.assert_array_equal(out[sel], _cells(d, q)[sel])
+    assert not np.array_equal(out[(m == 0) & inner], _cells(d, q)[(m == 0) & inner])
+
+
+def test_solid_two_over_sets_the_ratio_to_one():
+    n, k = 24, 9  # solid cell k; cell k+2 sees it only through the ratio of face k+1
+    d = _desc([n], "EulerRoe", integratorName="ForwardEuler", useFixedDT=True, fixedDT=0.002)
+    q = _state(d, 4)
+    m = np.zeros(n, np.uint8)
+    m[k] = 1
+    out = []
+    for E in (2.0, 5.0):
+        qq = _cells(d, q.copy())
+        qq[k, 2] = E
+        o = oracle.OracleSim(d, opts=oracle.ORACLE_OPT_FRESH_BASIS_FIXED_DT)
+        o.set_state(qq.ravel())
+        o.set_solid(m)
+        o.update()
+        out.append(_cells(d, o.get_state()))
+    np.testing.assert_array_equal(out[0][k + 2:], out[1][k + 2:])
+    assert not np.array_equal(out[0][k + 1], out[1][k + 1])  # the wall face itself reads the energy
+    # without the override the cell two over would follow the energy: the
+    # same grid with the solid cell made fluid changes cell k+2 with it
+    ref = []
+    for E in (2.0, 5.0):
+        qq = _cells(d, q.copy())
+        qq[k, 2] = E
+        o = oracle.OracleSim(d, opts=oracle.ORACLE_OPT_FRESH_BASIS_FIXED_DT)
+        o.set_state(qq.ravel())
+        o.update()
+        ref.append(_cells(d, o.get_state()))
+    assert not np.array_equal(ref[0][k + 2], ref[1][k + 2])
+
+
+@pytest.mark.parametrize("solver,size", [("EulerRoe", [40]), ("EulerRoe", [24, 22]), ("EulerRoe", [12, 10, 14]),
+                                         ("EulerBurgers", [40]), ("EulerBurgers", [24, 22]),
+                                         ("EulerBurgers", [12, 10, 14])])
+def test_fluid_cells_ignore_what_the_mirror_replaces(solver, size):
+    d = _desc(size, solver, integratorName="RungeKutta3TVD", cfl=0.4)
+    q = _cells(d, _state(d, 5))
+    # solid cells off the boundary's source cells (FREEFLOW copies cells 2
+    # and size-3 into fluid ghost cells, which would carry a solid state)
+    m = _mask(d, 6) * _interior(d, 3)
+    E = d.dim + 1
+    rng = np.random.default_rng(7)
+    outs = []
+    for variant in range(2):
+        qq = q.copy()
+        ns = int(m.sum())
+        qq[m == 1, 0] = 1.0 + 3.0 * rng.random(ns)  # density
+        if solver == "EulerRoe":
+            qq[m == 1, 1:E] = 5.0 * (rng.random((ns, d.dim)) - 0.5)  # momentum (fast solid cells)
+        else:
+            qq[m == 1, 1:E] = 0.0  # at rest; the energy varies instead
+            qq[m == 1, E] = 2.0 + rng.random(ns)
+        o = oracle.OracleSim(d)
+        o.set_state(qq.ravel())
+        o.set_solid(m)
+        dts = [o.update() for _ in range(3)]
+        outs.append((_cells(d, o.get_state()), dts))
+    np.testing.assert_array_equal(outs[0][0][m == 0], outs[1][0][m == 0])
+    assert outs[0][1] == outs[1][1]
+
+
+def test_solid_mask_is_ignored_by_hll():
+    d = _desc([24, 20], "EulerHLLC", integratorName="RungeKutta2", cfl=0.4)
+    q = _state(d, 8)
+    a = oracle.OracleSim(d)
+    a.set_state(q)
+    b = oracle.OracleSim(d)
+    b.set_state(q)
+    b.set_solid(_mask(d, 9))
+    for _ in range(3):
+        a.update()
+        b.update()
+    np.testing.assert_array_equal(a.get_state(), b.get_state())
+
+
+def test_burgers_work_pass_negates_the_solid_neighbours_velocity():
+    """res/EulerBurgers.cl:442-447: a solid neighbour enters the work pass
+    with its own velocity negated and this cell's pressure.  The solid cell's
+    momentum reaches the fluid cell next to it through that term only (the
+    advection face is a wall, the momentum pass reads this cell's pressure),
+    so flipping the solid momentum from +m to -m changes the fluid cell's
+    energy after one fixed-dt ForwardEuler update by
+    E(+m) - E(-m) = -(dt / kdx) .5 (p (-v) - p v) = dt p v / kdx, v = m / rho_s,
+    p > 0 the cell's pressure; without the negation the sign flips."""
+    n, k = 20, 12
+    d = _desc([n], "EulerBurgers", integratorName="ForwardEuler", useFixedDT=True, fixedDT=0.001)
+    q = _state(d, 11)
+    m = np.zeros(n, np.uint8)
+    m[k] = 1
+    E = []
+    for sign in (1.0, -1.0):
+        qq = _cells(d, q.copy())
+        qq[k, 1] = sign * 0.6
+        o = oracle.OracleSim(d)
+        o.set_state(qq.ravel())
+        o.set_solid(m)
+        o.update()
+        out = _cells(d, o.get_state())
+        E.append((out[k - 1, 2], out[k + 1, 2]))
+    v = 0.6 / q.reshape(-1, 3)[k, 0]
+    kdx = oracle.kernel_dx(1.0 / n)
+    # the cell left of the solid one sees it as its upper neighbour (+), the
+    # cell right of it as its lower neighbour (-)
+    p_left = (E[0][0] - E[1][0]) * kdx / (0.001 * v)
+    p_right = -(E[0][1] - E[1][1]) * kdx / (0.001 * v)
+    assert 0.8 < p_left < 1.5 and 0.8 < p_right < 1.5  # the cells' pressures (1 .. 1.2 initially)
