@@ -157,6 +157,20 @@ class Sim:
             a = np.ascontiguousarray(pot, dtype=np.float64).ravel()
             check(lib().hydro_sim_set_potential(self._h, _dp(a), a.size))
 
+    def reset_state(self, state, potential=None, solid=None):
+        """Solver::resetState's upload (src/Solver/Solver.cpp:351-392 and, for
+        the Euler solvers, SelfGravitation::resetState): the state, then the
+        potential channel (added into the energy; relaxed from -rho with
+        useGravity) and the solid channel, when the entry gives them."""
+        self.set_state(state)
+        if self.desc.equation != config.EQUATIONS["Euler"]:
+            return
+        pot = None if potential is None else np.asarray(potential, dtype=np.float64).ravel()
+        if (pot is not None and np.any(pot != 0)) or self.desc.use_gravity:
+            self.set_potential(pot)
+        if solid is not None and np.any(np.asarray(solid) != 0):
+            self.set_solid(solid)
+
     def set_solid(self, solid=None):
         """The solid flag per cell (nonzero = solid; None clears them), read
         by EulerRoe and EulerBurgers."""

@@ -10,16 +10,16 @@ outputs Euler::readStateCell / SRHD::readStateCell reduce to
 src/Equation/SRHD.cpp:78-93); total energy includes the magnetic term
 |B|^2 / (2 mu0) whenever an entry sets a field, as there.
 
-Entries served: every Euler/SRHD/Maxwell entry whose state needs neither
-solid cells nor self-gravity nor an external potential:
+Entries served: every Euler/SRHD/Maxwell entry:
   Constant, 1D Advect Wave, Sphere, Sod, Configuration 1..6, Sedov,
   Brio-Wu, Orszag-Tang, Colella-Woodward, Kelvin-Helmholtz,
   Double Mach Reflection, Square Cavity, Shock Bubble Interaction,
-  Relativistic Blast Wave Interaction, Marti & Muller 2008 Problem #1/#2,
-  Relativistic Jet, Maxwell-1.
-Left out: Flow Around Cylinder and Forward Facing Step (solid cells),
-Spiral Implosion and the self-gravitation tests (self-gravity),
-Rayleigh-Taylor (external potential), the numerical-relativity entries.
+  Flow Around Cylinder and Forward Facing Step (solid cells), Spiral
+  Implosion, Rayleigh-Taylor (an external potential), self-gravitation
+  tests 1, 1 spinning, 2 and 4 (useGravity), Relativistic Blast Wave
+  Interaction, Marti & Muller 2008 Problem #1/#2, Relativistic Jet,
+  Maxwell-1.
+Left out: the numerical-relativity entries (ADM/BSSNOK equations).
 "Double Mach Reflection" is the second entry of that name: the reference's
 by-name map is filled in list order (res/initConds.lua, end of file), so the
 later one wins.
@@ -27,10 +27,13 @@ later one wins.
 Usage: `setup(name, cfg)` applies an entry's setup to a config dict (keys as
 config.make_desc takes them) and returns (cfg, state_fn); `state_fn(desc)`
 gives the AoS state.  luaconfig runs the same setups on a Lua config's
-globals.  No entry draws random numbers (buildStateEuler's crand() noise is
-multiplied by a zero default in every served entry).
+globals.  Rayleigh-Taylor and the self-gravitation tests draw crand() noise
+(seeded here, see _crand); no other entry draws random numbers.
+`channels_function(name)` also gives the potential and solid channels, and
+a setup may set cfg["useGravity"].
 """
 import math
+import zlib
 
 import numpy as np
 
@@ -312,6 +315,142 @@ def _st_maxwell1(desc, gamma):
     return np.stack([c.reshape(-1) for c in comps], axis=1).reshape(-1).copy()
 
 
+# ---------------------------------------------------------------- solid cells, potentials, self-gravity
+# These entries also fill the potential and solid channels of buildStateEuler
+# (res/util.lua:63-79); their state functions return (state, potential or
+# None, solid or None).  crand() (res/util.lua:6, Lua's unseeded
+# math.random) is drawn here from a generator seeded by the entry's name,
+# so the noise is reproducible but not the draws of any reference run.
+
+def _crand(name, shape, count):
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    return [rng.random(shape) * 2 - 1 for _ in range(count)]
+
+
+def _su_cylinder(cfg):
+    _set_bm(cfg, ("NONE", "FREEFLOW"), FREE, FREE)
+
+
+def _st_cylinder(desc, gamma):
+    x, y, z = _centres(desc)
+    solid = (x * x + y * y + z * z < .25 * .25).astype(np.uint8)
+    return _euler_state(desc, gamma, 1.0, pressure=1.0, vx=.2), None, solid.reshape(-1)
+
+
+def _su_step(cfg):
+    _set_bm(cfg, FREE, ("MIRROR", "MIRROR"), ("MIRROR", "MIRROR"))
+
+
+def _st_step(desc, gamma):
+    x, y, _ = _centres(desc)
+    xmid = [.5 * (desc.xmin[a] + desc.xmax[a]) for a in range(3)]
+    inside = (x > xmid[0]) & (y < xmid[1])
+    return _euler_state(desc, gamma, 1.4, pressure=1.0, vx=3.0), None, inside.astype(np.uint8).reshape(-1)
+
+
+def _su_mirror(cfg):
+    _set_bm(cfg, ("MIRROR", "MIRROR"), ("MIRROR", "MIRROR"), ("MIRROR", "MIRROR"))
+
+
+def _st_spiral(desc, gamma):
+    # the entry's `useGravity = true` runs inside initState, after the app
+    # has read useGravity (src/HydroGPUApp.cpp:207), so it changes nothing
+    x, y, z = _centres(desc)
+    s = np.where(np.sqrt(x * x + y * y + z * z) > .25, -1.0, 1.0)
+    return _euler_state(desc, gamma, 1.1 - np.exp(-x * x - y * y), eint=1.0, vx=-y * s, vy=x * s)
+
+
+def _su_rt(cfg):
+    size = [int(v) for v in cfg["size"]]
+    k = len(size) - 1  # the last axis of the size table (`#size`)
+    xmin = list(cfg.get("xmin") or [-.5, -.5, -.5])
+    xmax = list(cfg.get("xmax") or [.5, .5, .5])
+    mid = .5 * (xmin[k] + xmax[k])
+    xmin[k] = mid + (xmin[k] - mid) * 3
+    xmax[k] = mid + (xmax[k] - mid) * 3
+    size[k] *= 3
+    cfg["xmin"], cfg["xmax"], cfg["size"] = xmin, xmax, size
+    bm = [("PERIODIC", "PERIODIC")] * 3
+    bm[k] = ("MIRROR", "MIRROR")
+    _set_bm(cfg, *bm)
+
+
+def _st_rt(desc, gamma):
+    xs = _centres(desc)
+    k = desc.dim - 1
+    xmid = .5 * (desc.xmin[k] + desc.xmax[k])
+    top = xs[k] > xmid
+    force = (0.0, 1.0, 0.0)
+    pot = np.zeros(xs[0].shape)
+    for a in range(desc.dim):
+        pot = pot + (xs[a] - desc.xmin[a]) * force[a]
+    rho = np.where(top, 2.0, 1.0)
+    noise = [.001 * c for c in _crand("Rayleigh-Taylor", xs[0].shape, 3)]
+    st = _euler_state(desc, gamma, rho, pressure=2.5 - rho * pot, vx=noise[0], vy=noise[1], vz=noise[2])
+    return st, pot.reshape(-1), None
+
+
+def _su_selfgrav(cfg):
+    _set_bm(cfg, FREE, FREE, FREE)
+    cfg["useGravity"] = True
+
+
+def _selfgrav_fields(desc, name, sources):
+    """buildSelfGravitationState (res/util.lua:98-134) per cell: the first
+    source (in order) that is nearer than the ones before and contains the
+    cell; else the nearest.  Returns (index of the source, inside, offset
+    from its centre, noise velocities)."""
+    x, y, z = _centres(desc)
+    best = np.full(x.shape, np.inf)
+    which = np.zeros(x.shape, dtype=int)
+    inside = np.zeros(x.shape, dtype=bool)
+    done = np.zeros(x.shape, dtype=bool)
+    for i, (c, r) in enumerate(sources):
+        d2 = (c[0] - x) ** 2 + (c[1] - y) ** 2 + (c[2] - z) ** 2
+        nearer = ~done & (d2 < best)
+        best = np.where(nearer, d2, best)
+        which = np.where(nearer, i, which)
+        hit = nearer & (d2 < r * r)
+        inside |= hit
+        done |= hit
+    cx = np.array([s[0][0] for s in sources])[which]
+    cy = np.array([s[0][1] for s in sources])[which]
+    cz = np.array([s[0][2] for s in sources])[which]
+    dx, dy, dz = x - cx, y - cy, z - cz
+    noise = np.exp(-100 * (dx * dx + dy * dy + dz * dz))
+    vel = [.01 * noise * c for c in _crand(name, x.shape, 3)]
+    return inside, (dx, dy, dz), vel
+
+
+def _st_selfgrav(name, sources, spin=None):
+    def st(desc, gamma):
+        inside, (dx, dy, dz), vel = _selfgrav_fields(desc, name, sources)
+        rho = np.where(inside, 1.0, .1)
+        vx, vy, vz = vel
+        if spin is not None:  # the source's own state inside it
+            vx = np.where(inside, -spin * dy, vx)
+            vy = np.where(inside, spin * dx, vy)
+            vz = np.where(inside, 0.0, vz)
+        return _euler_state(desc, gamma, rho, pressure=1.0, vx=vx, vy=vy, vz=vz), None, None
+    return st
+
+
+def _st_selfgrav2(desc, gamma):
+    # two sources whose inside state is (rho 1, p 1, at rest); the entry then
+    # replaces the momentum by (-5 rho y, 5 rho x) and keeps the energy
+    # (its eight returned values leave the potential and solid channels nil)
+    sources = [((-.25, 0, 0), .1), ((.25, 0, 0), .1)]
+    inside, _, vel = _selfgrav_fields(desc, "self-gravitation test 2", sources)
+    x, y, z = _centres(desc)
+    rho = np.where(inside, 1.0, .1)
+    vx, vy, vz = (np.where(inside, 0.0, v) for v in vel)
+    st = _euler_state(desc, gamma, rho, pressure=1.0, vx=vx, vy=vy, vz=vz).reshape(-1, desc.dim + 2)
+    st[:, 1] = (-5 * rho * y).reshape(-1)
+    if desc.dim > 1:
+        st[:, 2] = (5 * rho * x).reshape(-1)
+    return st.reshape(-1).copy(), None, None
+
+
 _EULER = ("Euler", "MHD", "SRHD")
 
 # name -> (equations, setup(cfg) or None, state(desc, gamma))
@@ -334,17 +473,22 @@ CATALOG = {
     "Marti & Muller 2008 Problem #2": (_EULER, _su_gamma53, _st_mm2),
     "Relativistic Jet": (_EULER, _su_gamma53, _st_jet),
     "Maxwell-1": (("Maxwell",), None, _st_maxwell1),
+    "Flow Around Cylinder": (_EULER, _su_cylinder, _st_cylinder),
+    "Forward Facing Step": (_EULER, _su_step, _st_step),
+    "Spiral Implosion": (_EULER, _su_mirror, _st_spiral),
+    "Rayleigh-Taylor": (_EULER, _su_rt, _st_rt),
+    "self-gravitation test 1": (_EULER, _su_selfgrav, _st_selfgrav("self-gravitation test 1", [((0, 0, 0), .2)])),
+    "self-gravitation test 1 spinning": (_EULER, _su_selfgrav,
+                                         _st_selfgrav("self-gravitation test 1 spinning", [((0, 0, 0), .2)],
+                                                      spin=10.0)),
+    "self-gravitation test 2": (_EULER, _su_selfgrav, _st_selfgrav2),
+    "self-gravitation test 4": (_EULER, _su_selfgrav,
+                                _st_selfgrav("self-gravitation test 4",
+                                             [((.25, .25, 0), .1), ((-.25, .25, 0), .1), ((.25, -.25, 0), .1),
+                                              ((-.25, -.25, 0), .1)])),
 }
 
 UNSUPPORTED = {
-    "Flow Around Cylinder": "solid cells",
-    "Forward Facing Step": "solid cells",
-    "Spiral Implosion": "self-gravity",
-    "self-gravitation test 1": "self-gravity",
-    "self-gravitation test 1 spinning": "self-gravity",
-    "self-gravitation test 2": "self-gravity",
-    "self-gravitation test 4": "self-gravity",
-    "Rayleigh-Taylor": "an external potential",
     "NR Gauge Shock Waves": "the ADM/BSSNOK equations",
     "NR Alcubierre Warp Bubble": "the ADM/BSSNOK equations",
     "NR Schwarzschild Black Hole": "the ADM/BSSNOK equations",
@@ -379,10 +523,22 @@ def state_function(name, gamma=None):
     specific heats as the config holds it (the Lua value, e.g. 5/3); the
     kernels see it rounded to 14 significant digits (config.lua_number), and
     buildStateEuler uses the unrounded value, as here when given."""
+    ch = channels_function(name, gamma)
+
+    def fn(desc):
+        return ch(desc)[0]
+    return fn
+
+
+def channels_function(name, gamma=None):
+    """fn(desc) -> (state, potential or None, solid or None) of entry `name`:
+    the state and the potential and solid channels that
+    SelfGravitationBehavior::Converter reads (one value per cell)."""
     _, _, st = _entry(name)
 
     def fn(desc):
-        return st(desc, float(gamma) if gamma is not None else desc.gamma)
+        out = st(desc, float(gamma) if gamma is not None else desc.gamma)
+        return out if isinstance(out, tuple) else (out, None, None)
     return fn
 
 
@@ -413,6 +569,8 @@ def lua_setups():
                         g.set("defs", defs)
                     for dk, dv in v.items():
                         defs.set(dk, dv)
+                elif k == "useGravity":
+                    g.set("useGravity", bool(v))
                 elif before.get(k) != v:
                     g.set(k, from_py([float(x) for x in v] if k in ("size", "xmin", "xmax") else v))
         return run

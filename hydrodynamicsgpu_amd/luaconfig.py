@@ -15,7 +15,7 @@ calls of `math.*`, `require` (ignored) and `initConds[...].setup(...)` --
 and hands back the globals.  Function definitions are recorded as opaque
 values: a config that defines `initState` or `calcSolid` itself is reported,
 since this build reproduces the catalogue of initial conditions by name
-(initconds.py) and has no solid cells.
+(initconds.py), whose potential and solid channels come with it.
 
 Lua semantics kept: `--` comments with `--[[ ... ]]` / `--[=[ ... ]=]` block
 forms (so `-- [[` toggles a block off, as the reference's config uses it),
@@ -740,6 +740,7 @@ def load(path_or_text, precision="fp64", flags=_config.FLAG_REFERENCE, t_end=0.0
     info = {
         "initCondName": name,
         "initState": initconds.state_function(name, defs.get("idealGas_heatCapacityRatio")) if name else None,
+        "initChannels": initconds.channels_function(name, defs.get("idealGas_heatCapacityRatio")) if name else None,
         "slopeLimiterName": g.get("slopeLimiterName"),
         "maxFrames": g.get("maxFrames"),
         "useGravity": truthy(g.get("useGravity")),
@@ -748,5 +749,6 @@ def load(path_or_text, precision="fp64", flags=_config.FLAG_REFERENCE, t_end=0.0
         "globals": g,
     }
     if info["calcSolid"]:
-        raise LuaError("solid cells (calcSolid) are not supported")
+        raise LuaError("calcSolid (a Lua function of the cell position) is not evaluated here; pass the solid "
+                       "cells as an array (Sim.set_solid / hydro_sim_set_solid)")
     return desc, info

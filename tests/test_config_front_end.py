@@ -141,8 +141,8 @@ def test_app_default_cfl_without_config_line():
 def test_unsupported_configs_are_reported():
     with pytest.raises(luaconfig.LuaError, match="calcSolid"):
         luaconfig.load("size = {32, 32}\nfunction calcSolid(x,y,z) return 0 end\n")
-    with pytest.raises(ValueError, match="self-gravity"):
-        luaconfig.load("size = {32, 32}\ninitConds['Spiral Implosion'].setup()\n")
+    with pytest.raises(ValueError, match="ADM"):
+        luaconfig.load("size = {32, 32}\ninitConds['NR Gauge Shock Waves'].setup()\n")
 
 
 REF_CONFIG = "/root/reference/res/config.lua"
@@ -180,8 +180,59 @@ def test_catalogue_setups_change_the_globals():
     assert all(b == {"min": "PERIODIC", "max": "PERIODIC"} for b in cfg["boundaryMethods"])
     cfg, _ = initconds.setup("Square Cavity", {"size": [64, 64]})
     assert cfg["boundaryMethods"][1] == {"min": "MIRROR", "max": "NONE"}
-    with pytest.raises(ValueError, match="solid"):
-        initconds.setup("Flow Around Cylinder", {"size": [64, 64]})
+    with pytest.raises(ValueError, match="ADM"):
+        initconds.setup("NR Stellar", {"size": [64, 64]})
+
+
+def test_catalogue_solid_potential_and_gravity_channels():
+    # Flow Around Cylinder: NONE on x min, solid disk r < .25 (res/initConds.lua:443-466)
+    cfg, _ = initconds.setup("Flow Around Cylinder", {"size": [64, 64]})
+    assert cfg["boundaryMethods"][0] == {"min": "NONE", "max": "FREEFLOW"}
+    d = config.make_desc(cfg["size"], boundaryMethods=cfg["boundaryMethods"])
+    q, pot, solid = initconds.channels_function("Flow Around Cylinder")(d)
+    x, y, _ = ics.cell_centers(d)[:3]
+    X, Y = np.meshgrid(x, y)
+    np.testing.assert_array_equal(solid.reshape(64, 64), (X * X + Y * Y < .0625).astype(np.uint8))
+    assert pot is None and np.allclose(q.reshape(-1, 4)[:, 1], .2)
+    # Forward Facing Step: solid where x > xmid and y < ymid
+    d = config.make_desc([40, 20])
+    _, _, solid = initconds.channels_function("Forward Facing Step")(d)
+    x, y, _ = ics.cell_centers(d)[:3]
+    X, Y = np.meshgrid(x, y)
+    np.testing.assert_array_equal(solid.reshape(20, 40), ((X > 0) & (Y < 0)).astype(np.uint8))
+    # Rayleigh-Taylor: the last axis tripled (extent and cells), mirror there,
+    # potential y - ymin, pressure 2.5 - rho phi, density 2 above the middle
+    cfg, _ = initconds.setup("Rayleigh-Taylor", {"size": [32, 20]})
+    assert cfg["size"] == [32, 60] and cfg["xmin"][1] == -1.5 and cfg["xmax"][1] == 1.5
+    assert cfg["boundaryMethods"][1] == {"min": "MIRROR", "max": "MIRROR"}
+    d = config.make_desc(cfg["size"], xmin=cfg["xmin"], xmax=cfg["xmax"])
+    q, pot, solid = initconds.channels_function("Rayleigh-Taylor")(d)
+    y = ics.cell_centers(d)[1]
+    Y = np.repeat(y, 32)
+    np.testing.assert_allclose(pot, Y + 1.5, rtol=0, atol=1e-15)
+    rho = q.reshape(-1, 4)[:, 0]
+    np.testing.assert_array_equal(rho, np.where(Y > 0, 2.0, 1.0))
+    v = q.reshape(-1, 4)[:, 1:3] / rho[:, None]
+    assert 0 < np.abs(v).max() <= .001 and solid is None
+    eint = (q.reshape(-1, 4)[:, 3] - .5 * rho * (v * v).sum(1)) / rho
+    np.testing.assert_allclose((d.gamma - 1) * rho * eint, 2.5 - rho * (Y + 1.5), rtol=1e-12)
+    # self-gravitation tests: useGravity, density 1 inside the sources, .1 outside
+    for name in ("self-gravitation test 1", "self-gravitation test 1 spinning", "self-gravitation test 2",
+                 "self-gravitation test 4"):
+        cfg, fn = initconds.setup(name, {"size": [40, 40]})
+        assert cfg["useGravity"] is True
+        d = config.make_desc([40, 40], useGravity=True)
+        q = fn(d).reshape(-1, 4)
+        assert set(np.unique(q[:, 0])) == {0.1, 1.0}, name
+    d = config.make_desc([40, 40])
+    q = initconds.state_function("self-gravitation test 2")(d).reshape(-1, 4)
+    x, y, _ = ics.cell_centers(d)[:3]
+    X, Y = np.meshgrid(x, y)
+    np.testing.assert_allclose(q[:, 1], (-5 * q[:, 0] * Y.ravel()), rtol=1e-15)
+    np.testing.assert_allclose(q[:, 2], (5 * q[:, 0] * X.ravel()), rtol=1e-15)
+    # the Lua front end passes useGravity on
+    desc, info = luaconfig.load("size = {32, 32}\ninitConds['self-gravitation test 4'].setup()\n")
+    assert desc.use_gravity == 1 and info["initChannels"] is not None
 
 
 def test_catalogue_energies():
