@@ -233,6 +233,51 @@ static int run_gpu() {
     };
     compare(app, 5, 1e-10, "MaxwellRoe wave 128 RK4");
   }
+  // the potential and solid channels of initState (the last two of ten,
+  // include/HydroGPU/Solver/SelfGravitationBehavior.h:64-74) reach the
+  // simulation through resetState: a solid disk and a linear potential
+  {
+    HydroGPU::HydroGPUApp app;
+    app.solverName = "EulerRoe";
+    app.integratorName = "RungeKutta2";
+    app.dim = 2;
+    app.size = {{48, 40, 1, 1}};
+    const real gamma = 1.4;
+    auto cyl = [&](real x, real y, real) {
+      std::vector<real> c = euler_cell(1.0, .2, 0.0, 1.0, gamma);
+      c[8] = .1 * y;
+      c[9] = x * x + y * y < .04 ? 1.0 : 0.0;
+      return c;
+    };
+    app.initState = cyl;
+    auto solver = app.createSolver();  // createSolver runs resetState
+    oracle_sim* o = oracle_create(&solver->desc(), 0);
+    std::vector<real> q0 = initial_state(app, *solver);
+    std::vector<real> pot((size_t)solver->getVolume());
+    std::vector<unsigned char> solid(pot.size());
+    int flat = 0;
+    for (int j = 0; j < app.size[1]; ++j)
+      for (int i = 0; i < app.size[0]; ++i, ++flat) {
+        const real x = real(app.xmax[0] - app.xmin[0]) * (i + .5) / app.size[0] + app.xmin[0];
+        const real y = real(app.xmax[1] - app.xmin[1]) * (j + .5) / app.size[1] + app.xmin[1];
+        const std::vector<real> c = cyl(x, y, 0);
+        pot[flat] = c[8];
+        solid[flat] = c[9] != 0;
+      }
+    oracle_set_state(o, q0.data(), q0.size());
+    oracle_set_potential(o, pot.data(), pot.size());
+    oracle_set_solid(o, solid.data(), solid.size());
+    for (int i = 0; i < 4; ++i) {
+      solver->update();
+      oracle_update(o);
+    }
+    std::vector<real> ref(q0.size());
+    oracle_get_state(o, ref.data(), ref.size());
+    const real e = relerr(solver->getState(), ref);
+    CHECK(e < 1e-10, "potential + solid channels: rel err %.3e", e);
+    std::printf("%-34s %4d updates  rel err %.2e\n", "EulerRoe potential + solid 48x40", 4, e);
+    oracle_destroy(o);
+  }
   // the reference's host-side form, Integrator::integrate(dt, callback) with
   // Solver::calcDeriv as the callback (Roe::step, src/Solver/Roe.cpp:94-99),
   // for a Runge-Kutta tableau and for ForwardEuler
