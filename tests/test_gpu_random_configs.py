@@ -167,3 +167,72 @@ def test_random_adjacent_configuration(gpu_lib, case):
         m.update()
     assert _relerr(m.get_state(), g.get_state()) < 1e-4
     assert m.health() == 0
+
+
+# A third draw over solid cells: random masks (ghost cells included) in
+# EulerRoe and EulerBurgers, which read them, and in EulerHLL/HLLC, which
+# ignore them, with or without a potential.  Same checks; FP32-mixed within
+# 2e-4 of FP64 (walls keep sharp jumps in the state).
+SOLID_SOLVERS = ["EulerRoe", "EulerBurgers", "EulerRoe", "EulerBurgers", "EulerHLLC", "EulerHLL"]
+
+
+def _draw_solid(n=48, seed=20261206):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        dim = int(rng.integers(1, 4))
+        pool = {1: SIZES_1D, 2: SIZES_2D, 3: SIZES_3D}[dim]
+        size = [int(rng.choice(pool)) for _ in range(dim)]
+        solver = str(rng.choice(SOLID_SOLVERS))
+        pot = bool(rng.integers(0, 2)) and not (solver == "EulerHLL" and dim == 3)
+        lim = str(rng.choice([l for l in config.LIMITERS if l not in NAN_UNSAFE]))
+        integ = str(rng.choice(config.INTEGRATORS))
+        bm = [(str(rng.choice(BOUNDS[:3])), str(rng.choice(BOUNDS[:3]))) for _ in range(3)]
+        frac = float(rng.choice([0.05, 0.15, 0.3]))
+        kahan = bool(rng.integers(0, 2))
+        out.append((i, size, solver, pot, lim, integ, bm, frac, kahan))
+    return out
+
+
+CASES_SOLID = _draw_solid()
+
+
+@pytest.mark.parametrize("case", CASES_SOLID, ids=["s%02d-%s-%s-%s-%s" % (c[0], "x".join(map(str, c[1])), c[2],
+                                                                          "pot" if c[3] else "nopot", c[5])
+                                                    for c in CASES_SOLID])
+def test_random_solid_configuration(gpu_lib, case):
+    from hydrodynamicsgpu_amd import capi
+
+    i, size, solver, with_pot, lim, integ, bm, frac, kahan = case
+    cfl = 0.02 if integ == "RungeKutta2NonTVD" else 0.3
+    kw = dict(xmin=[0.0] * 3, xmax=[1.0] * 3, solverName=solver, integratorName=integ, slopeLimiter=lim,
+              boundaryMethods=[{"min": a, "max": b} for a, b in bm], cfl=cfl)
+    d = config.make_desc(size, **kw)
+    q = _state(d, solver, 3000 + i)
+    x, y, z, _ = ics.grid(d)
+    pot = (0.2 * x - 0.1 * y + 0.05 * z).ravel() if with_pot else None
+    mask = (np.random.default_rng(4000 + i).random(config.volume(d)) < frac).astype(np.uint8)
+    g, r = capi.Sim(d), capi.Sim(d)
+    o = oracle.OracleSim(d)
+    for s in (g, r, o):
+        s.set_state(q)
+        if with_pot:
+            s.set_potential(pot)
+        s.set_solid(mask)
+    for _ in range(3):
+        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+    r.run(3)
+    assert _relerr(g.get_state(), o.get_state()) < 1e-10
+    np.testing.assert_array_equal(r.get_state(), g.get_state())
+    assert r.time() == g.time()
+    assert g.health() == 0
+    flags = config.FLAG_REFERENCE | (config.FLAG_KAHAN if kahan else 0)
+    m = capi.Sim(config.make_desc(size, precision="fp32_mixed", flags=flags, **kw))
+    m.set_state(q)
+    if with_pot:
+        m.set_potential(pot)
+    m.set_solid(mask)
+    for _ in range(3):
+        m.update()
+    assert _relerr(m.get_state(), g.get_state()) < 2e-4
+    assert m.health() == 0
