@@ -7,7 +7,8 @@ configuration runs three FP64 updates against the oracle (max |gpu -
 oracle| <= 1e-10 * max |oracle|, SRHD 1e-9), run(3) bit-identical to three
 update() calls, and the FP32-mixed form (plain or compensated) within 1e-4
 of the FP64 GPU state.  The draw is part of the test (same seed, same
-cases).
+cases).  Two more draws cover the adjacent solvers (EulerBurgers, a
+potential, useGravity) and solid masks.
 """
 import numpy as np
 import pytest
