@@ -1,5 +1,6 @@
-"""On-disk formats: per-channel FITS images as Solver::save writes them, and
-a raw checkpoint that resumes a run exactly.
+"""On-disk formats: per-channel FITS images as Solver::save writes them, a
+raw checkpoint that resumes a run exactly, and the solid-cell image a config
+names with `solidFilename` (PNG).
 
 FITS (src/Solver/Solver.cpp:547-589): one file per state channel, named
 `<channel name><index>.fits` with the channel names of the equation
@@ -226,3 +227,91 @@ def load_checkpoint(path, device=0, desc=None):
         state, low = state + low.astype(np.float64), None
     sim.restore(state, c["t"], c["steps"], prims=c["prims"], low=low)
     return sim
+
+
+# ---------------------------------------------------------------- solid image
+def read_png(path_or_bytes):
+    """A PNG image as a uint8 array (height, width, channels): 8-bit
+    greyscale, grey+alpha, RGB, RGBA or palette (expanded to RGB),
+    non-interlaced -- the forms an image library reads a mask from.  Rows
+    are un-filtered as the PNG specification defines (None, Sub, Up,
+    Average, Paeth)."""
+    import struct
+    import zlib
+
+    raw = path_or_bytes if isinstance(path_or_bytes, (bytes, bytearray)) else open(path_or_bytes, "rb").read()
+    if raw[:8] != b"\x89PNG\r\n\x1a\n":
+        raise ValueError("not a PNG file")
+    pos, idat, plte, hdr = 8, [], None, None
+    while pos < len(raw):
+        (n,) = struct.unpack(">I", raw[pos:pos + 4])
+        kind, body = raw[pos + 4:pos + 8], raw[pos + 8:pos + 8 + n]
+        pos += 12 + n
+        if kind == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif kind == b"PLTE":
+            plte = np.frombuffer(body, dtype=np.uint8).reshape(-1, 3)
+        elif kind == b"IDAT":
+            idat.append(body)
+        elif kind == b"IEND":
+            break
+    if hdr is None:
+        raise ValueError("PNG without IHDR")
+    w, h, depth, ctype, _, _, interlace = hdr
+    chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}.get(ctype)
+    if depth != 8 or chans is None or interlace != 0:
+        raise ValueError("PNG form not read here (bit depth %d, colour type %d, interlace %d)" % (depth, ctype,
+                                                                                                  interlace))
+    data = zlib.decompress(b"".join(idat))
+    stride = w * chans
+    out = np.zeros((h, stride), dtype=np.uint8)
+    prev = np.zeros(stride, dtype=np.int32)
+    for y in range(h):
+        f = data[y * (stride + 1)]
+        line = np.frombuffer(data, dtype=np.uint8, count=stride, offset=y * (stride + 1) + 1).astype(np.int32)
+        cur = np.zeros(stride, dtype=np.int32)
+        if f == 0:
+            cur = line
+        elif f == 2:
+            cur = (line + prev) & 255
+        elif f in (1, 3, 4):
+            for i in range(stride):
+                a = cur[i - chans] if i >= chans else 0
+                b = prev[i]
+                if f == 1:
+                    p = a
+                elif f == 3:
+                    p = (a + b) >> 1
+                else:
+                    c = prev[i - chans] if i >= chans else 0
+                    pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                    p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+                cur[i] = (line[i] + p) & 255
+        else:
+            raise ValueError("PNG row filter %d" % f)
+        out[y] = cur
+        prev = cur
+    img = out.reshape(h, w, chans)
+    if ctype == 3:
+        if plte is None:
+            raise ValueError("palette PNG without PLTE")
+        img = plte[img[:, :, 0]]
+    return img
+
+
+def solid_from_image(desc, img):
+    """SelfGravitation::resetState's solid mask from an image
+    (src/Solver/SelfGravitation.cpp:56-74): every cell (ghosts included)
+    samples pixel (x * W / size_x, H - 1 - y * H / size_y) of plane
+    z * planes / size_z, channel 0, solid where it exceeds 127.  `img` is
+    (H, W, channels) or (planes, H, W, channels)."""
+    a = np.asarray(img)
+    if a.ndim == 3:
+        a = a[None]
+    planes, H, W = a.shape[0], a.shape[1], a.shape[2]
+    sx, sy, sz = desc.size[0], desc.size[1], desc.size[2]
+    xs = (np.arange(sx) * W) // sx
+    ys = H - 1 - (np.arange(sy) * H) // sy
+    zs = (np.arange(sz) * planes) // sz
+    v = a[zs[:, None, None], ys[None, :, None], xs[None, None, :], 0]
+    return (v > 127).astype(np.uint8).reshape(-1)

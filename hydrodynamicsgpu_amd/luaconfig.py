@@ -23,6 +23,7 @@ forms (so `-- [[` toggles a block off, as the reference's config uses it),
 returning operands, integer-valued numbers printed as Lua does.
 """
 import math
+import os
 import re
 
 from . import config as _config
@@ -702,9 +703,11 @@ def load(path_or_text, precision="fp64", flags=_config.FLAG_REFERENCE, t_end=0.0
     from . import initconds
 
     text = path_or_text
+    base_dir = os.getcwd()
     if "\n" not in path_or_text and path_or_text.endswith(".lua"):
         with open(path_or_text) as f:
             text = f.read()
+        base_dir = os.path.dirname(os.path.abspath(path_or_text))
     g = evaluate(text, initconds.lua_setups())
     size = to_py(g.get("size"))
     if not size:
@@ -741,6 +744,7 @@ def load(path_or_text, precision="fp64", flags=_config.FLAG_REFERENCE, t_end=0.0
         "initCondName": name,
         "initState": initconds.state_function(name, defs.get("idealGas_heatCapacityRatio")) if name else None,
         "initChannels": initconds.channels_function(name, defs.get("idealGas_heatCapacityRatio")) if name else None,
+        "solidFilename": g.get("solidFilename"),
         "slopeLimiterName": g.get("slopeLimiterName"),
         "maxFrames": g.get("maxFrames"),
         "useGravity": truthy(g.get("useGravity")),
@@ -751,4 +755,17 @@ def load(path_or_text, precision="fp64", flags=_config.FLAG_REFERENCE, t_end=0.0
     if info["calcSolid"]:
         raise LuaError("calcSolid (a Lua function of the cell position) is not evaluated here; pass the solid "
                        "cells as an array (Sim.set_solid / hydro_sim_set_solid)")
+    sf = info["solidFilename"]
+    if isinstance(sf, str) and info["initChannels"] is not None:
+        # SelfGravitation::resetState replaces the initial condition's solid
+        # channel by the image (src/Solver/SelfGravitation.cpp:56-74)
+        from . import io as _io
+
+        path = sf if os.path.isabs(sf) else os.path.join(base_dir, sf)
+        channels = info["initChannels"]
+
+        def with_image(d, channels=channels, path=path):
+            q, pot, _ = channels(d)
+            return q, pot, _io.solid_from_image(d, _io.read_png(path))
+        info["initChannels"] = with_image
     return desc, info

@@ -237,3 +237,31 @@ cfl = .4
     for _ in range(4):
         assert g.update() == pytest.approx(o.update(), rel=1e-11)
     assert _relerr(g.get_state(), o.get_state()) < 1e-10
+
+
+def test_config_solid_filename_runs_on_the_hip_path(gpu_lib, tmp_path):
+    # solidFilename (src/Solver/SelfGravitation.cpp:56-74): a PNG written
+    # here, sampled onto the grid, replaces the initial condition's solid
+    # channel; then four updates against the oracle with the same mask
+    from hydrodynamicsgpu_amd import capi
+    from test_io import _png
+
+    img = np.zeros((32, 32, 3), np.uint8)
+    yy, xx = np.mgrid[0:32, 0:32]
+    img[(xx - 12) ** 2 + (yy - 16) ** 2 < 30, 0] = 255
+    (tmp_path / "solid.png").write_bytes(_png(img, filters=(0, 1, 2, 3, 4)))
+    cfg = tmp_path / "config.lua"
+    cfg.write_text("size = {64, 48}\nsolverName = 'EulerRoe'\nintegratorName = 'RungeKutta2'\n"
+                   "initCondName = 'Flow Around Cylinder'\ninitConds[initCondName].setup()\n"
+                   "solidFilename = 'solid.png'\ncfl = .4\n")
+    d, info = luaconfig.load(str(cfg))
+    q, pot, solid = info["initChannels"](d)
+    assert solid.sum() > 0
+    g = capi.Sim(d)
+    g.reset_state(q, pot, solid)
+    o = oracle.OracleSim(d)
+    o.set_state(q)
+    o.set_solid(solid)
+    for _ in range(4):
+        assert g.update() == pytest.approx(o.update(), rel=1e-11)
+    assert _relerr(g.get_state(), o.get_state()) < 1e-10

@@ -111,3 +111,96 @@ def test_checkpoint_shared_with_cpp(tmp_path):
     assert float(f[f.index("t") + 1]) == 2.5 and f[f.index("steps") + 1] == "11"
     assert float(f[f.index("sum") + 1]) == pytest.approx(q.sum(), abs=1e-12)
     assert f[f.index("prims") + 1] == "0" and f[f.index("low") + 1] == "0"
+
+
+# ---------------------------------------------------------------- solid image
+def _png(img, filters=(0,)):
+    """A PNG of `img` (H, W, C uint8, C = 1, 2, 3 or 4), row r filtered with
+    filters[r % len(filters)] as the PNG specification defines them."""
+    import struct
+    import zlib
+
+    h, w, ch = img.shape
+    ctype = {1: 0, 2: 4, 3: 2, 4: 6}[ch]
+    rows = img.reshape(h, w * ch).astype(np.int32)
+    out = bytearray()
+    prev = np.zeros(w * ch, np.int32)
+    for y in range(h):
+        f = filters[y % len(filters)]
+        cur = rows[y]
+        left = np.concatenate([np.zeros(ch, np.int32), cur[:-ch]])
+        upleft = np.concatenate([np.zeros(ch, np.int32), prev[:-ch]])
+        if f == 0:
+            pred = np.zeros_like(cur)
+        elif f == 1:
+            pred = left
+        elif f == 2:
+            pred = prev
+        elif f == 3:
+            pred = (left + prev) >> 1
+        else:
+            pa, pb, pc = np.abs(prev - upleft), np.abs(left - upleft), np.abs(left + prev - 2 * upleft)
+            pred = np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, prev, upleft))
+        out.append(f)
+        out += ((cur - pred) & 255).astype(np.uint8).tobytes()
+        prev = cur
+
+    def chunk(kind, body):
+        return struct.pack(">I", len(body)) + kind + body + struct.pack(">I", zlib.crc32(kind + body) & 0xFFFFFFFF)
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, ctype, 0, 0, 0))
+            + chunk(b"IDAT", zlib.compress(bytes(out))) + chunk(b"IEND", b""))
+
+
+@pytest.mark.parametrize("ch", [1, 2, 3, 4])
+def test_png_reader_undoes_every_row_filter(ch):
+    rng = np.random.default_rng(ch)
+    img = rng.integers(0, 256, (23, 17, ch), dtype=np.uint8)
+    np.testing.assert_array_equal(io.read_png(_png(img, filters=(0, 1, 2, 3, 4))), img)
+
+
+def test_solid_mask_from_image_follows_the_reference_sampling():
+    # src/Solver/SelfGravitation.cpp:60-71: pixel (x W / sx, H - 1 - y H / sy),
+    # channel 0 above 127; a 2-pixel-wide bar and a single bright pixel
+    H, W = 10, 16
+    img = np.zeros((H, W, 3), np.uint8)
+    img[:, 4:6, 0] = 200          # bar in columns 4..5
+    img[0, 15, 0] = 128           # top-right pixel (the grid's top row after the flip)
+    img[:, :, 1] = 255            # other channels do not count
+    d = config.make_desc([32, 20])
+    m = io.solid_from_image(d, io.read_png(_png(img))).reshape(20, 32)
+    for y in range(20):
+        for x in range(32):
+            px, py = x * W // 32, H - 1 - y * H // 20
+            assert m[y, x] == (1 if img[py, px, 0] > 127 else 0), (x, y)
+    # the bar covers grid columns 8..11; the pixel grid rows 18..19 x columns 30..31
+    assert m[:, 8:12].all() and m[18:, 30:].all() and m.sum() == 20 * 4 + 4
+
+
+def test_config_solid_filename_replaces_the_solid_channel(tmp_path):
+    from hydrodynamicsgpu_amd import luaconfig
+
+    img = np.zeros((8, 8, 3), np.uint8)
+    img[2:6, 2:6, 0] = 255
+    (tmp_path / "mask.png").write_bytes(_png(img))
+    cfg = tmp_path / "config.lua"
+    cfg.write_text("size = {24, 16}\nsolverName = 'EulerRoe'\ninitCondName = 'Sod'\n"
+                   "initConds[initCondName].setup()\nsolidFilename = 'mask.png'\n")
+    desc, info = luaconfig.load(str(cfg))
+    q, pot, solid = info["initChannels"](desc)
+    np.testing.assert_array_equal(solid, io.solid_from_image(desc, img))
+    assert solid.sum() == (24 // 2) * (16 // 2)
+
+
+REF_SOLID = "/root/reference/res/test-solid.png"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SOLID), reason="reference sources not present")
+def test_reference_solid_image_decodes():
+    # the image res/config.lua names in its commented solidFilename line: a
+    # 256x256 RGB PNG; the mask is its bright pixels
+    img = io.read_png(REF_SOLID)
+    assert img.shape == (256, 256, 3)
+    frac = (img[:, :, 0] > 127).mean()
+    assert 0.0 < frac < 0.5
+    m = io.solid_from_image(config.make_desc([256, 256]), img).reshape(256, 256)
+    np.testing.assert_array_equal(m, (img[::-1, :, 0] > 127).astype(np.uint8))
