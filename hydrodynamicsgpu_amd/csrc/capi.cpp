@@ -921,7 +921,7 @@ constexpr int kRun1dChunk = 256;
 
 bool run1d_eligible(const hydro_sim* s) {
   if (!s->use_run1d || s->profiling || s->dim != 1 || s->d.equation != HYDRO_EQN_EULER) return false;
-  if (s->pot_on || s->solid_on) return false;
+  if (s->pot_on || (s->solid_on && s->d.solver == HYDRO_SOLVER_ROE)) return false;  // HLL/HLLC ignore solids
   if (s->d.solver == HYDRO_SOLVER_BURGERS) return false;
   if (!boundary_covers_ghosts(s) || s->tab.order > kRun1dMaxStages) return false;
   const int strips = (s->sz[0] - 4 + 59) / 60;  // 60 output columns per strip
