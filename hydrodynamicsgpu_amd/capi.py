@@ -60,6 +60,7 @@ def lib():
         L.hydro_sim_set_potential.argtypes = [vp, dp, sz]
         L.hydro_sim_get_potential.argtypes = [vp, dp, sz]
         L.hydro_sim_set_solid.argtypes = [vp, ctypes.c_void_p, sz]
+        L.hydro_sim_get_solid.argtypes = [vp, ctypes.c_void_p, sz]
         L.hydro_sim_stream.argtypes = [vp]
         L.hydro_sim_stream.restype = vp
         L.hydro_sim_element_size.argtypes = [vp]
@@ -179,6 +180,12 @@ class Sim:
         else:
             a = np.ascontiguousarray(np.asarray(solid).ravel() != 0, dtype=np.uint8)
             check(lib().hydro_sim_set_solid(self._h, a.ctypes.data, a.size))
+
+    def get_solid(self):
+        """The solid flag of every cell (uint8, 0 or 1)."""
+        out = np.zeros(self.volume, dtype=np.uint8)
+        check(lib().hydro_sim_get_solid(self._h, out.ctypes.data, out.size))
+        return out
 
     def get_potential(self):
         out = np.empty(self.volume)
@@ -332,7 +339,7 @@ EXPORTED_SYMBOLS = [
     "hydro_sim_boundary", "hydro_sim_calc_timestep", "hydro_sim_step", "hydro_sim_update",
     "hydro_sim_run", "hydro_sim_sync", "hydro_sim_time", "hydro_sim_totals", "hydro_sim_health",
     "hydro_sim_stream", "hydro_sim_set_profiling", "hydro_sim_stage_timing", "hydro_sim_phase_timing",
-    "hydro_sim_set_potential", "hydro_sim_get_potential", "hydro_sim_set_solid",
+    "hydro_sim_set_potential", "hydro_sim_get_potential", "hydro_sim_set_solid", "hydro_sim_get_solid",
     "hydro_sim_element_size", "hydro_integrator_tableau", "hydro_sim_buffer_create", "hydro_sim_buffer_destroy",
     "hydro_sim_state_buffer", "hydro_sim_init_step", "hydro_sim_calc_deriv", "hydro_sim_buffer_copy",
     "hydro_sim_buffer_zero", "hydro_sim_mult_add", "hydro_sim_finish_step",

@@ -1704,6 +1704,29 @@ int hydro_sim_get_potential(hydro_sim* s, double* pot, size_t n) {
   return HYDRO_OK;
 }
 
+// Converter::fromGPU's solidBuffer read-back (include/HydroGPU/Solver/
+// SelfGravitationBehavior.h:83-88): bit 0 of the mask plane per cell
+int hydro_sim_get_solid(hydro_sim* s, unsigned char* solid, size_t n) {
+  if (!s || !solid) return fail(HYDRO_E_INVALID, "null argument");
+  if (s->d.equation != HYDRO_EQN_EULER) return fail(HYDRO_E_UNSUPPORTED, "solid cells belong to the Euler solvers");
+  if (n != (size_t)s->vol) return fail(HYDRO_E_INVALID, "solid mask size must be the volume");
+  if (!s->solid_on) {
+    std::fill(solid, solid + n, (unsigned char)0);
+    return HYDRO_OK;
+  }
+  std::vector<double> m(n);
+  if (s->mixed) {
+    std::vector<float> f(n);
+    if (copy_sync(f.data(), s->solid, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+      return fail(HYDRO_E_HIP, "copy (solid mask)");
+    std::copy(f.begin(), f.end(), m.begin());
+  } else if (copy_sync(m.data(), s->solid, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream) != hipSuccess) {
+    return fail(HYDRO_E_HIP, "copy (solid mask)");
+  }
+  for (size_t i = 0; i < n; ++i) solid[i] = (unsigned char)(((int)m[i]) & 1);
+  return HYDRO_OK;
+}
+
 int hydro_sim_get_primitives(hydro_sim* s, double* aos, size_t n) {
   if (!s || !aos) return fail(HYDRO_E_INVALID, "null argument");
   if (!s->prim) return fail(HYDRO_E_UNSUPPORTED, "primitives exist for SRHD only");

@@ -221,6 +221,10 @@ int hydro_sim_get_potential(hydro_sim* sim, double* pot, size_t n);
  * 372-400, 425-447).  EulerHLL/HLLC accept the mask and ignore it, as their
  * kernels do.  Euler solvers only; the mask stays until replaced. */
 int hydro_sim_set_solid(hydro_sim* sim, const unsigned char* solid, size_t n);
+/* The solid flags (one per cell, 0 or 1; all 0 without a mask): the
+ * solidBuffer read-back of SelfGravitationBehavior::Converter::fromGPU
+ * (include/HydroGPU/Solver/SelfGravitationBehavior.h:83-88). */
+int hydro_sim_get_solid(hydro_sim* sim, unsigned char* solid, size_t n);
 
 /* SRHD primitives (rho, v, eInt) in the reference layout (src/Solver/SRHDRoe.cpp:9-13).
  * They are stored in the state's precision: FP64, or FP32 for the FP32-mixed

@@ -197,6 +197,20 @@ def test_fp32_mixed_tracks_fp64(gpu_lib, solver, size, kahan):
     assert _relerr(b.get_state(), a.get_state()) < 5e-4
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_solid_mask_reads_back(gpu_lib, precision):
+    from hydrodynamicsgpu_amd import capi
+
+    d = _desc([40, 36], "EulerRoe", precision=precision)
+    g = capi.Sim(d)
+    assert not g.get_solid().any()
+    m = _mask(d, 17)
+    g.set_solid(m * 3)  # any nonzero value is solid
+    np.testing.assert_array_equal(g.get_solid(), (m != 0).astype(np.uint8))
+    g.set_solid(None)
+    assert not g.get_solid().any()
+
+
 def test_refusals(gpu_lib):
     from hydrodynamicsgpu_amd import capi
 
