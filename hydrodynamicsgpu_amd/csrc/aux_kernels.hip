@@ -346,7 +346,11 @@ __global__ void __launch_bounds__(256) step_head_kernel(HeadArgs h, S* planes, f
     if (prim) boundary_cell<S>(h.b, prim, nullptr, g);
     return;
   }
-  const int mb = (int)blockIdx.x;
+  // XCD-aware order (mblocks is a multiple of 8): consecutive logical blocks
+  // -- neighbouring strips, whose halo columns and rows overlap -- run on
+  // the same XCD and share its L2
+  const int per = h.mblocks >> 3;
+  const int mb = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
   const int wave = __builtin_amdgcn_readfirstlane(mb * 4 + (int)(threadIdx.x >> 6));
   double m = INFINITY;
   if (wave < h.a.nwaves) {

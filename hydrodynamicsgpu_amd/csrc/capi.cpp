@@ -1030,7 +1030,7 @@ int enqueue_head(hydro_sim* s) {
   HeadArgs h{};
   h.b = boundary_args(s);
   h.a = cfl_path(s->cg) == 1 ? cfl_march_args(s->cg, s->blockmin) : march3_args(s->cg, s->blockmin);
-  h.mblocks = (h.a.nwaves + 3) / 4;
+  h.mblocks = (((h.a.nwaves + 3) / 4) + 7) & ~7;  // a multiple of 8 for the XCD-aware order
   long long ghosts = 4;
   if (s->dim == 2) ghosts = 4LL * s->sz[0] + 4LL * (s->sz[1] - 4);
   if (s->dim == 3)
