@@ -11,11 +11,14 @@ arithmetic as the rows of a 32768 x 48 grid with the same x-profile and the
 same cell size (which the oracle runs in a second), and every row between them as that grid's middle row.
 Checks:
   * the rows between the first and last 24 bit-identical;
-  * the dt of every update equal to the oracle's (relative 1e-12);
+  * the dt of every update equal to the oracle's (relative 1e-12; FP32-mixed
+    1e-6, its CFL reads the FP32 state);
   * the first 24 rows, the last 24 rows and a middle row against the oracle's
     rows: FP64 relative 1e-11, FP32-mixed relative 2e-4 (the tolerances of the
     full-size parity tests).
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -35,6 +38,10 @@ def _desc(sy, precision):
                             slopeLimiter="MonotizedCentral",
                             boundaryMethods=[{"min": "PERIODIC", "max": "PERIODIC"}] * 2, cfl=0.5,
                             precision=precision)
+
+
+def _dt_tol(precision):
+    return 1e-12 if precision == "fp64" else 1e-6
 
 
 def _narrow_state(d):
@@ -66,7 +73,7 @@ def test_large_grid_rows_match_oracle(gpu_lib, precision, tol):
     g.set_state(big)
     del big
     for _ in range(2):
-        assert g.update() == pytest.approx(o.update(), rel=1e-12)
+        assert g.update() == pytest.approx(o.update(), rel=_dt_tol(precision))
     assert g.health() == 0
     q = g.get_state().reshape(SY, n * SX)
     del g
@@ -76,3 +83,65 @@ def test_large_grid_rows_match_oracle(gpu_lib, precision, tol):
     assert np.max(np.abs(q[:K] - ref[:K])) / scale < tol
     assert np.max(np.abs(q[SY - K:] - ref[K:])) / scale < tol
     assert np.max(np.abs(q[SY // 2] - ref[K])) / scale < tol
+
+
+# 3D: 4096 x 256 x 256 (268M cells), 3D HLLC with TVD-RK3 as in the 3D blast
+# workload, a state uniform in y and z (periodic there, mirror in x); the
+# oracle runs 4096 x 48 x 48 with the same cell size.  Cell (x, y, z) of the
+# large grid goes through the arithmetic of cell (x, m(y), m(z)) of the
+# oracle's grid, m(i) = i in the first K rows, i - S + 2K in the last K, K
+# between them (first-order fluxes over 2 updates x 3 stages reach 6 rows).
+SX3, S3 = 4096, 256
+
+
+def _desc3(s, precision, sx=SX3):
+    h = s / 512.0  # dy = dz = 1/256
+    return config.make_desc([sx, s, s], xmin=[-0.5, -h, -h], xmax=[0.5, h, h], solverName="EulerHLLC",
+                            integratorName="RungeKutta3TVD", slopeLimiter="MinMod",
+                            boundaryMethods=[{"min": "MIRROR", "max": "MIRROR"},
+                                             {"min": "PERIODIC", "max": "PERIODIC"},
+                                             {"min": "PERIODIC", "max": "PERIODIC"}],
+                            cfl=0.5, precision=precision)
+
+
+def _edge_map(s):
+    i = np.arange(s)
+    return np.where(i < K, i, np.where(i >= s - K, i - s + 2 * K, K))
+
+
+@functools.lru_cache(maxsize=1)
+def _oracle3():
+    # the oracle's grid, its two dts and its state (shared by both precisions)
+    dn = _desc3(NY, "fp64")
+    qn = _narrow_state(dn)
+    o = oracle.OracleSim(dn)
+    o.set_state(qn)
+    dts = [o.update() for _ in range(2)]
+    return qn, dts, o.get_state()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32_mixed", 2e-4)])
+def test_large_grid_3d_hllc_matches_oracle(gpu_lib, precision, tol):
+    from hydrodynamicsgpu_amd import capi
+
+    qn, dts, qo = _oracle3()
+    n = qn.size // (SX3 * NY * NY)
+    row = qn[: n * SX3]
+
+    print(f"large grid {SX3}x{S3}x{S3} HLLC {precision}", flush=True)
+    g = capi.Sim(_desc3(S3, precision))
+    big = np.empty(S3 * S3 * row.size)
+    big.reshape(S3 * S3, -1)[:] = row
+    g.set_state(big)
+    del big
+    for dt in dts:
+        assert g.update() == pytest.approx(dt, rel=_dt_tol(precision))
+    assert g.health() == 0
+    q = g.get_state().reshape(S3, S3, n * SX3)
+    del g
+    assert (q[K:S3 - K, K:S3 - K] == q[K, K]).all(), "middle rows of a y/z-uniform state differ"
+    ref = qo.reshape(NY, NY, n * SX3)
+    m = _edge_map(S3)
+    scale = np.max(np.abs(ref))
+    for z in range(S3):  # row blocks of one plane at a time
+        assert np.max(np.abs(q[z] - ref[m[z]][m])) / scale < tol, f"plane {z}"
