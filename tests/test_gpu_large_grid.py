@@ -1,4 +1,5 @@
-"""A large grid on the GPU: 32768 x 8192 cells (ghosts included, 268M cells),
+"""A large grid on the GPU: 32768 x 8192 cells (ghosts included, 268M cells;
+Euler Roe and SRHD Roe),
 whose state planes lie beyond 4 GB of their buffer's base and whose CFL has
 more row chunks than the strip march takes.
 
@@ -31,12 +32,17 @@ SX, SY = 32768, 8192
 NY, K = 48, 24  # rows of the oracle's grid; rows of each edge region
 
 
-def _desc(sy, precision):
+def _desc(sy, precision, family="euler", sx=SX):
     # dy = 1/8192 on both grids: the y-extent grows with the row count
     h = sy / 16384.0
-    return config.make_desc([SX, sy], xmin=[-0.5, -h], xmax=[0.5, h], integratorName="RungeKutta2",
-                            slopeLimiter="MonotizedCentral",
-                            boundaryMethods=[{"min": "PERIODIC", "max": "PERIODIC"}] * 2, cfl=0.5,
+    per = {"min": "PERIODIC", "max": "PERIODIC"}
+    if family == "srhd":
+        return config.make_desc([sx, sy], xmin=[-0.5, -h], xmax=[0.5, h], solverName="SRHDRoe",
+                                integratorName="RungeKutta2", slopeLimiter="Superbee",
+                                boundaryMethods=[{"min": "FREEFLOW", "max": "FREEFLOW"}, per], cfl=0.5,
+                                precision=precision, defs={"idealGas_heatCapacityRatio": 5.0 / 3.0})
+    return config.make_desc([sx, sy], xmin=[-0.5, -h], xmax=[0.5, h], integratorName="RungeKutta2",
+                            slopeLimiter="MonotizedCentral", boundaryMethods=[per] * 2, cfl=0.5,
                             precision=precision)
 
 
@@ -44,29 +50,33 @@ def _dt_tol(precision):
     return 1e-12 if precision == "fp64" else 1e-6
 
 
-def _narrow_state(d):
+def _narrow_state(d, vmax=0.2):
     x = ics.grid(d)[0]
     rho = 1.0 + 0.3 * np.sin(2 * np.pi * x) + 0.5 * (x > 0.1)
-    vx = 0.2 * np.cos(2 * np.pi * x)
+    vx = vmax * np.cos(2 * np.pi * x)
     p = 1.0 + 0.2 * np.sin(4 * np.pi * x) + 0.4 * (x < -0.2)
     z = np.zeros(x.shape)
     return ics.build_state_euler(d, rho, vx, z, z, p)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32_mixed", 2e-4)])
-def test_large_grid_rows_match_oracle(gpu_lib, precision, tol):
+# SRHD: velocities up to 0.8 (W up to 1.67), FREEFLOW in x; tolerances of the
+# full-size SRHD parity test (FP64 1e-9; FP32-mixed 1e-3)
+@pytest.mark.parametrize("family,precision,tol", [("euler", "fp64", 1e-11), ("euler", "fp32_mixed", 2e-4),
+                                                  ("srhd", "fp64", 1e-9), ("srhd", "fp32_mixed", 1e-3)])
+def test_large_grid_rows_match_oracle(gpu_lib, family, precision, tol):
     from hydrodynamicsgpu_amd import capi
 
-    dn = _desc(NY, precision)
-    qn = _narrow_state(dn)
+    vmax = 0.8 if family == "srhd" else 0.2
+    dn = _desc(NY, "fp64", family)
+    qn = _narrow_state(dn, vmax)
     n = qn.size // (SX * NY)  # states per cell
     row = qn[: n * SX]
     assert np.array_equal(qn.reshape(NY, n * SX), np.broadcast_to(row, (NY, n * SX)))  # y-uniform
-    o = oracle.OracleSim(_desc(NY, "fp64"))
+    o = oracle.OracleSim(dn)
     o.set_state(qn)
 
-    d = _desc(SY, precision)
-    print(f"large grid {SX}x{SY} {precision}", flush=True)
+    d = _desc(SY, precision, family)
+    print(f"large grid {SX}x{SY} {family} {precision}", flush=True)
     g = capi.Sim(d)
     big = np.empty(SY * row.size)
     big.reshape(SY, -1)[:] = row
