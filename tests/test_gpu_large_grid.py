@@ -1,5 +1,5 @@
 """A large grid on the GPU: 32768 x 8192 cells (ghosts included, 268M cells;
-Euler Roe and SRHD Roe),
+Euler Roe, SRHD Roe and Maxwell Roe),
 whose state planes lie beyond 4 GB of their buffer's base and whose CFL has
 more row chunks than the strip march takes.
 
@@ -41,6 +41,10 @@ def _desc(sy, precision, family="euler", sx=SX):
                                 integratorName="RungeKutta2", slopeLimiter="Superbee",
                                 boundaryMethods=[{"min": "FREEFLOW", "max": "FREEFLOW"}, per], cfl=0.5,
                                 precision=precision, defs={"idealGas_heatCapacityRatio": 5.0 / 3.0})
+    if family == "maxwell":
+        return config.make_desc([sx, sy], xmin=[-0.5, -h], xmax=[0.5, h], solverName="MaxwellRoe",
+                                integratorName="RungeKutta2", slopeLimiter="Superbee",
+                                boundaryMethods=[per] * 2, cfl=0.5, precision=precision)
     return config.make_desc([sx, sy], xmin=[-0.5, -h], xmax=[0.5, h], integratorName="RungeKutta2",
                             slopeLimiter="MonotizedCentral", boundaryMethods=[per] * 2, cfl=0.5,
                             precision=precision)
@@ -52,6 +56,11 @@ def _dt_tol(precision):
 
 def _narrow_state(d, vmax=0.2):
     x = ics.grid(d)[0]
+    if d.equation == 2:  # Maxwell: E and B fields varying in x
+        eps = d.maxwell_permittivity
+        comps = [0.1 * np.sin(2 * np.pi * x), np.cos(2 * np.pi * x) + (x > 0.1), np.full(x.shape, eps),
+                 0.2 * np.cos(4 * np.pi * x), np.sin(2 * np.pi * x) - (x < -0.2), np.where(x <= 0, 1.0, -1.0)]
+        return np.stack([c.reshape(-1) for c in comps], axis=1).reshape(-1).copy()
     rho = 1.0 + 0.3 * np.sin(2 * np.pi * x) + 0.5 * (x > 0.1)
     vx = vmax * np.cos(2 * np.pi * x)
     p = 1.0 + 0.2 * np.sin(4 * np.pi * x) + 0.4 * (x < -0.2)
@@ -60,9 +69,11 @@ def _narrow_state(d, vmax=0.2):
 
 
 # SRHD: velocities up to 0.8 (W up to 1.67), FREEFLOW in x; tolerances of the
-# full-size SRHD parity test (FP64 1e-9; FP32-mixed 1e-3)
+# full-size SRHD parity test (FP64 1e-9; FP32-mixed 1e-3).  Maxwell: E and B
+# varying in x (FP64 1e-10, as the Maxwell parity tests; FP32-mixed 5e-4)
 @pytest.mark.parametrize("family,precision,tol", [("euler", "fp64", 1e-11), ("euler", "fp32_mixed", 2e-4),
-                                                  ("srhd", "fp64", 1e-9), ("srhd", "fp32_mixed", 1e-3)])
+                                                  ("srhd", "fp64", 1e-9), ("srhd", "fp32_mixed", 1e-3),
+                                                  ("maxwell", "fp64", 1e-10), ("maxwell", "fp32_mixed", 5e-4)])
 def test_large_grid_rows_match_oracle(gpu_lib, family, precision, tol):
     from hydrodynamicsgpu_amd import capi
 
