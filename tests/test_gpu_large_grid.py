@@ -1,5 +1,5 @@
 """A large grid on the GPU: 32768 x 8192 cells (ghosts included, 268M cells;
-Euler Roe, SRHD Roe and Maxwell Roe),
+Euler Roe, SRHD Roe, Maxwell Roe and EulerBurgers),
 whose state planes lie beyond 4 GB of their buffer's base and whose CFL has
 more row chunks than the strip march takes.
 
@@ -41,6 +41,10 @@ def _desc(sy, precision, family="euler", sx=SX):
                                 integratorName="RungeKutta2", slopeLimiter="Superbee",
                                 boundaryMethods=[{"min": "FREEFLOW", "max": "FREEFLOW"}, per], cfl=0.5,
                                 precision=precision, defs={"idealGas_heatCapacityRatio": 5.0 / 3.0})
+    if family == "burgers":
+        return config.make_desc([sx, sy], xmin=[-0.5, -h], xmax=[0.5, h], solverName="EulerBurgers",
+                                integratorName="RungeKutta2", slopeLimiter="MonotizedCentral",
+                                boundaryMethods=[per] * 2, cfl=0.5, precision=precision)
     if family == "maxwell":
         return config.make_desc([sx, sy], xmin=[-0.5, -h], xmax=[0.5, h], solverName="MaxwellRoe",
                                 integratorName="RungeKutta2", slopeLimiter="Superbee",
@@ -70,10 +74,12 @@ def _narrow_state(d, vmax=0.2):
 
 # SRHD: velocities up to 0.8 (W up to 1.67), FREEFLOW in x; tolerances of the
 # full-size SRHD parity test (FP64 1e-9; FP32-mixed 1e-3).  Maxwell: E and B
-# varying in x (FP64 1e-10, as the Maxwell parity tests; FP32-mixed 5e-4)
+# varying in x (FP64 1e-10, as the Maxwell parity tests; FP32-mixed 5e-4).
+# EulerBurgers: its pressure passes see zero y-gradients away from the edges
 @pytest.mark.parametrize("family,precision,tol", [("euler", "fp64", 1e-11), ("euler", "fp32_mixed", 2e-4),
                                                   ("srhd", "fp64", 1e-9), ("srhd", "fp32_mixed", 1e-3),
-                                                  ("maxwell", "fp64", 1e-10), ("maxwell", "fp32_mixed", 5e-4)])
+                                                  ("maxwell", "fp64", 1e-10), ("maxwell", "fp32_mixed", 5e-4),
+                                                  ("burgers", "fp64", 1e-11), ("burgers", "fp32_mixed", 2e-4)])
 def test_large_grid_rows_match_oracle(gpu_lib, family, precision, tol):
     from hydrodynamicsgpu_amd import capi
 
